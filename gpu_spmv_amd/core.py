@@ -1,0 +1,324 @@
+"""Python objects over the C ABI: host CSR matrices, generators, format
+builders, device buffers and plans.  Used by the tests and the benchmark;
+the computation happens in libaxspmv.so."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+# x, b, X vectors use this RNG stream of the configuration's seed (the
+# generators use stream 0 and streams 1..n for rows).
+VEC_STREAM = 1 << 40
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class CsrMatrix:
+    """Host CSR matrix backed by numpy arrays (row/col uint32, val float64)."""
+
+    def __init__(self, row, col, val, ncols: int | None = None):
+        self.row = np.ascontiguousarray(row, dtype=np.uint32)
+        self.col = np.ascontiguousarray(col, dtype=np.uint32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        nrows = self.row.size - 1
+        if ncols is None:
+            ncols = max(nrows, int(self.col.max()) + 1 if self.col.size else 0)
+        self.rl = np.zeros(max(nrows, 1), dtype=np.uint32)
+        self._s = L.Csr(nrows=nrows, ncols=ncols, nnz=self.col.size, owner=0,
+                        row=_ptr(self.row, L.u32), col=_ptr(self.col, L.u32),
+                        val=_ptr(self.val, L.f64), rl=_ptr(self.rl, L.u32))
+
+    @classmethod
+    def _adopt(cls, p) -> "CsrMatrix":
+        """Copy a library-owned CSR into numpy arrays and free it."""
+        s = p.contents
+        n, nnz = s.nrows, s.nnz
+        row = np.ctypeslib.as_array(s.row, shape=(n + 1,)).copy()
+        col = np.ctypeslib.as_array(s.col, shape=(nnz,)).copy() if nnz else np.zeros(0, np.uint32)
+        val = np.ctypeslib.as_array(s.val, shape=(nnz,)).copy() if nnz else np.zeros(0)
+        m = cls(row, col, val, ncols=s.ncols)
+        L.load().axspmv_csr_free(p)
+        m.stats()
+        return m
+
+    @property
+    def nrows(self) -> int:
+        return self._s.nrows
+
+    @property
+    def ncols(self) -> int:
+        return self._s.ncols
+
+    @property
+    def nnz(self) -> int:
+        return self._s.nnz
+
+    @property
+    def ref(self):
+        return C.byref(self._s)
+
+    def check(self) -> None:
+        L.call("axspmv_csr_check", self.ref)
+
+    def stats(self) -> dict:
+        L.call("axspmv_csr_stats", self.ref)
+        s = self._s
+        return dict(nrows=s.nrows, nnz=s.nnz, rmin=s.rmin, rave=s.rave, rmax=s.rmax, rsd=s.rsd, bw=s.bw)
+
+    def transpose(self) -> "CsrMatrix":
+        out = C.POINTER(L.Csr)()
+        L.call("axspmv_csr_transpose", self.ref, C.byref(out))
+        return CsrMatrix._adopt(out)
+
+    def write(self, path: str) -> None:
+        L.call("axspmv_csr_write", self.ref, path.encode())
+
+    @staticmethod
+    def read(path: str) -> "CsrMatrix":
+        out = C.POINTER(L.Csr)()
+        L.call("axspmv_csr_read", path.encode(), C.byref(out))
+        return CsrMatrix._adopt(out)
+
+    def row_ids(self) -> np.ndarray:
+        return np.repeat(np.arange(self.nrows, dtype=np.uint32), np.diff(self.row))
+
+    def to_dense(self) -> np.ndarray:
+        d = np.zeros((self.nrows, self.ncols))
+        d[self.row_ids(), self.col] = self.val
+        return d
+
+
+# ------------------------------------------------------------------ data
+def _gen(name: str, *args) -> CsrMatrix:
+    out = C.POINTER(L.Csr)()
+    L.call(name, *args, C.byref(out))
+    return CsrMatrix._adopt(out)
+
+
+def laplace2d(grid: int, diag: float = 4.0, off: float = -1.0) -> CsrMatrix:
+    return _gen("axspmv_gen_laplace2d", grid, diag, off)
+
+
+def powerlaw(n: int, alpha: float = 2.0, rlmax: int = 2048, mean: float = 20.0, seed: int = 1) -> CsrMatrix:
+    return _gen("axspmv_gen_powerlaw", n, alpha, rlmax, mean, seed)
+
+
+def stencil27(grid: int, diag: float = 26.0, off: float = -1.0, permute: bool = True, seed: int = 2) -> CsrMatrix:
+    return _gen("axspmv_gen_stencil27", grid, diag, off, int(permute), seed)
+
+
+def banded(n: int, hbw: int, fill: float = 0.3, seed: int = 5) -> CsrMatrix:
+    return _gen("axspmv_gen_banded", n, hbw, fill, seed)
+
+
+def random_sparse(n: int, rlmin: int, rlmax: int, seed: int) -> CsrMatrix:
+    return _gen("axspmv_gen_random", n, rlmin, rlmax, seed)
+
+
+def fill_values_normal(m: CsrMatrix, mu: float, sigma: float, seed: int) -> None:
+    L.call("axspmv_fill_values_normal", m.ref, mu, sigma, seed)
+
+
+def uniform(seed: int, n: int, lo: float = -1.0, hi: float = 1.0, stream: int = VEC_STREAM) -> np.ndarray:
+    out = np.empty(n)
+    L.call("axspmv_rng_uniform", seed, stream, lo, hi, _ptr(out, L.f64), n)
+    return out
+
+
+def normal(seed: int, n: int, mu: float = 0.0, sigma: float = 1.0, stream: int = VEC_STREAM) -> np.ndarray:
+    out = np.empty(n)
+    L.call("axspmv_rng_normal", seed, stream, mu, sigma, _ptr(out, L.f64), n)
+    return out
+
+
+def rng_u64(seed: int, stream: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint64)
+    L.call("axspmv_rng_u64", seed, stream, _ptr(out, L.u64), n)
+    return out
+
+
+# -------------------------------------------------------------- builders
+def _arr(p, n, dtype):
+    if not p or n == 0:
+        return None if not p else np.zeros(0, dtype)
+    return np.ctypeslib.as_array(p, shape=(n,)).astype(dtype, copy=True)
+
+
+def _fd(fd: L.FmtData) -> dict:
+    return dict(name=fd.name.decode(), mfp=fd.mfp, beta=fd.beta, ct=fd.ct)
+
+
+def build_k1(m: CsrMatrix, chunk: int = 64, reorder: bool = True) -> dict:
+    out, fd = C.POINTER(L.K1)(), L.FmtData()
+    L.call("axspmv_build_k1", m.ref, chunk, int(reorder), C.byref(out), C.byref(fd))
+    s = out.contents
+    r = dict(chunk=s.chunk, chunkNum=s.chunkNum, lenVC=s.lenVC, fd=_fd(fd),
+             permi=_arr(s.permi, (s.chunkNum + 1) * s.chunk, np.uint32),
+             nmc=_arr(s.nmc, s.chunkNum, np.uint32), chp=_arr(s.chp, s.chunkNum, np.uint32),
+             val=_arr(s.val, s.lenVC, np.float64), col=_arr(s.col, s.lenVC, np.uint32))
+    L.load().axspmv_k1_free(out)
+    return r
+
+
+def build_axc(m: CsrMatrix, x: np.ndarray | None, hbs: int, flags: int = L.BUILD_AX | L.BUILD_PLANES) -> dict:
+    out, fd = C.POINTER(L.Axc)(), L.FmtData()
+    xp = _ptr(np.ascontiguousarray(x, np.float64), L.f64) if x is not None else None
+    L.call("axspmv_build_axc", m.ref, xp, hbs, flags, C.byref(out), C.byref(fd))
+    s = out.contents
+    r = dict(hbs=s.hbs, lenAX=s.lenAX, lenBRP=s.lenBRP, lenMAPX=s.lenMAPX, fd=_fd(fd),
+             ax=_arr(s.ax, s.lenAX, np.float64), brp=_arr(s.brp, s.lenBRP, np.uint32),
+             mapx=_arr(s.mapx, s.lenMAPX, np.uint32), vplane=_arr(s.vplane, s.lenAX // 2, np.float64))
+    L.load().axspmv_axc_free(out)
+    return r
+
+
+def build_axt(m: CsrMatrix, x: np.ndarray | None, bs: int, thw: int, th: int, mode: int,
+              flags: int = L.BUILD_AX | L.BUILD_PLANES) -> dict:
+    out, fd = C.POINTER(L.Axt)(), L.FmtData()
+    xp = _ptr(np.ascontiguousarray(x, np.float64), L.f64) if x is not None else None
+    L.call("axspmv_build_axt", m.ref, xp, bs, thw, th, mode, flags, C.byref(out), C.byref(fd))
+    s = out.contents
+    r = dict(name=s.name.decode(), tileN=s.tileN, lenAX=s.lenAX, lenSEC=s.lenSEC, lenCON=s.lenCON,
+             log=s.log, logTH=s.logTH, fd=_fd(fd),
+             ax=_arr(s.ax, s.lenAX, np.float64), sec=_arr(s.sec, s.lenSEC, np.uint32),
+             con=_arr(s.con, s.lenCON + 1, np.uint32), vplane=_arr(s.vplane, s.lenAX // 2, np.float64),
+             cplane=_arr(s.cplane, s.lenAX // 2, np.uint32))
+    L.load().axspmv_axt_free(out)
+    return r
+
+
+# ---------------------------------------------------------------- device
+def device_count() -> int:
+    n = C.c_int(0)
+    L.call("axspmv_device_count", C.byref(n))
+    return n.value
+
+
+class DeviceBuffer:
+    """A hipMalloc'd buffer owned by Python."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        L.call("axspmv_dev_alloc", self.nbytes, C.byref(p))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        L.call("axspmv_memcpy_h2d", self.ptr, a.ctypes.data, a.nbytes, None)
+
+    def download(self, dtype=np.float64, count: int | None = None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype)
+        L.call("axspmv_memcpy_d2h", out.ctypes.data, self.ptr, out.nbytes, None)
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            L.load().axspmv_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def opts(fmt: int, **kw) -> L.Opts:
+    o = L.Opts()
+    L.call("axspmv_opts_default", fmt, C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown option {k}")
+        setattr(o, k, int(v))
+    return o
+
+
+TIME_EVENTS, TIME_GRAPH, TIME_REGION = 0, 1, 2
+
+
+@dataclass
+class TimingResult:
+    mean_ms: float
+    median_ms: float
+    min_ms: float
+    max_ms: float
+    iters: int
+    mode: int
+
+
+class Plan:
+    """One matrix uploaded in one format (axspmv_plan)."""
+
+    def __init__(self, m: CsrMatrix, fmt: int = L.FMT_AXT, **kw):
+        self.opts = opts(fmt, **kw)
+        p = C.c_void_p()
+        L.call("axspmv_plan_create", m.ref, C.byref(self.opts), None, C.byref(p))
+        self.ptr = p.value
+        self.nrows, self.ncols, self.nnz = m.nrows, m.ncols, m.nnz
+
+    def info(self) -> dict:
+        i = L.PlanInfo()
+        L.call("axspmv_plan_get_info", self.ptr, C.byref(i))
+        return dict(name=i.name.decode(), nrows=i.nrows, ncols=i.ncols, nnz=i.nnz, format=i.format,
+                    slots=i.slots, device_bytes=i.device_bytes, stream_bytes=i.stream_bytes,
+                    alg_bytes=i.alg_bytes, beta=i.beta, launches=i.launches, nblocks=i.nblocks)
+
+    def spmv(self, dx, dy, stream=None) -> None:
+        L.call("axspmv_spmv", self.ptr, _dptr(dx), _dptr(dy), stream)
+
+    def time(self, dx, dy, iters: int = 100, warmup: int = 10, mode: int = TIME_EVENTS,
+             batch: int = 1, stream=None) -> TimingResult:
+        """Kernel time per SpMV (TIME_EVENTS: per-launch events as the
+        reference; TIME_GRAPH: graph replays of `batch` SpMVs; TIME_REGION:
+        one event pair around `iters` back-to-back launches)."""
+        t = L.Timing()
+        L.call("axspmv_time_spmv", self.ptr, _dptr(dx), _dptr(dy), iters, warmup, mode, batch,
+               stream, C.byref(t))
+        return TimingResult(t.mean_ms, t.median_ms, t.min_ms, t.max_ms, t.iters, t.mode)
+
+    def close(self) -> None:
+        if self.ptr:
+            L.load().axspmv_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _dptr(d):
+    if d is None:
+        return None
+    if isinstance(d, DeviceBuffer):
+        return d.ptr
+    if hasattr(d, "data_ptr"):
+        return d.data_ptr()
+    return int(d)
+
+
+def spmv_host(plan: Plan, x: np.ndarray) -> np.ndarray:
+    """Upload x, run one SpMV, download y (test convenience)."""
+    dx = DeviceBuffer.from_array(np.ascontiguousarray(x, np.float64))
+    dy = DeviceBuffer(8 * max(plan.nrows, 1))
+    plan.spmv(dx, dy)
+    L.call("axspmv_device_sync")
+    return dy.download(np.float64, plan.nrows)

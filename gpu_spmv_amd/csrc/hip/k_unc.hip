@@ -1,0 +1,228 @@
+// AXT UNC kernels on row-aligned wave blocks.
+//
+// The reference runs one FP64 atomicAdd per tile (gaxtuh1hw16w/08w/04w,
+// cudaSpmv.cu:1570-1850) or per tile column (gaxtuh, cudaSpmv.cu:1939-1959)
+// into a y it zeroed beforehand.  Here every wave owns a contiguous range of
+// "units" that starts and ends on row boundaries (built on the host), so each
+// row is reduced inside one wave and stored once with a plain store: no
+// atomics, no y zeroing, deterministic results.
+//
+//   unit (th == 1): K consecutive slots of one tile (K divides thw); the lane
+//                   reads K values and K column indices with vector loads and
+//                   gathers K x entries.
+//   unit (th >  1): one tile column g (th slots, one per level); level f of
+//                   64 neighbouring columns is contiguous, so the lanes' loads
+//                   coalesce.
+//
+// A wave walks its units 64 at a time.  Lane sums are combined across lanes
+// by a DPP segmented scan keyed on the row id (sec[]); the last lane of each
+// row segment stores y.  A segment still open at lane 63 is carried to the
+// next 64 units in a scalar register.
+#include "launch.hpp"
+#include "wave64.hpp"
+
+namespace {
+using namespace axspmv::dev;
+using axspmv::launch::Block;
+
+template <class VT, int K>
+struct VecLoad;
+template <>
+struct VecLoad<double, 4> {
+    __device__ static void get(const double *p, double *v) {
+        const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+};
+template <>
+struct VecLoad<double, 2> {
+    __device__ static void get(const double *p, double *v) {
+        const double2 a = *reinterpret_cast<const double2 *>(p);
+        v[0] = a.x; v[1] = a.y;
+    }
+};
+template <>
+struct VecLoad<double, 1> {
+    __device__ static void get(const double *p, double *v) { v[0] = *p; }
+};
+template <>
+struct VecLoad<float, 4> {
+    __device__ static void get(const float *p, float *v) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+};
+template <>
+struct VecLoad<float, 2> {
+    __device__ static void get(const float *p, float *v) {
+        const float2 a = *reinterpret_cast<const float2 *>(p);
+        v[0] = a.x; v[1] = a.y;
+    }
+};
+template <>
+struct VecLoad<float, 1> {
+    __device__ static void get(const float *p, float *v) { v[0] = *p; }
+};
+template <int K>
+__device__ __forceinline__ void load_cols(const uint32_t *p, uint32_t *c) {
+    if constexpr (K == 4) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(p);
+        c[0] = a.x; c[1] = a.y; c[2] = a.z; c[3] = a.w;
+    } else if constexpr (K == 2) {
+        const uint2 a = *reinterpret_cast<const uint2 *>(p);
+        c[0] = a.x; c[1] = a.y;
+    } else {
+        c[0] = *p;
+    }
+}
+
+// th == 1: unit u covers plane slots [u*K, u*K + K), tile (u*K)/thw.
+template <class VT, int K>
+struct UncH1Src {
+    const VT *__restrict__ vp;
+    const uint32_t *__restrict__ cp;
+    const uint32_t *__restrict__ sec;
+    uint32_t ushift;  // log2(thw / K)
+    __device__ __forceinline__ double sum(uint32_t u, const double *__restrict__ x) const {
+        const size_t q = size_t(u) * K;
+        VT v[K];
+        uint32_t c[K];
+        VecLoad<VT, K>::get(vp + q, v);
+        load_cols<K>(cp + q, c);
+        double xv[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) xv[j] = x[c[j]];
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc = fma(double(v[j]), xv[j], acc);
+        return acc;
+    }
+    __device__ __forceinline__ uint32_t row(uint32_t u) const { return sec[u >> ushift]; }
+};
+
+// th > 1: unit g is global tile column g; level f sits at
+// (g/thw)*thw*th + f*thw + g%thw.  TH > 0 fixes th at compile time.
+template <class VT, int TH>
+struct UncHSrc {
+    const VT *__restrict__ vp;
+    const uint32_t *__restrict__ cp;
+    const uint32_t *__restrict__ sec;
+    uint32_t thw_log, th;
+    __device__ __forceinline__ double sum(uint32_t g, const double *__restrict__ x) const {
+        const uint32_t thw = 1u << thw_log;
+        const uint32_t h = TH > 0 ? uint32_t(TH) : th;
+        const size_t base = size_t(g >> thw_log) * (size_t(h) << thw_log) + (g & (thw - 1));
+        double acc = 0.0;
+        if constexpr (TH > 0) {
+            double xv[TH];
+            VT v[TH];
+#pragma unroll
+            for (int f = 0; f < TH; ++f) {
+                v[f] = vp[base + size_t(f) * thw];
+                xv[f] = x[cp[base + size_t(f) * thw]];
+            }
+#pragma unroll
+            for (int f = 0; f < TH; ++f) acc = fma(double(v[f]), xv[f], acc);
+        } else {
+#pragma unroll 4
+            for (uint32_t f = 0; f < h; ++f) acc = fma(double(vp[base + size_t(f) * thw]), x[cp[base + size_t(f) * thw]], acc);
+        }
+        return acc;
+    }
+    __device__ __forceinline__ uint32_t row(uint32_t g) const { return sec[g]; }
+};
+
+template <class Src>
+__global__ void __launch_bounds__(256) rowblock_kernel(Src src, const Block *__restrict__ blocks, uint32_t nblocks,
+                                                       const uint32_t *__restrict__ empty, uint32_t nempty,
+                                                       const double *__restrict__ x, double *__restrict__ y) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = wave * 64 + lane; i < nempty; i += nwaves * 64) y[empty[i]] = 0.0;
+    if (wave >= nblocks) return;
+    const Block b = blocks[wave];
+    uint32_t crow = NOROW;  // row of the segment still open at the end of the previous 64 units
+    double carry = 0.0;
+    for (uint32_t u0 = b.beg; u0 < b.end; u0 += 64) {
+        const uint32_t u = u0 + lane;
+        const bool act = u < b.end;
+        double s = 0.0;
+        uint32_t r = NOROW;
+        if (act) {
+            s = src.sum(u, x);
+            r = src.row(u);
+        }
+        // The carried segment ends here if these units start with another row.
+        const uint32_t r0 = readfirst_u32(r);
+        if (crow != NOROW && r0 != crow) {
+            if (lane == 0) y[crow] = carry;
+            crow = NOROW;
+            carry = 0.0;
+        }
+        uint32_t head = r != shift_up_u32(r, crow);
+        double v = seg_scan_f64(s, head);
+        if (!head) v += carry;  // still inside the carried row
+        const bool last = u0 + 64 >= b.end;
+        bool end = act && shift_down_u32(r, NOROW) != r;
+        if (!last && lane == 63) end = false;
+        if (end) y[r] = v;
+        if (!last) {
+            crow = readlane_u32(r, 63);
+            carry = readlane_f64(v, 63);
+        }
+    }
+}
+
+template <class Src>
+hipError_t run(const Src &src, const axspmv::launch::SegArgs &a, const double *x, double *y, hipStream_t s) {
+    const uint32_t block = 256;
+    uint32_t waves = a.nblocks ? a.nblocks : 1;
+    const unsigned grid = (waves + block / 64 - 1) / (block / 64);
+    rowblock_kernel<Src><<<grid, block, 0, s>>>(src, a.blocks, a.nblocks, a.empty, a.nempty, x, y);
+    return hipGetLastError();
+}
+
+template <class VT>
+hipError_t unc_h1_vt(const axspmv::launch::SegArgs &a, const double *x, double *y, hipStream_t s) {
+    const uint32_t thw_log = __builtin_ctz(a.thw);
+    const VT *vp = static_cast<const VT *>(a.vplane);
+    if (a.thw >= 4) return run(UncH1Src<VT, 4>{vp, a.cplane, a.rowsrc, thw_log - 2}, a, x, y, s);
+    if (a.thw == 2) return run(UncH1Src<VT, 2>{vp, a.cplane, a.rowsrc, 0}, a, x, y, s);
+    return run(UncH1Src<VT, 1>{vp, a.cplane, a.rowsrc, 0}, a, x, y, s);
+}
+
+template <class VT>
+hipError_t unc_h_vt(const axspmv::launch::SegArgs &a, const double *x, double *y, hipStream_t s) {
+    const uint32_t thw_log = __builtin_ctz(a.thw);
+    const VT *vp = static_cast<const VT *>(a.vplane);
+    switch (a.th) {
+#define AX_TH_CASE(T) \
+    case T: return run(UncHSrc<VT, T>{vp, a.cplane, a.rowsrc, thw_log, a.th}, a, x, y, s);
+        AX_TH_CASE(2)
+        AX_TH_CASE(4)
+        AX_TH_CASE(6)
+        AX_TH_CASE(8)
+        AX_TH_CASE(10)
+        AX_TH_CASE(12)
+        AX_TH_CASE(16)
+#undef AX_TH_CASE
+        default: return run(UncHSrc<VT, 0>{vp, a.cplane, a.rowsrc, thw_log, a.th}, a, x, y, s);
+    }
+}
+
+}  // namespace
+
+namespace axspmv {
+namespace launch {
+
+hipError_t axt_unc_h1(const SegArgs &a, const double *x, double *y, hipStream_t s) {
+    return a.vtype == 1 ? unc_h1_vt<float>(a, x, y, s) : unc_h1_vt<double>(a, x, y, s);
+}
+
+hipError_t axt_unc_h(const SegArgs &a, const double *x, double *y, hipStream_t s) {
+    return a.vtype == 1 ? unc_h_vt<float>(a, x, y, s) : unc_h_vt<double>(a, x, y, s);
+}
+
+}  // namespace launch
+}  // namespace axspmv

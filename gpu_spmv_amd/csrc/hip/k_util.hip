@@ -1,0 +1,48 @@
+// Utility kernels: a plain copy (the bandwidth reference used to calibrate
+// the HBM byte counters) and a fill.
+#include "launch.hpp"
+#include "wave64.hpp"
+
+namespace {
+
+__global__ void __launch_bounds__(256) copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n2) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
+}
+
+__global__ void copy_tail(const double *__restrict__ src, double *__restrict__ dst, size_t n) {
+    if (threadIdx.x == 0 && (n & 1)) dst[n - 1] = src[n - 1];
+}
+
+__global__ void __launch_bounds__(256) fill_kernel(double *__restrict__ dst, size_t n, double v) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
+}
+
+unsigned grid_for(size_t work, unsigned block) {
+    size_t g = (work + block - 1) / block;
+    const size_t cap = 256 * 64;  // 256 CUs x 64 workgroups: enough to fill the chip
+    if (g > cap) g = cap;
+    return g ? unsigned(g) : 1u;
+}
+
+}  // namespace
+
+namespace axspmv {
+namespace launch {
+
+hipError_t copy_f64(const double *src, double *dst, size_t n, hipStream_t s) {
+    const size_t n2 = n / 2;
+    if (n2) copy_kernel<<<grid_for(n2, 256), 256, 0, s>>>(reinterpret_cast<const double2 *>(src),
+                                                            reinterpret_cast<double2 *>(dst), n2);
+    if (n & 1) copy_tail<<<1, 64, 0, s>>>(src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s) {
+    if (n) fill_kernel<<<grid_for(n, 256), 256, 0, s>>>(dst, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace launch
+}  // namespace axspmv

@@ -1,0 +1,42 @@
+// Shared host-side helpers: per-thread last-error message and status macros.
+#pragma once
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "axspmv.h"
+
+namespace axspmv {
+
+// Records a formatted message as the calling thread's last error and returns
+// `code`, so call sites read `return fail(AXSPMV_E_INVALID, "...")`.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+// Checked host allocation; returns nullptr (and records the error) on failure.
+template <class T>
+T *host_calloc(size_t n) {
+    if (n == 0) n = 1;
+    T *p = static_cast<T *>(std::calloc(n, sizeof(T)));
+    if (!p) fail(AXSPMV_E_NOMEM, "host allocation of %zu bytes failed", n * sizeof(T));
+    return p;
+}
+
+inline bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
+inline uint32_t ilog2(uint32_t v) { uint32_t r = 0; while (v >>= 1) ++r; return r; }
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Wall-clock seconds (CLOCK_MONOTONIC, as the reference's GT macro).
+double now_s();
+
+}  // namespace axspmv
+
+#define AX_TRY(expr)                        \
+    do {                                    \
+        int ax_rc_ = (expr);                \
+        if (ax_rc_ != AXSPMV_OK) return ax_rc_; \
+    } while (0)

@@ -1,0 +1,159 @@
+// Device helpers and the timing loop.
+#include <algorithm>
+#include <vector>
+
+#include "plan.hpp"
+
+using namespace axspmv;
+
+extern "C" int axspmv_dev_alloc(size_t bytes, void **ptr) {
+    if (!ptr) return fail(AXSPMV_E_INVALID, "dev_alloc: NULL");
+    *ptr = nullptr;
+    const hipError_t e = hipMalloc(ptr, bytes ? bytes : 256);
+    return e == hipSuccess ? AXSPMV_OK : fail(AXSPMV_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+}
+
+extern "C" int axspmv_dev_free(void *ptr) {
+    const hipError_t e = hipFree(ptr);
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "hipFree");
+}
+
+extern "C" int axspmv_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return AXSPMV_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "memcpy h2d");
+}
+
+extern "C" int axspmv_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return AXSPMV_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "memcpy d2h");
+}
+
+extern "C" int axspmv_stream_sync(void *stream) {
+    const hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "stream sync");
+}
+
+extern "C" int axspmv_device_sync(void) {
+    const hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "device sync");
+}
+
+extern "C" int axspmv_device_count(int *count) {
+    if (!count) return fail(AXSPMV_E_INVALID, "device_count: NULL");
+    *count = 0;
+    if (hipGetDeviceCount(count) != hipSuccess) *count = 0;
+    return AXSPMV_OK;
+}
+
+extern "C" int axspmv_copy_f64(const double *d_src, double *d_dst, size_t n, void *stream) {
+    const hipError_t e = launch::copy_f64(d_src, d_dst, n, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "copy launch");
+}
+
+namespace {
+
+struct Events {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+void summarize(std::vector<double> &ms, double scale, axspmv_timing *out) {
+    for (auto &v : ms) v *= scale;
+    double sum = 0.0;
+    for (double v : ms) sum += v;
+    std::vector<double> s = ms;
+    std::sort(s.begin(), s.end());
+    out->mean_ms = sum / double(ms.size());
+    out->median_ms = s.size() % 2 ? s[s.size() / 2] : 0.5 * (s[s.size() / 2 - 1] + s[s.size() / 2]);
+    out->min_ms = s.front();
+    out->max_ms = s.back();
+}
+
+}  // namespace
+
+// Timing loop.  EVENTS: mean of per-launch event times as in the reference
+// (cudaSpmv.cu:563-581), plus the median.  GRAPH: hipGraph replays of
+// `graph_batch` SpMVs each, reported per SpMV.  REGION: one event pair
+// around `iters` back-to-back launches.
+extern "C" int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, uint32_t iters, uint32_t warmup,
+                                uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
+    if (!p || !out || iters == 0 || mode > AXSPMV_TIME_REGION) return fail(AXSPMV_E_INVALID, "time_spmv: bad argument");
+    std::memset(out, 0, sizeof *out);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipStream_t own = nullptr;
+    Events ev;
+    if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess)
+        return fail(AXSPMV_E_HIP, "time_spmv: event creation failed");
+    std::vector<double> ms;
+    ms.reserve(iters);
+    int rc = AXSPMV_OK;
+    out->mode = mode;
+    out->iters = iters;
+    if (mode != AXSPMV_TIME_GRAPH) {
+        for (uint32_t i = 0; i < warmup; ++i)
+            if (plan_launch(p, d_x, d_y, s) != hipSuccess) return fail(AXSPMV_E_HIP, "time_spmv: launch failed");
+        if (mode == AXSPMV_TIME_REGION) {
+            hipError_t e = hipStreamSynchronize(s);
+            if (e == hipSuccess) e = hipEventRecord(ev.a, s);
+            for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) e = plan_launch(p, d_x, d_y, s);
+            if (e == hipSuccess) e = hipEventRecord(ev.b, s);
+            if (e == hipSuccess) e = hipEventSynchronize(ev.b);
+            float t = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, ev.a, ev.b);
+            if (e != hipSuccess) return hip_fail(e, "time_spmv (region)");
+            out->mean_ms = out->median_ms = out->min_ms = out->max_ms = double(t) / iters;
+            return AXSPMV_OK;
+        }
+        for (uint32_t i = 0; i < iters && rc == AXSPMV_OK; ++i) {
+            hipError_t e = hipEventRecord(ev.a, s);
+            if (e == hipSuccess) e = plan_launch(p, d_x, d_y, s);
+            if (e == hipSuccess) e = hipEventRecord(ev.b, s);
+            if (e == hipSuccess) e = hipEventSynchronize(ev.b);
+            float t = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, ev.a, ev.b);
+            if (e != hipSuccess) rc = hip_fail(e, "time_spmv");
+            ms.push_back(t);
+        }
+        if (rc) return rc;
+        summarize(ms, 1.0, out);
+        return AXSPMV_OK;
+    }
+    if (graph_batch == 0) graph_batch = 1;
+    if (!s) {
+        if (hipStreamCreate(&own) != hipSuccess) return fail(AXSPMV_E_HIP, "time_spmv: stream creation failed");
+        s = own;
+    }
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    for (uint32_t i = 0; i < graph_batch && e == hipSuccess; ++i) e = plan_launch(p, d_x, d_y, s);
+    const hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    for (uint32_t i = 0; i < warmup && e == hipSuccess; ++i) e = hipGraphLaunch(ge, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) {
+        e = hipEventRecord(ev.a, s);
+        if (e == hipSuccess) e = hipGraphLaunch(ge, s);
+        if (e == hipSuccess) e = hipEventRecord(ev.b, s);
+        if (e == hipSuccess) e = hipEventSynchronize(ev.b);
+        float t = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, ev.a, ev.b);
+        ms.push_back(t);
+    }
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    if (own) (void)hipStreamDestroy(own);
+    if (e != hipSuccess) return hip_fail(e, "time_spmv (graph)");
+    summarize(ms, 1.0 / graph_batch, out);
+    return AXSPMV_OK;
+}

@@ -1,0 +1,288 @@
+// Device plans: upload a matrix in one format and run its kernel.
+//
+// Replaces the allocate / copy / launch / copy back / free sequence that
+// every test_g*() of the reference repeats per call (cudaSpmv.cu:547-601,
+// 871-929, 1100-1149, 1603-1663, 1963-2037, 2101-2170, 2213-2285): a plan
+// uploads once, and axspmv_spmv() only launches.
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <numeric>
+
+#include "../hip/launch.hpp"
+#include "common.hpp"
+#include "plan.hpp"
+
+using namespace axspmv;
+
+namespace axspmv {
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(AXSPMV_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+}  // namespace axspmv
+
+void *axspmv_plan::dalloc(size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        fail(AXSPMV_E_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+        return nullptr;
+    }
+    bufs.push_back(p);
+    device_bytes += bytes;
+    return p;
+}
+
+int axspmv_plan::upload(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (!bytes) return AXSPMV_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "plan upload");
+}
+
+axspmv_plan::~axspmv_plan() {
+    for (void *p : bufs) (void)hipFree(p);
+}
+
+namespace axspmv {
+
+// Greedy row-aligned wave blocks over units in row order: rows are packed
+// into a block while the block holds at most `budget` units; a row longer
+// than the budget gets a block of its own.  Long blocks go first, so the
+// waves that loop longest start earliest.
+std::vector<launch::Block> row_blocks(const std::vector<uint32_t> &units_per_row, uint32_t budget) {
+    std::vector<launch::Block> small, big;
+    uint64_t pos = 0;
+    uint32_t beg = 0, cur = 0;
+    bool open = false;
+    for (uint32_t u : units_per_row) {
+        if (u == 0) continue;
+        if (u > budget) {
+            if (open) small.push_back({beg, uint32_t(pos)});
+            open = false;
+            big.push_back({uint32_t(pos), uint32_t(pos + u)});
+            pos += u;
+            continue;
+        }
+        if (open && cur + u > budget) {
+            small.push_back({beg, uint32_t(pos)});
+            open = false;
+        }
+        if (!open) {
+            beg = uint32_t(pos);
+            cur = 0;
+            open = true;
+        }
+        cur += u;
+        pos += u;
+    }
+    if (open) small.push_back({beg, uint32_t(pos)});
+    std::stable_sort(big.begin(), big.end(),
+                     [](const launch::Block &a, const launch::Block &b) { return a.end - a.beg > b.end - b.beg; });
+    big.insert(big.end(), small.begin(), small.end());
+    return big;
+}
+
+}  // namespace axspmv
+
+namespace {
+
+int plan_csr(Plan &p, const axspmv_csr *a, hipStream_t s) {
+    const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
+    p.d_val = p.dalloc(vs * a->nnz);
+    p.d_col = static_cast<uint32_t *>(p.dalloc(4ull * a->nnz));
+    p.d_row = static_cast<uint32_t *>(p.dalloc(4ull * (a->nrows + 1)));
+    if (!p.d_val || !p.d_col || !p.d_row) return AXSPMV_E_NOMEM;
+    if (p.o.vtype == AXSPMV_F32) {
+        std::vector<float> v(a->val, a->val + a->nnz);
+        AX_TRY(p.upload(p.d_val, v.data(), 4ull * a->nnz, s));
+    } else {
+        AX_TRY(p.upload(p.d_val, a->val, 8ull * a->nnz, s));
+    }
+    AX_TRY(p.upload(p.d_col, a->col, 4ull * a->nnz, s));
+    AX_TRY(p.upload(p.d_row, a->row, 4ull * (a->nrows + 1), s));
+    p.slots = a->nnz;
+    p.stream_bytes = (vs + 4) * a->nnz + 4ull * (a->nrows + 1);
+    if (p.o.kernel == AXSPMV_KER_STREAM) return fail(AXSPMV_E_UNSUPPORTED, "CSR stream kernel not built");
+    p.kind = Plan::CSR_NAIVE;
+    std::snprintf(p.name, sizeof p.name, "csr_naive");
+    return AXSPMV_OK;
+}
+
+int upload_planes(Plan &p, const double *vplane, const uint32_t *cplane, uint64_t nslots, hipStream_t s) {
+    const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
+    p.d_vplane = p.dalloc(vs * nslots);
+    p.d_cplane = static_cast<uint32_t *>(p.dalloc(4 * nslots));
+    if (!p.d_vplane || !p.d_cplane) return AXSPMV_E_NOMEM;
+    if (p.o.vtype == AXSPMV_F32) {
+        std::vector<float> v(vplane, vplane + nslots);
+        AX_TRY(p.upload(p.d_vplane, v.data(), 4 * nslots, s));
+    } else {
+        AX_TRY(p.upload(p.d_vplane, vplane, 8 * nslots, s));
+    }
+    AX_TRY(p.upload(p.d_cplane, cplane, 4 * nslots, s));
+    p.slots = nslots;
+    return AXSPMV_OK;
+}
+
+int upload_blocks(Plan &p, const std::vector<launch::Block> &blocks, const std::vector<uint32_t> &empty, hipStream_t s) {
+    p.nblocks = uint32_t(blocks.size());
+    p.d_blocks = static_cast<launch::Block *>(p.dalloc(sizeof(launch::Block) * blocks.size()));
+    p.nempty = uint32_t(empty.size());
+    p.d_empty = static_cast<uint32_t *>(p.dalloc(4 * empty.size()));
+    if (!p.d_blocks || !p.d_empty) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(p.d_blocks, blocks.data(), sizeof(launch::Block) * blocks.size(), s));
+    AX_TRY(p.upload(p.d_empty, empty.data(), 4 * empty.size(), s));
+    return AXSPMV_OK;
+}
+
+int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
+    const uint32_t thw = p.o.thw, th = p.o.th;
+    if (!is_pow2(thw) || thw > 64) return fail(AXSPMV_E_UNSUPPORTED, "AXT kernels need thw a power of two <= 64, got %u", thw);
+    if (p.o.mode == AXSPMV_AXT_COM) return fail(AXSPMV_E_UNSUPPORTED, "AXT COM kernels not built");
+    axspmv_axt *m = nullptr;
+    AX_TRY(axspmv_build_axt(a, nullptr, p.o.bs, thw, th, p.o.mode, AXSPMV_BUILD_PLANES, &m, nullptr));
+    std::unique_ptr<axspmv_axt, void (*)(axspmv_axt *)> guard(m, axspmv_axt_free);
+    AX_TRY(upload_planes(p, m->vplane, m->cplane, uint64_t(m->lenAX) / 2, s));
+    p.d_sec = static_cast<uint32_t *>(p.dalloc(4ull * m->lenSEC));
+    if (!p.d_sec) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(p.d_sec, m->sec, 4ull * m->lenSEC, s));
+    const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
+    std::vector<uint32_t> units(a->nrows), empty;
+    const uint32_t K = std::min<uint32_t>(4, thw);
+    for (uint32_t r = 0; r < a->nrows; ++r) {
+        const uint32_t rl = a->row[r + 1] - a->row[r];
+        if (rl == 0) empty.push_back(r);
+        units[r] = th == 1 ? uint32_t(ceil_div(rl, thw) * (thw / K)) : uint32_t(ceil_div(rl, th));
+    }
+    // Budget in units: 4 passes of 64 lanes per wave block by default.
+    const uint32_t budget = p.o.budget ? p.o.budget : 256;
+    AX_TRY(upload_blocks(p, row_blocks(units, budget), empty, s));
+    p.thw = thw;
+    p.th = th;
+    p.stream_bytes = (vs + 4) * p.slots + 4ull * m->lenSEC + 8ull * p.nblocks;
+    if (th == 1) {
+        p.kind = Plan::AXT_UNC_H1;
+        std::snprintf(p.name, sizeof p.name, "axt_unc_h1_thw%u", thw);
+    } else {
+        p.kind = Plan::AXT_UNC_H;
+        std::snprintf(p.name, sizeof p.name, "axt_unc_h%u_thw%u", th, thw);
+    }
+    return AXSPMV_OK;
+}
+
+launch::SegArgs seg_args(const Plan &p, const uint32_t *rowsrc) {
+    launch::SegArgs g{};
+    g.vplane = p.d_vplane;
+    g.cplane = p.d_cplane;
+    g.rowsrc = rowsrc;
+    g.blocks = p.d_blocks;
+    g.nblocks = p.nblocks;
+    g.empty = p.d_empty;
+    g.nempty = p.nempty;
+    g.thw = p.thw;
+    g.th = p.th;
+    g.vtype = p.o.vtype;
+    g.block = p.o.block;
+    return g;
+}
+
+}  // namespace
+
+extern "C" int axspmv_opts_default(uint32_t format, axspmv_opts *o) {
+    if (!o) return fail(AXSPMV_E_INVALID, "opts_default: NULL");
+    std::memset(o, 0, sizeof *o);
+    o->format = format;
+    o->kernel = AXSPMV_KER_DEFAULT;
+    o->mode = AXSPMV_AXT_UNC;
+    o->thw = 4;
+    o->th = 1;
+    o->bs = 256;
+    o->hbs = 64;
+    o->chunk = 64;
+    o->reorder = 1;
+    o->vtype = AXSPMV_F64;
+    o->block = 256;
+    o->budget = 0;
+    if (format > AXSPMV_FMT_AXT) return fail(AXSPMV_E_INVALID, "opts_default: unknown format %u", format);
+    return AXSPMV_OK;
+}
+
+extern "C" int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream, axspmv_plan **out) {
+    if (!a || !o || !out) return fail(AXSPMV_E_INVALID, "plan_create: NULL argument");
+    *out = nullptr;
+    if (!a->row || (a->nnz && (!a->col || !a->val)) || a->row[a->nrows] != a->nnz)
+        return fail(AXSPMV_E_INVALID, "plan_create: inconsistent CSR");
+    if (o->vtype > AXSPMV_F32) return fail(AXSPMV_E_INVALID, "plan_create: unknown value type %u", o->vtype);
+    auto p = std::make_unique<Plan>();
+    p->o = *o;
+    if (p->o.th == 0) p->o.th = 1;
+    if (p->o.thw == 0) p->o.thw = p->o.th == 1 ? 4 : 64;
+    if (p->o.hbs == 0) p->o.hbs = 64;
+    if (p->o.chunk == 0) p->o.chunk = 64;
+    if (p->o.bs == 0) p->o.bs = 256;
+    if (hipGetDevice(&p->device) != hipSuccess) return fail(AXSPMV_E_HIP, "plan_create: no HIP device");
+    p->nrows = a->nrows;
+    p->ncols = a->ncols;
+    p->nnz = a->nnz;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = AXSPMV_OK;
+    switch (o->format) {
+        case AXSPMV_FMT_CSR: rc = plan_csr(*p, a, s); break;
+        case AXSPMV_FMT_AXT: rc = plan_axt(*p, a, s); break;
+        default: rc = fail(AXSPMV_E_UNSUPPORTED, "plan_create: format %u not built", o->format);
+    }
+    if (rc) return rc;
+    const size_t vs = p->o.vtype == AXSPMV_F32 ? 4 : 8;
+    p->alg_bytes = (vs + 4) * uint64_t(a->nnz) + 4ull * (a->nrows + 1) + 8ull * a->ncols + 8ull * a->nrows;
+    p->stream_bytes += 8ull * a->ncols + 8ull * a->nrows;
+    p->beta = p->slots ? double(a->nnz) / double(p->slots) : 0.0;
+    p->launches = 1;
+    *out = p.release();
+    return AXSPMV_OK;
+}
+
+extern "C" int axspmv_plan_destroy(axspmv_plan *p) {
+    delete p;
+    return AXSPMV_OK;
+}
+
+extern "C" int axspmv_plan_get_info(const axspmv_plan *p, axspmv_plan_info *info) {
+    if (!p || !info) return fail(AXSPMV_E_INVALID, "plan_get_info: NULL argument");
+    std::memset(info, 0, sizeof *info);
+    std::snprintf(info->name, sizeof info->name, "%s", p->name);
+    info->nrows = p->nrows;
+    info->ncols = p->ncols;
+    info->nnz = p->nnz;
+    info->format = p->o.format;
+    info->slots = p->slots;
+    info->device_bytes = p->device_bytes;
+    info->stream_bytes = p->stream_bytes;
+    info->alg_bytes = p->alg_bytes;
+    info->beta = p->beta;
+    info->launches = p->launches;
+    info->nblocks = p->nblocks;
+    return AXSPMV_OK;
+}
+
+namespace axspmv {
+
+hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s) {
+    switch (p->kind) {
+        case Plan::CSR_NAIVE:
+            return launch::csr_naive(p->nrows, p->d_val, p->o.vtype, p->d_col, p->d_row, x, y, p->o.block, s);
+        case Plan::AXT_UNC_H1: return launch::axt_unc_h1(seg_args(*p, p->d_sec), x, y, s);
+        case Plan::AXT_UNC_H: return launch::axt_unc_h(seg_args(*p, p->d_sec), x, y, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace axspmv
+
+extern "C" int axspmv_spmv(axspmv_plan *p, const double *d_x, double *d_y, void *stream) {
+    if (!p || (!d_x && p->ncols) || (!d_y && p->nrows)) return fail(AXSPMV_E_INVALID, "spmv: NULL argument");
+    const hipError_t e = plan_launch(p, d_x, d_y, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "spmv launch");
+}

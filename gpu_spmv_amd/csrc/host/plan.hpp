@@ -1,0 +1,43 @@
+// The opaque axspmv_plan: device buffers and launch parameters of one matrix
+// in one format.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <vector>
+
+#include "../hip/launch.hpp"
+#include "common.hpp"
+
+struct axspmv_plan {
+    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H };
+    Kind kind = NONE;
+    int device = 0;
+    axspmv_opts o{};
+    uint32_t nrows = 0, ncols = 0, nnz = 0;
+    char name[64] = "";
+    uint64_t slots = 0, device_bytes = 0, stream_bytes = 0, alg_bytes = 0;
+    double beta = 0.0;
+    uint32_t launches = 1, nblocks = 0, nempty = 0;
+    uint32_t thw = 0, th = 0;
+
+    // CSR arrays
+    void *d_val = nullptr;
+    uint32_t *d_col = nullptr, *d_row = nullptr;
+    // planes + metadata of the tiled formats
+    void *d_vplane = nullptr;
+    uint32_t *d_cplane = nullptr, *d_sec = nullptr, *d_empty = nullptr;
+    axspmv::launch::Block *d_blocks = nullptr;
+
+    std::vector<void *> bufs;
+    void *dalloc(size_t bytes);
+    int upload(void *dst, const void *src, size_t bytes, hipStream_t s);
+    ~axspmv_plan();
+};
+
+namespace axspmv {
+using Plan = axspmv_plan;
+int hip_fail(hipError_t e, const char *what);
+hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s);
+std::vector<launch::Block> row_blocks(const std::vector<uint32_t> &units_per_row, uint32_t budget);
+}  // namespace axspmv

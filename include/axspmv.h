@@ -1,0 +1,296 @@
+/*
+ * axspmv.h -- C ABI of the MI355X (gfx950) sparse matrix-vector library.
+ *
+ * The library reimplements the SpMV path of edoardoCB/gpu_spmv (file
+ * cudaSpmv.cu) for one AMD Instinct MI355X: host format builders for CSR,
+ * K1, AXC and AXT, and hand-written wave64 HIP kernels for y = A*x in each
+ * format.  Every function returns an int status (AXSPMV_OK = 0, negative =
+ * error) and never exits the process; axspmv_last_error() returns a message
+ * for the calling thread's last failure.
+ *
+ * Conventions
+ *  - Indices are uint32 (the reference's UIN), values are IEEE double unless
+ *    a name says f32.  Matrices are square in the reference (x has nrows
+ *    entries); ncols is kept for the transposed and rectangular cases.
+ *  - Device pointers are plain pointers from hipMalloc (or from
+ *    axspmv_dev_alloc); `stream` is a hipStream_t passed as void* (NULL =
+ *    the default stream).  Device calls are asynchronous and stream ordered.
+ *  - Host objects returned through `**out` are owned by the library and are
+ *    released with the matching *_free().
+ *  - A plan (device handle) is bound to the device that was current when it
+ *    was created.  Distinct plans may be used from distinct threads; one plan
+ *    must not be used by two threads at once.
+ */
+#ifndef AXSPMV_H
+#define AXSPMV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AXSPMV_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- status */
+enum axspmv_status {
+    AXSPMV_OK = 0,
+    AXSPMV_E_INVALID = -1,     /* bad argument or inconsistent matrix      */
+    AXSPMV_E_HIP = -2,         /* HIP runtime error                        */
+    AXSPMV_E_NOMEM = -3,       /* host or device allocation failed         */
+    AXSPMV_E_UNSUPPORTED = -4, /* parameter combination not implemented    */
+    AXSPMV_E_OVERFLOW = -5,    /* 32-bit index or header packing overflow  */
+    AXSPMV_E_IO = -6           /* matrix file could not be read / written  */
+};
+
+/* Message for the calling thread's most recent failure ("" if none).
+ * Replaces the print-and-exit of HANDLE_CUDA_ERROR / TEST_POINTER / ABORT
+ * (cudaSpmv.cu:146-148, 189-191, 255-257). */
+const char *axspmv_last_error(void);
+int axspmv_abi_version(void);
+
+/* ------------------------------------------------------------ host CSR */
+/* Host CSR matrix.  Mirrors str_matCSR (cudaSpmv.cu:261): val/col/row plus
+ * the row-length array rl[] and the statistics printMatrixStats fills
+ * (cudaSpmv.cu:332-381).  `owner` is 1 when the library allocated the
+ * arrays (free with axspmv_csr_free), 0 for caller-owned arrays. */
+typedef struct axspmv_csr {
+    uint32_t nrows, ncols, nnz, owner;
+    uint32_t *row;   /* nrows + 1 */
+    uint32_t *col;   /* nnz, sorted within each row */
+    double *val;     /* nnz */
+    uint32_t *rl;    /* nrows (filled by axspmv_csr_stats) */
+    uint32_t rmin, rmax, bw, pad_;
+    double rave, rsd;
+} axspmv_csr;
+
+/* Allocate an empty library-owned CSR with the given sizes (arrays zeroed). */
+int axspmv_csr_alloc(uint32_t nrows, uint32_t ncols, uint32_t nnz, axspmv_csr **out);
+void axspmv_csr_free(axspmv_csr *a);
+
+/* Structural checks: row[0]==0, row nondecreasing, row[nrows]==nnz,
+ * col < ncols, columns strictly increasing within a row. */
+int axspmv_csr_check(const axspmv_csr *a);
+
+/* rl[i] = row[i+1]-row[i] and rmin/rave/rmax/rsd/bw.
+ * Replaces printMatrixStats (cudaSpmv.cu:332-381) minus the file write. */
+int axspmv_csr_stats(axspmv_csr *a);
+
+/* The `.sta` statistics file printMatrixStats writes (cudaSpmv.cu:359-379). */
+int axspmv_csr_write_sta(const axspmv_csr *a, const char *matname, const char *path);
+
+/* Matrix files in the reference's two formats (matrixReading,
+ * cudaSpmv.cu:265-328): ".csr" text "nrows nnz" then nnz lines "val col"
+ * then nrows+1 row offsets; ".bin" = u32 nrows, u32 nnz, f64 val[nnz],
+ * u32 col[nnz], u32 row[nrows+1].  The format follows the file extension. */
+int axspmv_csr_read(const char *path, axspmv_csr **out);
+int axspmv_csr_write(const axspmv_csr *a, const char *path);
+
+/* A^T as a CSR (CSR -> CSC), columns sorted; used for the adjoint. */
+int axspmv_csr_transpose(const axspmv_csr *a, axspmv_csr **out);
+
+/* ------------------------------------------------------------ RNG + data */
+/* xoshiro256** seeded through splitmix64(seed ^ stream * 0x9E3779B97F4A7C15).
+ * u64 stream, U[lo,hi) doubles from (next>>11)*2^-53, and N(mu,sigma) by
+ * Marsaglia's polar method. */
+int axspmv_rng_u64(uint64_t seed, uint64_t stream, uint64_t *out, size_t n);
+int axspmv_rng_uniform(uint64_t seed, uint64_t stream, double lo, double hi,
+                       double *out, size_t n);
+int axspmv_rng_normal(uint64_t seed, uint64_t stream, double mu, double sigma,
+                      double *out, size_t n);
+
+/* Synthetic matrices (no reference counterpart; the benchmark inputs).
+ *  laplace2d : 5-point Laplacian on grid x grid, lexicographic order,
+ *              diagonal `diag`, off-diagonals `off`.
+ *  powerlaw  : n rows; z ~ Zipf(alpha) on {1..rlmax} by exact inverse CDF,
+ *              rl = clamp(round(s*z), 1, rlmax) with s fixed so the mean row
+ *              length is `mean`; columns uniform without replacement, sorted;
+ *              values N(0,1).  Row i draws from stream i+1 of `seed`.
+ *  stencil27 : 27-point stencil on grid^3, diagonal `diag`, off `off`; with
+ *              permute != 0 the symmetric permutation P A P^T of a
+ *              Fisher-Yates shuffle (seed `seed`, stream 0) is applied.
+ *  banded    : n rows, every (i,j) with |i-j| <= hbw is kept with
+ *              probability `fill` (the diagonal too), values N(0,1).
+ *  random    : n rows, rl uniform in [rlmin, rlmax], distinct uniform
+ *              columns, values N(0,1). */
+int axspmv_gen_laplace2d(uint32_t grid, double diag, double off, axspmv_csr **out);
+int axspmv_gen_powerlaw(uint32_t n, double alpha, uint32_t rlmax, double mean,
+                        uint64_t seed, axspmv_csr **out);
+int axspmv_gen_stencil27(uint32_t grid, double diag, double off, int permute,
+                         uint64_t seed, axspmv_csr **out);
+int axspmv_gen_banded(uint32_t n, uint32_t hbw, double fill, uint64_t seed,
+                      axspmv_csr **out);
+int axspmv_gen_random(uint32_t n, uint32_t rlmin, uint32_t rlmax, uint64_t seed,
+                      axspmv_csr **out);
+/* Overwrite the values of `a` with N(mu, sigma) draws (seed, stream 0). */
+int axspmv_fill_values_normal(axspmv_csr *a, double mu, double sigma, uint64_t seed);
+
+/* ------------------------------------------------------- format builders */
+/* str_formatData (cudaSpmv.cu:385): name, memory footprint in bytes,
+ * occupancy beta = nnz / value slots, and the build time in seconds. */
+typedef struct axspmv_fmtdata {
+    char name[48];
+    double mfp, beta, ct;
+} axspmv_fmtdata;
+
+/* Builder flags. */
+#define AXSPMV_BUILD_AX 1u     /* emit the reference's interleaved ax[] (needs x) */
+#define AXSPMV_BUILD_PLANES 2u /* emit the value plane / column plane        */
+
+/* K1 (str_matK1, cudaSpmv.cu:712; builders cudaSpmv.cu:716-846).
+ * chunk = rows per chunk (the reference's CHUNK_SIZE; 64 = one wavefront).
+ * reorder = 1: rows sorted by length, descending, ties by ascending row
+ * (a stable sort; the reference's qsort is glibc's merge sort);
+ * reorder = 0: natural row order, nmc[c] = longest row of the chunk. */
+typedef struct axspmv_k1 {
+    uint32_t nrows, nnz, chunk, chunkNum, lenVC, reorder;
+    uint32_t *permi;  /* (chunkNum+1)*chunk, zero padded */
+    uint32_t *nmc;    /* chunkNum */
+    uint32_t *chp;    /* chunkNum */
+    double *val;      /* lenVC */
+    uint32_t *col;    /* lenVC */
+} axspmv_k1;
+int axspmv_build_k1(const axspmv_csr *a, uint32_t chunk, int reorder,
+                    axspmv_k1 **out, axspmv_fmtdata *fd);
+void axspmv_k1_free(axspmv_k1 *m);
+
+/* AXC (str_matAXC, cudaSpmv.cu:933; builders cudaSpmv.cu:937-1062).
+ * hbs = brick half size (the reference's TILE_HW = 32).  With
+ * AXSPMV_BUILD_AX, ax[] holds bricks of hbs values followed by hbs gathered
+ * x-values (x required); with AXSPMV_BUILD_PLANES, vplane[] holds the value
+ * halves (lenAX/2) and mapx[] is the column plane. */
+typedef struct axspmv_axc {
+    uint32_t nrows, nnz, hbs, lenAX, lenBRP, lenMAPX;
+    double *ax;       /* lenAX (AXSPMV_BUILD_AX) */
+    uint32_t *brp;    /* nrows + 1 */
+    uint32_t *mapx;   /* lenMAPX = lenAX/2 + 8 */
+    double *vplane;   /* lenAX/2 (AXSPMV_BUILD_PLANES) */
+} axspmv_axc;
+int axspmv_build_axc(const axspmv_csr *a, const double *x, uint32_t hbs, uint32_t flags,
+                     axspmv_axc **out, axspmv_fmtdata *fd);
+void axspmv_axc_free(axspmv_axc *m);
+
+/* AXT (str_matAXT, cudaSpmv.cu:1154; builders cudaSpmv.cu:1158-1566).
+ * mode AXSPMV_AXT_UNC: uncompacted, tiles padded per row;
+ * mode AXSPMV_AXT_COM: compacted, packed by nonzero index with segment
+ * headers (row << log) | (segment length - 1).
+ * th = tile height (1 = the "H1" layouts), thw = tile half width,
+ * bs = block size (only COM with th == 1 uses it: headers split at bs).
+ * vplane[]/cplane[] (lenAX/2 each) are the value halves of ax[] and the
+ * column of every slot (0 in padding): x[cplane[q]] reproduces the x half. */
+#define AXSPMV_AXT_UNC 0u
+#define AXSPMV_AXT_COM 1u
+typedef struct axspmv_axt {
+    char name[48];
+    uint32_t nrows, nnz, mode, tileHW, tileH, logTH, tileN;
+    uint32_t lenAX, lenSEC, lenCON, log, bs;
+    double *ax;        /* lenAX (AXSPMV_BUILD_AX) */
+    uint32_t *sec;     /* lenSEC: row per tile (UNC H1), row per column
+                          (UNC H>1) or per-slot headers (COM) */
+    uint32_t *con;     /* lenCON + 1: first ax slot of each row (UNC) */
+    double *vplane;    /* lenAX/2 (AXSPMV_BUILD_PLANES) */
+    uint32_t *cplane;  /* lenAX/2 (AXSPMV_BUILD_PLANES) */
+} axspmv_axt;
+int axspmv_build_axt(const axspmv_csr *a, const double *x, uint32_t bs, uint32_t thw,
+                     uint32_t th, uint32_t mode, uint32_t flags, axspmv_axt **out,
+                     axspmv_fmtdata *fd);
+void axspmv_axt_free(axspmv_axt *m);
+
+/* ------------------------------------------------------- device plans */
+/* Formats and kernels.  A plan uploads one matrix in one format and runs the
+ * kernel chosen by (format, kernel). */
+enum axspmv_format {
+    AXSPMV_FMT_CSR = 0, /* val/col/row                                  */
+    AXSPMV_FMT_K1 = 1,  /* sliced ELL, chunk rows per wavefront          */
+    AXSPMV_FMT_AXC = 2, /* row bricks of hbs slots                       */
+    AXSPMV_FMT_AXT = 3  /* tiled: UNC/COM x th==1 / th>1                 */
+};
+enum axspmv_kernel {
+    AXSPMV_KER_DEFAULT = 0, /* the format's own kernel                  */
+    AXSPMV_KER_NAIVE = 1,   /* CSR: one lane per row (gcsr)              */
+    AXSPMV_KER_STREAM = 2   /* CSR: row-aligned wave blocks, segmented   */
+};
+enum axspmv_vtype { AXSPMV_F64 = 0, AXSPMV_F32 = 1 };
+
+typedef struct axspmv_opts {
+    uint32_t format;  /* enum axspmv_format */
+    uint32_t kernel;  /* enum axspmv_kernel */
+    uint32_t mode;    /* AXT: AXSPMV_AXT_UNC / AXSPMV_AXT_COM */
+    uint32_t thw;     /* AXT tile half width (0 = 64, or 4 for UNC th==1) */
+    uint32_t th;      /* AXT tile height (0 = 1) */
+    uint32_t bs;      /* AXT COM th==1 block size (0 = 256) */
+    uint32_t hbs;     /* AXC brick size (0 = 64) */
+    uint32_t chunk;   /* K1 rows per chunk (0 = 64) */
+    uint32_t reorder; /* K1: 1 = sort rows by length (default), 0 = natural */
+    uint32_t vtype;   /* enum axspmv_vtype: value storage */
+    uint32_t block;   /* threads per workgroup (0 = kernel default) */
+    uint32_t budget;  /* slots per wave block for segmented kernels (0 = auto) */
+    uint32_t flags;   /* AXSPMV_PLAN_* */
+    uint32_t reserved[3];
+} axspmv_opts;
+
+#define AXSPMV_PLAN_TRANSPOSE 1u /* also build A^T (needed by the adjoint) */
+
+/* Fill `o` with the defaults of a format (every field defined). */
+int axspmv_opts_default(uint32_t format, axspmv_opts *o);
+
+typedef struct axspmv_plan axspmv_plan;
+
+/* Build (host) and upload (device) A in the format of `o`.  Replaces the
+ * getFormatData*() + cudaMalloc/cudaMemcpy part of every test_g*()
+ * (cudaSpmv.cu:547-563, 871-897, 1100-1119, 1603-1622, ...). */
+int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream,
+                       axspmv_plan **out);
+int axspmv_plan_destroy(axspmv_plan *p);
+
+typedef struct axspmv_plan_info {
+    char name[64];            /* kernel name, e.g. "axt_unc_h1_thw4"      */
+    uint32_t nrows, ncols, nnz, format;
+    uint64_t slots;           /* value slots incl. padding                */
+    uint64_t device_bytes;    /* bytes of matrix data held on the device  */
+    uint64_t stream_bytes;    /* matrix bytes one SpMV reads (incl. pad)  */
+    uint64_t alg_bytes;       /* algorithmic bytes of one FP64 SpMV:
+                                 (s_v+4)*nnz + 4*(n+1) + 8*ncols + 8*nrows */
+    double beta;              /* nnz / slots                              */
+    uint32_t launches;        /* kernel launches per SpMV                 */
+    uint32_t nblocks;         /* wave blocks (segmented kernels)          */
+} axspmv_plan_info;
+int axspmv_plan_get_info(const axspmv_plan *p, axspmv_plan_info *info);
+
+/* y = A x.  y is fully overwritten (no zeroing needed by the caller). */
+int axspmv_spmv(axspmv_plan *p, const double *d_x, double *d_y, void *stream);
+
+/* ---------------------------------------------------------- timing */
+typedef struct axspmv_timing {
+    double mean_ms, median_ms, min_ms, max_ms; /* per operation */
+    uint32_t iters, mode;                      /* the AXSPMV_TIME_* used */
+} axspmv_timing;
+#define AXSPMV_TIME_EVENTS 0u /* one hipEvent pair + sync per launch (reference) */
+#define AXSPMV_TIME_GRAPH 1u  /* replays of a hipGraph of `graph_batch` SpMVs    */
+#define AXSPMV_TIME_REGION 2u /* one event pair around `iters` back-to-back SpMVs */
+/* Time `iters` SpMVs after `warmup` untimed ones.  AXSPMV_TIME_EVENTS is the
+ * reference's loop (cudaSpmv.cu:563-581): mean and median of per-launch
+ * times.  AXSPMV_TIME_GRAPH reports per-SpMV times of graph replays (for
+ * kernels shorter than the launch latency).  AXSPMV_TIME_REGION reports the
+ * average over one timed region of back-to-back launches (min = max = mean). */
+int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, uint32_t iters,
+                     uint32_t warmup, uint32_t mode, uint32_t graph_batch, void *stream,
+                     axspmv_timing *out);
+
+/* ---------------------------------------------------- device helpers */
+int axspmv_dev_alloc(size_t bytes, void **ptr);
+int axspmv_dev_free(void *ptr);
+int axspmv_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int axspmv_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int axspmv_stream_sync(void *stream);
+int axspmv_device_sync(void);
+/* Number of visible HIP devices (0 if none). */
+int axspmv_device_count(int *count);
+/* Plain device copy dst[i] = src[i] (a bandwidth reference kernel). */
+int axspmv_copy_f64(const double *d_src, double *d_dst, size_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AXSPMV_H */

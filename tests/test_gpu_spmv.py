@@ -1,0 +1,126 @@
+"""GPU parity: every kernel's y against the oracle's ncsr on the same inputs.
+
+Bar (FP64): signed norm-wise relative error max|y - y_ref| / max|y_ref| <=
+1e-12 (summation order and FMA contraction differ from the left-to-right
+CPU sum).  FP32 value storage is compared against ncsr on the same
+float-rounded values at the same 1e-12.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import gpu_spmv_amd as ax
+from conftest import FIXTURES, FixtureMatrix, rel_err
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+KERNELS = [
+    ("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+    ("unc_h1_thw4", ax.FMT_AXT, dict(thw=4, th=1)),
+    ("unc_h1_thw8", ax.FMT_AXT, dict(thw=8, th=1)),
+    ("unc_h1_thw16", ax.FMT_AXT, dict(thw=16, th=1)),
+    ("unc_h1_thw32", ax.FMT_AXT, dict(thw=32, th=1)),
+    ("unc_h1_thw64", ax.FMT_AXT, dict(thw=64, th=1)),
+    ("unc_h1_thw2", ax.FMT_AXT, dict(thw=2, th=1)),
+    ("unc_h4_thw64", ax.FMT_AXT, dict(thw=64, th=4)),
+    ("unc_h8_thw64", ax.FMT_AXT, dict(thw=64, th=8)),
+    ("unc_h6_thw32", ax.FMT_AXT, dict(thw=32, th=6)),
+    ("unc_h12_thw32", ax.FMT_AXT, dict(thw=32, th=12)),
+    ("unc_h5_thw64", ax.FMT_AXT, dict(thw=64, th=5)),
+    ("unc_h1_thw4_small_budget", ax.FMT_AXT, dict(thw=4, th=1, budget=64)),
+]
+
+
+def long_rows():
+    """Rows of 1..5000 entries (several passes of one wave) between short ones."""
+    rng = np.random.default_rng(3)
+    n = 3000
+    rl = rng.integers(0, 6, n)
+    rl[rng.choice(n, 8, replace=False)] = rng.integers(600, 5000, 8)
+    row = np.concatenate([[0], np.cumsum(rl)]).astype(np.uint32)
+    col = np.concatenate([np.sort(rng.choice(n * 2, k, replace=False)) for k in rl]).astype(np.uint32)
+    return ax.CsrMatrix(row, col, rng.standard_normal(col.size), ncols=2 * n)
+
+
+def matrices():
+    for name in FIXTURES:
+        yield name, FixtureMatrix(name).csr()
+    yield "laplace256", ax.laplace2d(256)
+    yield "powerlaw64k", ax.powerlaw(1 << 16, 2.0, 2048, 20.0, seed=1)
+    yield "stencil20p", ax.stencil27(20, permute=True, seed=2)
+    yield "banded_empty", ax.banded(4000, 2, 0.3, seed=6)
+    yield "long_rows", long_rows()
+
+
+MATS = dict(matrices())
+
+
+def test_native_library_loaded():
+    lib = ax._lib.load()
+    assert ax.device_count() >= 1
+    maps = open("/proc/self/maps").read()
+    assert os.path.abspath(ax._lib.LIB_PATH) in maps
+    hip_libs = {line.split()[-1] for line in maps.splitlines() if "libamdhip64" in line}
+    assert len(hip_libs) == 1, hip_libs
+    assert lib.axspmv_abi_version() == 1
+
+
+@pytest.mark.parametrize("mname", list(MATS))
+@pytest.mark.parametrize("kname,fmt,kw", KERNELS, ids=[k[0] for k in KERNELS])
+def test_spmv_f64(mname, kname, fmt, kw):
+    m = MATS[mname]
+    x = ax.uniform(11, m.ncols)
+    ref = orc.ncsr(m, x, nthreads=4)
+    plan = ax.Plan(m, fmt, **kw)
+    y = ax.spmv_host(plan, x)
+    assert rel_err(ref, y) <= TOL, (mname, kname, rel_err(ref, y))
+    # y is fully overwritten: a second call into a dirty buffer gives the same
+    dx = ax.DeviceBuffer.from_array(x)
+    dy = ax.DeviceBuffer.from_array(np.full(max(m.nrows, 1), np.nan))
+    plan.spmv(dx, dy)
+    y2 = dy.download(np.float64, m.nrows)
+    assert np.array_equal(y, y2)
+
+
+@pytest.mark.parametrize("mname", ["rand200", "powerlaw64k", "long_rows"])
+@pytest.mark.parametrize("kname,fmt,kw", [KERNELS[0], KERNELS[1], KERNELS[7]], ids=["csr", "h1", "h4"])
+def test_spmv_f32_values(mname, kname, fmt, kw):
+    m = MATS[mname]
+    x = ax.uniform(12, m.ncols)
+    ref = orc.ncsr_f32val(m, x)
+    y = ax.spmv_host(ax.Plan(m, fmt, vtype=ax.F32, **kw), x)
+    assert rel_err(ref, y) <= TOL
+    # and the stated accuracy against the FP64 product
+    assert rel_err(orc.ncsr(m, x), y) < 1e-6
+
+
+def test_powerlaw_config2_full_size():
+    m = ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
+    x = ax.uniform(1, m.nrows)
+    ref = orc.ncsr(m, x, nthreads=int(os.environ.get("OMP_NUM_THREADS", "8")))
+    for kw in (dict(thw=4, th=1), dict(thw=64, th=4)):
+        y = ax.spmv_host(ax.Plan(m, ax.FMT_AXT, **kw), x)
+        assert rel_err(ref, y) <= TOL, kw
+
+
+def test_edge_shapes():
+    one = ax.CsrMatrix(np.array([0, 1], np.uint32), np.array([0], np.uint32), np.array([2.5]))
+    assert ax.spmv_host(ax.Plan(one, ax.FMT_AXT, thw=4), np.array([3.0]))[0] == 7.5
+    empty = ax.CsrMatrix(np.zeros(7, np.uint32), np.zeros(0, np.uint32), np.zeros(0), ncols=6)
+    for fmt, kw in ((ax.FMT_AXT, dict(thw=4)), (ax.FMT_AXT, dict(thw=64, th=4)), (ax.FMT_CSR, dict(kernel=ax.KER_NAIVE))):
+        y = ax.spmv_host(ax.Plan(empty, fmt, **kw), np.ones(6))
+        assert np.array_equal(y, np.zeros(6))
+
+
+def test_timing_modes():
+    m = ax.laplace2d(128)
+    x = ax.uniform(0, m.ncols)
+    p = ax.Plan(m, ax.FMT_AXT, thw=4)
+    dx, dy = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer(8 * m.nrows)
+    for mode in (ax.core.TIME_EVENTS, ax.core.TIME_GRAPH, ax.core.TIME_REGION):
+        t = p.time(dx, dy, iters=20, warmup=2, mode=mode, batch=10)
+        assert t.mean_ms > 0 and t.min_ms <= t.median_ms <= t.max_ms and t.mode == mode
+    assert rel_err(orc.ncsr(m, x), dy.download(np.float64, m.nrows)) <= TOL
