@@ -38,6 +38,13 @@ OTHERS = [
     ("axt_unc_h4_thw64", ax.FMT_AXT, dict(thw=64, th=4)),
     ("axt_unc_h8_thw64", ax.FMT_AXT, dict(thw=64, th=8)),
     ("axt_unc_h1_thw4_f32val", ax.FMT_AXT, dict(thw=4, th=1, vtype=ax.F32)),
+    ("axt_com_h1_thw64_bs256", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=256)),
+    ("axt_com_h4_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)),
+    ("axt_com_h8_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=8)),
+    ("k1_c64", ax.FMT_K1, dict(chunk=64)),
+    ("k1_c64_natural", ax.FMT_K1, dict(chunk=64, reorder=0)),
+    ("axc_hbs4", ax.FMT_AXC, dict(hbs=4)),
+    ("axc_hbs64", ax.FMT_AXC, dict(hbs=64)),
 ]
 
 

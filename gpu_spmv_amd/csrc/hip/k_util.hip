@@ -19,6 +19,11 @@ __global__ void __launch_bounds__(256) fill_kernel(double *__restrict__ dst, siz
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
 }
 
+__global__ void __launch_bounds__(256) fill_rows_kernel(const uint32_t *__restrict__ rows, uint32_t n, double *__restrict__ y) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[rows[i]] = 0.0;
+}
+
 unsigned grid_for(size_t work, unsigned block) {
     size_t g = (work + block - 1) / block;
     const size_t cap = 256 * 64;  // 256 CUs x 64 workgroups: enough to fill the chip
@@ -41,6 +46,11 @@ hipError_t copy_f64(const double *src, double *dst, size_t n, hipStream_t s) {
 
 hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s) {
     if (n) fill_kernel<<<grid_for(n, 256), 256, 0, s>>>(dst, n, v);
+    return hipGetLastError();
+}
+
+hipError_t fill_rows(const uint32_t *rows, uint32_t n, double *y, hipStream_t s) {
+    if (n) fill_rows_kernel<<<(n + 255) / 256, 256, 0, s>>>(rows, n, y);
     return hipGetLastError();
 }
 

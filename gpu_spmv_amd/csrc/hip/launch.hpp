@@ -32,8 +32,58 @@ hipError_t axt_unc_h(const SegArgs &a, const double *x, double *y, hipStream_t s
 hipError_t csr_naive(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,
                      const double *x, double *y, uint32_t block, hipStream_t s);
 
+// K1: a work item is steps [j0, j1) of one chunk; slot == 0xffffffff writes
+// y directly, otherwise partial sums go to part[slot*chunk + r] and the
+// split list (chunk, first slot, slot count) sums them.
+struct K1Item {
+    uint32_t chunk, chp, j0, j1, slot;
+};
+struct K1Args {
+    const K1Item *items;
+    uint32_t nitems;
+    const uint32_t *split;  // 3 words per split chunk
+    uint32_t nsplit;
+    uint32_t chunk, nrows, vtype;
+    const void *val;
+    const uint32_t *col, *permi;
+    double *part;
+};
+hipError_t k1(const K1Args &a, const double *x, double *y, hipStream_t s);
+
+// AXC: groups of hbs lanes per row; rows longer than long_slots slots are
+// listed in longrows and done one wave per row.
+struct AxcArgs {
+    uint32_t nrows, hbs, vtype, long_slots;
+    const uint32_t *brp, *mapx;
+    const void *vplane;
+    const uint32_t *longrows;
+    uint32_t nlong;
+};
+hipError_t axc(const AxcArgs &a, const double *x, double *y, hipStream_t s);
+
+// AXT COM: a wave owns slots (th == 1) or tile columns (th > 1) [beg, end);
+// flags bit 0: its first run continues a row from the previous wave, bit 1:
+// its last run continues into the next wave.
+struct ComWave {
+    uint32_t beg, end, flags, pad;
+};
+struct ComArgs {
+    const void *vplane;
+    const uint32_t *cplane, *hdr;
+    const ComWave *waves;
+    uint32_t nwaves, thw, th, bs, log, nnz, vtype;
+    double *head, *tail;        // per-wave partials of spanning rows
+    const uint32_t *span;       // (row, first wave, last wave) per spanning row
+    uint32_t nspan;
+    const uint32_t *empty;      // rows without entries
+    uint32_t nempty;
+};
+hipError_t axt_com(const ComArgs &a, const double *x, double *y, hipStream_t s);
+
 hipError_t copy_f64(const double *src, double *dst, size_t n, hipStream_t s);
 hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s);
+// y[rows[i]] = 0 for i < n
+hipError_t fill_rows(const uint32_t *rows, uint32_t n, double *y, hipStream_t s);
 
 }  // namespace launch
 }  // namespace axspmv

@@ -138,10 +138,99 @@ int upload_blocks(Plan &p, const std::vector<launch::Block> &blocks, const std::
     return AXSPMV_OK;
 }
 
+// Row of CSR position q (q < nnz).
+uint32_t row_of(const axspmv_csr *a, uint32_t q) {
+    return uint32_t(std::upper_bound(a->row, a->row + a->nrows + 1, q) - a->row) - 1;
+}
+
+int plan_axt_com(Plan &p, const axspmv_csr *a, hipStream_t s) {
+    const uint32_t thw = p.o.thw, th = p.o.th;
+    axspmv_axt *m = nullptr;
+    AX_TRY(axspmv_build_axt(a, nullptr, p.o.bs, thw, th, AXSPMV_AXT_COM, AXSPMV_BUILD_PLANES, &m, nullptr));
+    std::unique_ptr<axspmv_axt, void (*)(axspmv_axt *)> guard(m, axspmv_axt_free);
+    AX_TRY(upload_planes(p, m->vplane, m->cplane, uint64_t(m->lenAX) / 2, s));
+    p.d_sec = static_cast<uint32_t *>(p.dalloc(4ull * m->lenSEC));
+    if (!p.d_sec) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(p.d_sec, m->sec, 4ull * m->lenSEC, s));
+    const uint32_t nnz = a->nnz;
+    // Wave ranges: th == 1 in slots, a multiple of bs and of 256 (4 slots x
+    // 64 lanes); th > 1 in tile columns, 4 passes of 64 columns.
+    const uint32_t wr = th == 1 ? std::max<uint32_t>(p.o.bs, 1024) : 256;
+    const uint32_t units = th == 1 ? nnz : uint32_t(ceil_div(nnz, th));
+    std::vector<launch::ComWave> waves;
+    for (uint64_t b = 0; b < units; b += wr) {
+        launch::ComWave w{uint32_t(b), uint32_t(std::min<uint64_t>(b + wr, units)), 0, 0};
+        const uint64_t S = th == 1 ? w.beg : uint64_t(w.beg) * th;
+        const uint64_t E = th == 1 ? w.end : std::min<uint64_t>(uint64_t(w.end) * th, nnz);
+        if (S > 0 && row_of(a, uint32_t(S)) == row_of(a, uint32_t(S - 1))) w.flags |= 1;
+        if (E < nnz && row_of(a, uint32_t(E - 1)) == row_of(a, uint32_t(E))) w.flags |= 2;
+        waves.push_back(w);
+    }
+    // Rows spanning waves: (row, first wave, last wave).
+    std::vector<uint32_t> span;
+    bool open = false;
+    uint32_t orow = 0, w0 = 0;
+    for (uint32_t k = 0; k < waves.size(); ++k) {
+        const uint64_t E = th == 1 ? waves[k].end : std::min<uint64_t>(uint64_t(waves[k].end) * th, nnz);
+        const bool out = waves[k].flags & 2;
+        const uint32_t lastrow = row_of(a, uint32_t(E - 1));
+        if (open && !(out && lastrow == orow)) {
+            span.insert(span.end(), {orow, w0, k});
+            open = false;
+        }
+        if (out && !open) {
+            open = true;
+            orow = lastrow;
+            w0 = k;
+        }
+    }
+    std::vector<uint32_t> empty;
+    for (uint32_t r = 0; r < a->nrows; ++r)
+        if (a->row[r + 1] == a->row[r]) empty.push_back(r);
+    auto *dw = static_cast<launch::ComWave *>(p.dalloc(sizeof(launch::ComWave) * waves.size()));
+    auto *dspan = static_cast<uint32_t *>(p.dalloc(4 * span.size()));
+    auto *dempty = static_cast<uint32_t *>(p.dalloc(4 * empty.size()));
+    auto *head = static_cast<double *>(p.dalloc(8 * waves.size()));
+    auto *tail = static_cast<double *>(p.dalloc(8 * waves.size()));
+    if (!dw || !dspan || !dempty || !head || !tail) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(dw, waves.data(), sizeof(launch::ComWave) * waves.size(), s));
+    AX_TRY(p.upload(dspan, span.data(), 4 * span.size(), s));
+    AX_TRY(p.upload(dempty, empty.data(), 4 * empty.size(), s));
+    launch::ComArgs c{};
+    c.vplane = p.d_vplane;
+    c.cplane = p.d_cplane;
+    c.hdr = p.d_sec;
+    c.waves = dw;
+    c.nwaves = uint32_t(waves.size());
+    c.thw = thw;
+    c.th = th;
+    c.bs = p.o.bs;
+    c.log = m->log;
+    c.nnz = nnz;
+    c.vtype = p.o.vtype;
+    c.head = head;
+    c.tail = tail;
+    c.span = dspan;
+    c.nspan = uint32_t(span.size() / 3);
+    c.empty = dempty;
+    c.nempty = uint32_t(empty.size());
+    p.com = c;
+    const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
+    p.stream_bytes = (vs + 8) * p.slots + sizeof(launch::ComWave) * waves.size();
+    p.nblocks = c.nwaves;
+    p.launches = 1 + (c.nspan > 0) + (c.nempty > 0);
+    p.kind = Plan::AXT_COM;
+    if (th == 1)
+        std::snprintf(p.name, sizeof p.name, "axt_com_h1_thw%u_bs%u", thw, p.o.bs);
+    else
+        std::snprintf(p.name, sizeof p.name, "axt_com_h%u_thw%u", th, thw);
+    return AXSPMV_OK;
+}
+
 int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
     const uint32_t thw = p.o.thw, th = p.o.th;
     if (!is_pow2(thw) || thw > 64) return fail(AXSPMV_E_UNSUPPORTED, "AXT kernels need thw a power of two <= 64, got %u", thw);
-    if (p.o.mode == AXSPMV_AXT_COM) return fail(AXSPMV_E_UNSUPPORTED, "AXT COM kernels not built");
+    if (p.o.mode == AXSPMV_AXT_COM) return plan_axt_com(p, a, s);
     axspmv_axt *m = nullptr;
     AX_TRY(axspmv_build_axt(a, nullptr, p.o.bs, thw, th, p.o.mode, AXSPMV_BUILD_PLANES, &m, nullptr));
     std::unique_ptr<axspmv_axt, void (*)(axspmv_axt *)> guard(m, axspmv_axt_free);
@@ -170,6 +259,94 @@ int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
         p.kind = Plan::AXT_UNC_H;
         std::snprintf(p.name, sizeof p.name, "axt_unc_h%u_thw%u", th, thw);
     }
+    return AXSPMV_OK;
+}
+
+// Steps of one 64-lane K1 work item at most; longer chunks are split.
+constexpr uint32_t K1_STEPS = 64;
+
+int plan_k1(Plan &p, const axspmv_csr *a, hipStream_t s) {
+    const uint32_t C = p.o.chunk;
+    if (!is_pow2(C) || C > 64) return fail(AXSPMV_E_UNSUPPORTED, "K1 kernel needs chunk a power of two <= 64, got %u", C);
+    axspmv_k1 *m = nullptr;
+    AX_TRY(axspmv_build_k1(a, C, p.o.reorder ? 1 : 0, &m, nullptr));
+    std::unique_ptr<axspmv_k1, void (*)(axspmv_k1 *)> guard(m, axspmv_k1_free);
+    const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
+    std::vector<launch::K1Item> items;
+    std::vector<uint32_t> split;
+    uint32_t nslot = 0;
+    for (uint32_t c = 0; c < m->chunkNum; ++c) {
+        const uint32_t nm = m->nmc[c];
+        if (nm <= K1_STEPS) {
+            items.push_back({c, m->chp[c], 0, nm, 0xffffffffu});  // writes y directly
+            continue;
+        }
+        const uint32_t parts = uint32_t(ceil_div(nm, K1_STEPS));
+        split.insert(split.end(), {c, nslot, parts});
+        for (uint32_t k = 0; k < parts; ++k)
+            items.push_back({c, m->chp[c], k * K1_STEPS, std::min(nm, (k + 1) * K1_STEPS), nslot + k});
+        nslot += parts;
+    }
+    void *val = p.dalloc(vs * m->lenVC);
+    auto *col = static_cast<uint32_t *>(p.dalloc(4ull * m->lenVC));
+    auto *permi = static_cast<uint32_t *>(p.dalloc(4ull * (m->chunkNum + 1) * C));
+    auto *it = static_cast<launch::K1Item *>(p.dalloc(sizeof(launch::K1Item) * items.size()));
+    auto *sp = static_cast<uint32_t *>(p.dalloc(4 * split.size()));
+    auto *part = static_cast<double *>(p.dalloc(8ull * nslot * C));
+    if (!val || !col || !permi || !it || !sp || !part) return AXSPMV_E_NOMEM;
+    if (p.o.vtype == AXSPMV_F32) {
+        std::vector<float> v(m->val, m->val + m->lenVC);
+        AX_TRY(p.upload(val, v.data(), 4ull * m->lenVC, s));
+    } else {
+        AX_TRY(p.upload(val, m->val, 8ull * m->lenVC, s));
+    }
+    AX_TRY(p.upload(col, m->col, 4ull * m->lenVC, s));
+    AX_TRY(p.upload(permi, m->permi, 4ull * (m->chunkNum + 1) * C, s));
+    AX_TRY(p.upload(it, items.data(), sizeof(launch::K1Item) * items.size(), s));
+    AX_TRY(p.upload(sp, split.data(), 4 * split.size(), s));
+    p.k1 = launch::K1Args{it, uint32_t(items.size()), sp, uint32_t(split.size() / 3), C, a->nrows, p.o.vtype,
+                          val, col, permi, part};
+    p.slots = m->lenVC;
+    p.stream_bytes = (vs + 4) * uint64_t(m->lenVC) + 4ull * a->nrows + sizeof(launch::K1Item) * items.size();
+    p.launches = split.empty() ? 1 : 2;
+    p.nblocks = uint32_t(items.size());
+    p.kind = Plan::K1;
+    std::snprintf(p.name, sizeof p.name, "k1_c%u%s", C, p.o.reorder ? "" : "_natural");
+    return AXSPMV_OK;
+}
+
+int plan_axc(Plan &p, const axspmv_csr *a, hipStream_t s) {
+    const uint32_t G = p.o.hbs;
+    if (!is_pow2(G) || G > 64) return fail(AXSPMV_E_UNSUPPORTED, "AXC kernel needs hbs a power of two <= 64, got %u", G);
+    axspmv_axc *m = nullptr;
+    AX_TRY(axspmv_build_axc(a, nullptr, G, AXSPMV_BUILD_PLANES, &m, nullptr));
+    std::unique_ptr<axspmv_axc, void (*)(axspmv_axc *)> guard(m, axspmv_axc_free);
+    const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
+    const uint32_t long_slots = std::max<uint32_t>(64 * G, 256);
+    std::vector<uint32_t> longrows;
+    for (uint32_t r = 0; r < a->nrows; ++r)
+        if ((m->brp[r + 1] - m->brp[r]) / 2 > long_slots) longrows.push_back(r);
+    const uint64_t nslots = m->lenAX / 2;
+    void *vp = p.dalloc(vs * nslots);
+    auto *mapx = static_cast<uint32_t *>(p.dalloc(4ull * m->lenMAPX));
+    auto *brp = static_cast<uint32_t *>(p.dalloc(4ull * m->lenBRP));
+    auto *lr = static_cast<uint32_t *>(p.dalloc(4 * longrows.size()));
+    if (!vp || !mapx || !brp || !lr) return AXSPMV_E_NOMEM;
+    if (p.o.vtype == AXSPMV_F32) {
+        std::vector<float> v(m->vplane, m->vplane + nslots);
+        AX_TRY(p.upload(vp, v.data(), 4 * nslots, s));
+    } else {
+        AX_TRY(p.upload(vp, m->vplane, 8 * nslots, s));
+    }
+    AX_TRY(p.upload(mapx, m->mapx, 4ull * m->lenMAPX, s));
+    AX_TRY(p.upload(brp, m->brp, 4ull * m->lenBRP, s));
+    AX_TRY(p.upload(lr, longrows.data(), 4 * longrows.size(), s));
+    p.axc = launch::AxcArgs{a->nrows, G, p.o.vtype, long_slots, brp, mapx, vp, lr, uint32_t(longrows.size())};
+    p.slots = nslots;
+    p.stream_bytes = (vs + 4) * nslots + 4ull * m->lenBRP;
+    p.launches = longrows.empty() ? 1 : 2;
+    p.kind = Plan::AXC;
+    std::snprintf(p.name, sizeof p.name, "axc_hbs%u", G);
     return AXSPMV_OK;
 }
 
@@ -231,15 +408,16 @@ extern "C" int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, voi
     int rc = AXSPMV_OK;
     switch (o->format) {
         case AXSPMV_FMT_CSR: rc = plan_csr(*p, a, s); break;
+        case AXSPMV_FMT_K1: rc = plan_k1(*p, a, s); break;
+        case AXSPMV_FMT_AXC: rc = plan_axc(*p, a, s); break;
         case AXSPMV_FMT_AXT: rc = plan_axt(*p, a, s); break;
-        default: rc = fail(AXSPMV_E_UNSUPPORTED, "plan_create: format %u not built", o->format);
+        default: rc = fail(AXSPMV_E_INVALID, "plan_create: unknown format %u", o->format);
     }
     if (rc) return rc;
     const size_t vs = p->o.vtype == AXSPMV_F32 ? 4 : 8;
     p->alg_bytes = (vs + 4) * uint64_t(a->nnz) + 4ull * (a->nrows + 1) + 8ull * a->ncols + 8ull * a->nrows;
     p->stream_bytes += 8ull * a->ncols + 8ull * a->nrows;
     p->beta = p->slots ? double(a->nnz) / double(p->slots) : 0.0;
-    p->launches = 1;
     *out = p.release();
     return AXSPMV_OK;
 }
@@ -275,6 +453,9 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
             return launch::csr_naive(p->nrows, p->d_val, p->o.vtype, p->d_col, p->d_row, x, y, p->o.block, s);
         case Plan::AXT_UNC_H1: return launch::axt_unc_h1(seg_args(*p, p->d_sec), x, y, s);
         case Plan::AXT_UNC_H: return launch::axt_unc_h(seg_args(*p, p->d_sec), x, y, s);
+        case Plan::AXT_COM: return launch::axt_com(p->com, x, y, s);
+        case Plan::K1: return launch::k1(p->k1, x, y, s);
+        case Plan::AXC: return launch::axc(p->axc, x, y, s);
         default: return hipErrorInvalidValue;
     }
 }

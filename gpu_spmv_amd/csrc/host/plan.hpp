@@ -10,7 +10,7 @@
 #include "common.hpp"
 
 struct axspmv_plan {
-    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H };
+    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC };
     Kind kind = NONE;
     int device = 0;
     axspmv_opts o{};
@@ -28,6 +28,10 @@ struct axspmv_plan {
     void *d_vplane = nullptr;
     uint32_t *d_cplane = nullptr, *d_sec = nullptr, *d_empty = nullptr;
     axspmv::launch::Block *d_blocks = nullptr;
+    // K1 work items / AXC long rows
+    axspmv::launch::K1Args k1{};
+    axspmv::launch::AxcArgs axc{};
+    axspmv::launch::ComArgs com{};
 
     std::vector<void *> bufs;
     void *dalloc(size_t bytes);

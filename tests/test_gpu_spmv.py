@@ -31,6 +31,22 @@ KERNELS = [
     ("unc_h12_thw32", ax.FMT_AXT, dict(thw=32, th=12)),
     ("unc_h5_thw64", ax.FMT_AXT, dict(thw=64, th=5)),
     ("unc_h1_thw4_small_budget", ax.FMT_AXT, dict(thw=4, th=1, budget=64)),
+    ("k1_c64", ax.FMT_K1, dict(chunk=64)),
+    ("k1_c64_natural", ax.FMT_K1, dict(chunk=64, reorder=0)),
+    ("k1_c32", ax.FMT_K1, dict(chunk=32)),
+    ("axc_h64", ax.FMT_AXC, dict(hbs=64)),
+    ("axc_h32", ax.FMT_AXC, dict(hbs=32)),
+    ("axc_h8", ax.FMT_AXC, dict(hbs=8)),
+    ("axc_h4", ax.FMT_AXC, dict(hbs=4)),
+    ("com_h1_thw64_bs256", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=256)),
+    ("com_h1_thw64_bs1024", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=1024)),
+    ("com_h1_thw32_bs64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=32, th=1, bs=64)),
+    ("com_h1_thw4_bs2", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=4, th=1, bs=2)),
+    ("com_h4_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)),
+    ("com_h8_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=8)),
+    ("com_h16_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=16)),
+    ("com_h4_thw32", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=32, th=4)),
+    ("com_h32_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=32)),
 ]
 
 
@@ -85,8 +101,12 @@ def test_spmv_f64(mname, kname, fmt, kw):
     assert np.array_equal(y, y2)
 
 
+F32_KERNELS = [k for k in KERNELS if k[0] in ("csr_naive", "unc_h1_thw4", "unc_h4_thw64", "k1_c64", "axc_h8",
+                                                "com_h1_thw64_bs256", "com_h8_thw64")]
+
+
 @pytest.mark.parametrize("mname", ["rand200", "powerlaw64k", "long_rows"])
-@pytest.mark.parametrize("kname,fmt,kw", [KERNELS[0], KERNELS[1], KERNELS[7]], ids=["csr", "h1", "h4"])
+@pytest.mark.parametrize("kname,fmt,kw", F32_KERNELS, ids=[k[0] for k in F32_KERNELS])
 def test_spmv_f32_values(mname, kname, fmt, kw):
     m = MATS[mname]
     x = ax.uniform(12, m.ncols)
@@ -101,8 +121,11 @@ def test_powerlaw_config2_full_size():
     m = ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
     x = ax.uniform(1, m.nrows)
     ref = orc.ncsr(m, x, nthreads=int(os.environ.get("OMP_NUM_THREADS", "8")))
-    for kw in (dict(thw=4, th=1), dict(thw=64, th=4)):
-        y = ax.spmv_host(ax.Plan(m, ax.FMT_AXT, **kw), x)
+    for fmt, kw in ((ax.FMT_AXT, dict(thw=4, th=1)), (ax.FMT_AXT, dict(thw=64, th=4)),
+                    (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=256)),
+                    (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)), (ax.FMT_K1, dict(chunk=64)),
+                    (ax.FMT_AXC, dict(hbs=4))):
+        y = ax.spmv_host(ax.Plan(m, fmt, **kw), x)
         assert rel_err(ref, y) <= TOL, kw
 
 
