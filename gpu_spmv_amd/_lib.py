@@ -72,7 +72,7 @@ class Axt(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("format", u32), ("kernel", u32), ("mode", u32), ("thw", u32), ("th", u32),
                 ("bs", u32), ("hbs", u32), ("chunk", u32), ("reorder", u32), ("vtype", u32),
-                ("block", u32), ("budget", u32), ("flags", u32), ("reserved", u32 * 3)]
+                ("block", u32), ("budget", u32), ("flags", u32), ("ktile", u32), ("reserved", u32 * 2)]
 
 
 class PlanInfo(C.Structure):
@@ -84,6 +84,15 @@ class PlanInfo(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("mean_ms", f64), ("median_ms", f64), ("min_ms", f64), ("max_ms", f64),
                 ("iters", u32), ("mode", u32)]
+
+
+# enum axspmv_op
+OP_SPMV, OP_ADJOINT, OP_VBAR, OP_FWD_ADJ, OP_FWD_ADJ_SPLIT, OP_SPMM = 0, 1, 2, 3, 4, 5
+
+
+class Operands(C.Structure):
+    _fields_ = [("x", vp), ("b", vp), ("ybar", vp), ("X", vp), ("y", vp), ("r", vp), ("xbar", vp),
+                ("vbar", vp), ("Y", vp), ("k", u32), ("pad_", u32)]
 
 
 # name -> (restype, argtypes); every entry point declared in include/axspmv.h
@@ -118,7 +127,12 @@ SIGNATURES = {
     "axspmv_plan_destroy": (C.c_int, [vp]),
     "axspmv_plan_get_info": (C.c_int, [vp, P(PlanInfo)]),
     "axspmv_spmv": (C.c_int, [vp, vp, vp, vp]),
+    "axspmv_adjoint": (C.c_int, [vp, vp, vp, vp]),
+    "axspmv_vbar": (C.c_int, [vp, vp, vp, vp, vp]),
+    "axspmv_fwd_adj": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]),
+    "axspmv_spmm": (C.c_int, [vp, u32, vp, vp, vp]),
     "axspmv_time_spmv": (C.c_int, [vp, vp, vp, u32, u32, u32, u32, vp, P(Timing)]),
+    "axspmv_time_op": (C.c_int, [vp, u32, P(Operands), u32, u32, u32, u32, vp, P(Timing)]),
     "axspmv_dev_alloc": (C.c_int, [C.c_size_t, P(vp)]),
     "axspmv_dev_free": (C.c_int, [vp]),
     "axspmv_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t, vp]),

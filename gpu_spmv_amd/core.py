@@ -293,6 +293,33 @@ class Plan:
                stream, C.byref(t))
         return TimingResult(t.mean_ms, t.median_ms, t.min_ms, t.max_ms, t.iters, t.mode)
 
+    def adjoint(self, dybar, dxbar, stream=None) -> None:
+        """x_bar = A^T y_bar (plan built with flags=PLAN_TRANSPOSE)."""
+        L.call("axspmv_adjoint", self.ptr, _dptr(dybar), _dptr(dxbar), stream)
+
+    def vbar(self, dybar, dx, dvbar, stream=None) -> None:
+        """v_bar[k] = y_bar[row(k)] * x[col(k)] in CSR order."""
+        L.call("axspmv_vbar", self.ptr, _dptr(dybar), _dptr(dx), _dptr(dvbar), stream)
+
+    def fwd_adj(self, dx, db, dy, dr, dxbar, dvbar, fused: bool = True, stream=None) -> None:
+        """y = A x, r = y - b, x_bar = A^T r, v_bar = r[row] * x[col]."""
+        L.call("axspmv_fwd_adj", self.ptr, _dptr(dx), _dptr(db), _dptr(dy), _dptr(dr), _dptr(dxbar),
+               _dptr(dvbar), int(fused), stream)
+
+    def spmm(self, k: int, dX, dY, stream=None) -> None:
+        """Y = A X, X (ncols x k) and Y (nrows x k) row-major."""
+        L.call("axspmv_spmm", self.ptr, k, _dptr(dX), _dptr(dY), stream)
+
+    def time_op(self, op: int, iters: int = 100, warmup: int = 10, mode: int = TIME_REGION, batch: int = 1,
+                stream=None, **operands) -> TimingResult:
+        """Time one operation (L.OP_*); operands by name (x, b, ybar, X, y, r, xbar, vbar, Y, k)."""
+        o = L.Operands()
+        for name, v in operands.items():
+            setattr(o, name, v if name == "k" else _dptr(v))
+        t = L.Timing()
+        L.call("axspmv_time_op", self.ptr, op, C.byref(o), iters, warmup, mode, batch, stream, C.byref(t))
+        return TimingResult(t.mean_ms, t.median_ms, t.min_ms, t.max_ms, t.iters, t.mode)
+
     def close(self) -> None:
         if self.ptr:
             L.load().axspmv_plan_destroy(self.ptr)

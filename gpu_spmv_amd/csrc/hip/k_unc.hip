@@ -98,6 +98,26 @@ struct UncH1Src {
         return acc;
     }
     __device__ __forceinline__ uint32_t row(uint32_t u) const { return sec[u >> ushift]; }
+    // As sum(), keeping the gathered x values.
+    __device__ __forceinline__ double sum_keep(uint32_t u, const double *__restrict__ x, double *xv) const {
+        const size_t q = size_t(u) * K;
+        VT v[K];
+        uint32_t c[K];
+        VecLoad<VT, K>::get(vp + q, v);
+        load_cols<K>(cp + q, c);
+#pragma unroll
+        for (int j = 0; j < K; ++j) xv[j] = x[c[j]];
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc = fma(double(v[j]), xv[j], acc);
+        return acc;
+    }
+    __device__ __forceinline__ void gather_x(uint32_t u, const double *__restrict__ x, double *xv) const {
+        uint32_t c[K];
+        load_cols<K>(cp + size_t(u) * K, c);
+#pragma unroll
+        for (int j = 0; j < K; ++j) xv[j] = x[c[j]];
+    }
 };
 
 // th > 1: unit g is global tile column g; level f sits at
@@ -174,6 +194,99 @@ __global__ void __launch_bounds__(256) rowblock_kernel(Src src, const Block *__r
     }
 }
 
+// ------------------------------------------------------------------ adjoint
+// Value gradient of the AXT UNC th == 1 layout: plane slot q of row r is CSR
+// position rowptr[r] + q - first[r] (first[r] = first plane slot of row r);
+// slots past the row length are padding.
+using axspmv::launch::VbarArgs;
+
+template <int K>
+__device__ __forceinline__ void put_vbar(uint32_t u, uint32_t r, double s, const double *xv, const VbarArgs &va,
+                                         double *__restrict__ vbar) {
+    const uint32_t off = u * K - va.first[r];
+    const uint32_t b = va.rowptr[r], len = va.rowptr[r + 1] - b;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (off + j < len) vbar[b + off + j] = s * xv[j];
+}
+
+// v_bar[k] = y_bar[row(k)] * x[col(k)], one lane per unit of K slots.
+template <class VT, int K>
+__global__ void __launch_bounds__(256) vbar_kernel(UncH1Src<VT, K> src, uint32_t nunits, const double *__restrict__ x,
+                                                   const double *__restrict__ ybar, double *__restrict__ vbar,
+                                                   VbarArgs va) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    const uint32_t r = src.row(u);
+    double xv[K];
+    src.gather_x(u, x, xv);
+    put_vbar<K>(u, r, ybar[r], xv, va, vbar);
+}
+
+// Fused forward pass of the least-squares adjoint: y = A x, r = y - b and
+// v_bar[k] = r[row(k)] * x[col(k)] in one sweep over A.  Blocks hold at most
+// 64 units (one pass of the wave), so each row's total is known while the
+// gathered x values are still in registers; the row's residual goes back to
+// its lanes with ballot + bpermute.  A block of more than 64 units is a
+// single long row and takes a second sweep for v_bar.
+template <class VT, int K>
+__global__ void __launch_bounds__(256) fused_kernel(UncH1Src<VT, K> src, const Block *__restrict__ blocks,
+                                                    uint32_t nblocks, const uint32_t *__restrict__ empty,
+                                                    uint32_t nempty, const double *__restrict__ x,
+                                                    const double *__restrict__ b, double *__restrict__ y,
+                                                    double *__restrict__ res, double *__restrict__ vbar, VbarArgs va) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = wave * 64 + lane; i < nempty; i += nwaves * 64) {
+        const uint32_t r = empty[i];
+        y[r] = 0.0;
+        res[r] = -b[r];
+    }
+    if (wave >= nblocks) return;
+    const Block bl = blocks[wave];
+    if (bl.end - bl.beg <= 64) {
+        const uint32_t u = bl.beg + lane;
+        const bool act = u < bl.end;
+        double xv[K];
+        double s = 0.0;
+        uint32_t r = NOROW;
+        if (act) {
+            s = src.sum_keep(u, x, xv);
+            r = src.row(u);
+        }
+        uint32_t head = r != shift_up_u32(r, NOROW);
+        const double v = seg_scan_f64(s, head);
+        const bool end = act && shift_down_u32(r, NOROW) != r;
+        double rv = 0.0;
+        if (end) {
+            rv = v - b[r];
+            y[r] = v;
+            res[r] = rv;
+        }
+        const uint64_t endmask = __ballot(end);
+        const int src_lane = act ? int(lane) + __builtin_ctzll(endmask >> lane) : int(lane);
+        rv = __shfl(rv, src_lane);
+        if (act) put_vbar<K>(u, r, rv, xv, va, vbar);
+        return;
+    }
+    // one long row
+    const uint32_t r = src.row(bl.beg);
+    double acc = 0.0;
+    for (uint32_t u = bl.beg + lane; u < bl.end; u += 64) acc += src.sum(u, x);
+    const double tot = group_sum_f64<64>(acc);
+    const double rv = tot - b[r];
+    if (lane == 0) {
+        y[r] = tot;
+        res[r] = rv;
+    }
+    for (uint32_t u = bl.beg + lane; u < bl.end; u += 64) {
+        double xv[K];
+        src.gather_x(u, x, xv);
+        put_vbar<K>(u, r, rv, xv, va, vbar);
+    }
+}
+
 template <class Src>
 hipError_t run(const Src &src, const axspmv::launch::SegArgs &a, const double *x, double *y, hipStream_t s) {
     const uint32_t block = 256;
@@ -222,6 +335,51 @@ hipError_t axt_unc_h1(const SegArgs &a, const double *x, double *y, hipStream_t 
 
 hipError_t axt_unc_h(const SegArgs &a, const double *x, double *y, hipStream_t s) {
     return a.vtype == 1 ? unc_h_vt<float>(a, x, y, s) : unc_h_vt<double>(a, x, y, s);
+}
+
+namespace {
+template <class VT, int K>
+hipError_t vbar_vk(const SegArgs &a, uint32_t ushift, uint32_t nunits, const double *x, const double *ybar,
+                   double *vbar, const VbarArgs &va, hipStream_t s) {
+    const UncH1Src<VT, K> src{static_cast<const VT *>(a.vplane), a.cplane, a.rowsrc, ushift};
+    if (nunits) vbar_kernel<VT, K><<<(nunits + 255) / 256, 256, 0, s>>>(src, nunits, x, ybar, vbar, va);
+    return hipGetLastError();
+}
+template <class VT, int K>
+hipError_t fused_vk(const SegArgs &a, uint32_t ushift, const double *x, const double *b, double *y, double *res,
+                    double *vbar, const VbarArgs &va, hipStream_t s) {
+    const UncH1Src<VT, K> src{static_cast<const VT *>(a.vplane), a.cplane, a.rowsrc, ushift};
+    const uint32_t waves = a.nblocks ? a.nblocks : 1;
+    fused_kernel<VT, K><<<(waves + 3) / 4, 256, 0, s>>>(src, a.blocks, a.nblocks, a.empty, a.nempty, x, b, y, res,
+                                                       vbar, va);
+    return hipGetLastError();
+}
+template <class VT>
+hipError_t vbar_v(const SegArgs &a, uint32_t nunits, const double *x, const double *ybar, double *vbar,
+                  const VbarArgs &va, hipStream_t s) {
+    const uint32_t tl = __builtin_ctz(a.thw);
+    if (a.thw >= 4) return vbar_vk<VT, 4>(a, tl - 2, nunits, x, ybar, vbar, va, s);
+    if (a.thw == 2) return vbar_vk<VT, 2>(a, 0, nunits, x, ybar, vbar, va, s);
+    return vbar_vk<VT, 1>(a, 0, nunits, x, ybar, vbar, va, s);
+}
+template <class VT>
+hipError_t fused_v(const SegArgs &a, const double *x, const double *b, double *y, double *res, double *vbar,
+                   const VbarArgs &va, hipStream_t s) {
+    const uint32_t tl = __builtin_ctz(a.thw);
+    if (a.thw >= 4) return fused_vk<VT, 4>(a, tl - 2, x, b, y, res, vbar, va, s);
+    if (a.thw == 2) return fused_vk<VT, 2>(a, 0, x, b, y, res, vbar, va, s);
+    return fused_vk<VT, 1>(a, 0, x, b, y, res, vbar, va, s);
+}
+}  // namespace
+
+hipError_t axt_unc_h1_vbar(const SegArgs &a, uint32_t nunits, const double *x, const double *ybar, double *vbar,
+                           const VbarArgs &va, hipStream_t s) {
+    return a.vtype == 1 ? vbar_v<float>(a, nunits, x, ybar, vbar, va, s) : vbar_v<double>(a, nunits, x, ybar, vbar, va, s);
+}
+
+hipError_t axt_unc_h1_fused(const SegArgs &a, const double *x, const double *b, double *y, double *res,
+                            double *vbar, const VbarArgs &va, hipStream_t s) {
+    return a.vtype == 1 ? fused_v<float>(a, x, b, y, res, vbar, va, s) : fused_v<double>(a, x, b, y, res, vbar, va, s);
 }
 
 }  // namespace launch

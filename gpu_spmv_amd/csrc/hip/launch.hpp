@@ -29,6 +29,34 @@ struct SegArgs {
 hipError_t axt_unc_h1(const SegArgs &a, const double *x, double *y, hipStream_t s);
 hipError_t axt_unc_h(const SegArgs &a, const double *x, double *y, hipStream_t s);
 
+// Plane slot -> CSR position of the AXT UNC th == 1 layout.
+struct VbarArgs {
+    const uint32_t *rowptr;  // CSR row pointer, nrows + 1
+    const uint32_t *first;   // first plane slot of each row
+};
+hipError_t axt_unc_h1_vbar(const SegArgs &a, uint32_t nunits, const double *x, const double *ybar, double *vbar,
+                           const VbarArgs &va, hipStream_t s);
+// a.blocks must hold blocks of at most 64 units or single rows.
+hipError_t axt_unc_h1_fused(const SegArgs &a, const double *x, const double *b, double *y, double *res,
+                            double *vbar, const VbarArgs &va, hipStream_t s);
+hipError_t csr_vbar(uint32_t nrows, const uint32_t *col, const uint32_t *row, const double *x, const double *ybar,
+                    double *vbar, hipStream_t s);
+hipError_t csr_fused(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,
+                     const double *x, const double *b, double *y, double *res, double *vbar, hipStream_t s);
+hipError_t residual(uint32_t n, const double *y, const double *b, double *r, hipStream_t s);
+
+// SpMM on a row-contiguous layout: row r = slots first[r] .. first[r] + rl - 1.
+struct SpmmArgs {
+    uint32_t nrows, vtype, long_slots;
+    const uint32_t *first, *rowptr;
+    const void *val;
+    const uint32_t *col;
+    const uint32_t *longrows;
+    uint32_t nlong;
+};
+// kt (k-tile) in {8, 16, 32, 64}
+hipError_t spmm(const SpmmArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s);
+
 hipError_t csr_naive(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,
                      const double *x, double *y, uint32_t block, hipStream_t s);
 

@@ -84,9 +84,10 @@ void summarize(std::vector<double> &ms, double scale, axspmv_timing *out) {
 // (cudaSpmv.cu:563-581), plus the median.  GRAPH: hipGraph replays of
 // `graph_batch` SpMVs each, reported per SpMV.  REGION: one event pair
 // around `iters` back-to-back launches.
-extern "C" int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, uint32_t iters, uint32_t warmup,
-                                uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
-    if (!p || !out || iters == 0 || mode > AXSPMV_TIME_REGION) return fail(AXSPMV_E_INVALID, "time_spmv: bad argument");
+extern "C" int axspmv_time_op(axspmv_plan *p, uint32_t op, const axspmv_operands *ops, uint32_t iters,
+                              uint32_t warmup, uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
+    if (!p || !ops || !out || iters == 0 || mode > AXSPMV_TIME_REGION) return fail(AXSPMV_E_INVALID, "time_op: bad argument");
+    AX_TRY(plan_op_check(p, op, *ops));
     std::memset(out, 0, sizeof *out);
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipStream_t own = nullptr;
@@ -100,11 +101,11 @@ extern "C" int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, 
     out->iters = iters;
     if (mode != AXSPMV_TIME_GRAPH) {
         for (uint32_t i = 0; i < warmup; ++i)
-            if (plan_launch(p, d_x, d_y, s) != hipSuccess) return fail(AXSPMV_E_HIP, "time_spmv: launch failed");
+            if (plan_op(p, op, *ops, s) != hipSuccess) return fail(AXSPMV_E_HIP, "time_spmv: launch failed");
         if (mode == AXSPMV_TIME_REGION) {
             hipError_t e = hipStreamSynchronize(s);
             if (e == hipSuccess) e = hipEventRecord(ev.a, s);
-            for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) e = plan_launch(p, d_x, d_y, s);
+            for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) e = plan_op(p, op, *ops, s);
             if (e == hipSuccess) e = hipEventRecord(ev.b, s);
             if (e == hipSuccess) e = hipEventSynchronize(ev.b);
             float t = 0.f;
@@ -115,7 +116,7 @@ extern "C" int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, 
         }
         for (uint32_t i = 0; i < iters && rc == AXSPMV_OK; ++i) {
             hipError_t e = hipEventRecord(ev.a, s);
-            if (e == hipSuccess) e = plan_launch(p, d_x, d_y, s);
+            if (e == hipSuccess) e = plan_op(p, op, *ops, s);
             if (e == hipSuccess) e = hipEventRecord(ev.b, s);
             if (e == hipSuccess) e = hipEventSynchronize(ev.b);
             float t = 0.f;
@@ -135,7 +136,7 @@ extern "C" int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, 
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-    for (uint32_t i = 0; i < graph_batch && e == hipSuccess; ++i) e = plan_launch(p, d_x, d_y, s);
+    for (uint32_t i = 0; i < graph_batch && e == hipSuccess; ++i) e = plan_op(p, op, *ops, s);
     const hipError_t e2 = hipStreamEndCapture(s, &g);
     if (e == hipSuccess) e = e2;
     if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -156,4 +157,12 @@ extern "C" int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, 
     if (e != hipSuccess) return hip_fail(e, "time_spmv (graph)");
     summarize(ms, 1.0 / graph_batch, out);
     return AXSPMV_OK;
+}
+
+extern "C" int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, uint32_t iters, uint32_t warmup,
+                                uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
+    axspmv_operands o{};
+    o.x = d_x;
+    o.y = d_y;
+    return axspmv_time_op(p, AXSPMV_OP_SPMV, &o, iters, warmup, mode, graph_batch, stream, out);
 }

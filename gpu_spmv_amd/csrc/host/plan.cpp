@@ -43,6 +43,7 @@ int axspmv_plan::upload(void *dst, const void *src, size_t bytes, hipStream_t s)
 }
 
 axspmv_plan::~axspmv_plan() {
+    delete T;
     for (void *p : bufs) (void)hipFree(p);
 }
 
@@ -89,6 +90,36 @@ std::vector<launch::Block> row_blocks(const std::vector<uint32_t> &units_per_row
 
 namespace {
 
+// Rows longer than this many slots are done one wave per row by SpMM.
+constexpr uint32_t SPMM_LONG = 256;
+
+// Row-contiguous layouts: record where each row starts (device array
+// `first`, uploaded here unless it is the CSR row pointer itself) for SpMM
+// and the value gradient.
+int setup_rowwise(Plan &p, const axspmv_csr *a, const uint32_t *d_first, const void *d_val, const uint32_t *d_col,
+                  hipStream_t s) {
+    if (!p.d_rowptr) {
+        if (p.d_row) {
+            p.d_rowptr = p.d_row;
+        } else {
+            p.d_rowptr = static_cast<uint32_t *>(p.dalloc(4ull * (a->nrows + 1)));
+            if (!p.d_rowptr) return AXSPMV_E_NOMEM;
+            AX_TRY(p.upload(p.d_rowptr, a->row, 4ull * (a->nrows + 1), s));
+        }
+    }
+    p.d_first = const_cast<uint32_t *>(d_first);
+    std::vector<uint32_t> longrows;
+    for (uint32_t r = 0; r < a->nrows; ++r)
+        if (a->row[r + 1] - a->row[r] > SPMM_LONG) longrows.push_back(r);
+    auto *lr = static_cast<uint32_t *>(p.dalloc(4 * longrows.size()));
+    if (!lr) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(lr, longrows.data(), 4 * longrows.size(), s));
+    p.spmm = launch::SpmmArgs{a->nrows, p.o.vtype, SPMM_LONG, p.d_first, p.d_rowptr, d_val, d_col, lr,
+                              uint32_t(longrows.size())};
+    p.rowwise = true;
+    return AXSPMV_OK;
+}
+
 int plan_csr(Plan &p, const axspmv_csr *a, hipStream_t s) {
     const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
     p.d_val = p.dalloc(vs * a->nnz);
@@ -108,7 +139,7 @@ int plan_csr(Plan &p, const axspmv_csr *a, hipStream_t s) {
     if (p.o.kernel == AXSPMV_KER_STREAM) return fail(AXSPMV_E_UNSUPPORTED, "CSR stream kernel not built");
     p.kind = Plan::CSR_NAIVE;
     std::snprintf(p.name, sizeof p.name, "csr_naive");
-    return AXSPMV_OK;
+    return setup_rowwise(p, a, p.d_row, p.d_val, p.d_col, s);
 }
 
 int upload_planes(Plan &p, const double *vplane, const uint32_t *cplane, uint64_t nslots, hipStream_t s) {
@@ -252,14 +283,27 @@ int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
     p.thw = thw;
     p.th = th;
     p.stream_bytes = (vs + 4) * p.slots + 4ull * m->lenSEC + 8ull * p.nblocks;
-    if (th == 1) {
-        p.kind = Plan::AXT_UNC_H1;
-        std::snprintf(p.name, sizeof p.name, "axt_unc_h1_thw%u", thw);
-    } else {
+    if (th != 1) {
         p.kind = Plan::AXT_UNC_H;
         std::snprintf(p.name, sizeof p.name, "axt_unc_h%u_thw%u", th, thw);
+        return AXSPMV_OK;
     }
-    return AXSPMV_OK;
+    p.kind = Plan::AXT_UNC_H1;
+    std::snprintf(p.name, sizeof p.name, "axt_unc_h1_thw%u", thw);
+    // Rows are contiguous in the planes: first slot of row r is con[r]/2.
+    std::vector<uint32_t> first(a->nrows ? a->nrows : 1);
+    for (uint32_t r = 0; r < a->nrows; ++r) first[r] = m->con[r] / 2;
+    auto *d_first = static_cast<uint32_t *>(p.dalloc(4ull * first.size()));
+    if (!d_first) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(d_first, first.data(), 4ull * first.size(), s));
+    p.nunits = uint32_t(uint64_t(m->tileN) * thw / K);
+    // One-pass blocks (<= 64 units) for the fused forward/adjoint kernel.
+    const std::vector<launch::Block> fb = row_blocks(units, 64);
+    p.nfblocks = uint32_t(fb.size());
+    p.d_fblocks = static_cast<launch::Block *>(p.dalloc(sizeof(launch::Block) * fb.size()));
+    if (!p.d_fblocks) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(p.d_fblocks, fb.data(), sizeof(launch::Block) * fb.size(), s));
+    return setup_rowwise(p, a, d_first, p.d_vplane, p.d_cplane, s);
 }
 
 // Steps of one 64-lane K1 work item at most; longer chunks are split.
@@ -347,7 +391,13 @@ int plan_axc(Plan &p, const axspmv_csr *a, hipStream_t s) {
     p.launches = longrows.empty() ? 1 : 2;
     p.kind = Plan::AXC;
     std::snprintf(p.name, sizeof p.name, "axc_hbs%u", G);
-    return AXSPMV_OK;
+    // Rows are contiguous in the value plane: first slot of row r is brp[r]/2.
+    std::vector<uint32_t> first(a->nrows ? a->nrows : 1);
+    for (uint32_t r = 0; r < a->nrows; ++r) first[r] = m->brp[r] / 2;
+    auto *d_first = static_cast<uint32_t *>(p.dalloc(4ull * first.size()));
+    if (!d_first) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(d_first, first.data(), 4ull * first.size(), s));
+    return setup_rowwise(p, a, d_first, vp, mapx, s);
 }
 
 launch::SegArgs seg_args(const Plan &p, const uint32_t *rowsrc) {
@@ -414,6 +464,7 @@ extern "C" int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, voi
         default: rc = fail(AXSPMV_E_INVALID, "plan_create: unknown format %u", o->format);
     }
     if (rc) return rc;
+    if (p->o.flags & AXSPMV_PLAN_TRANSPOSE) AX_TRY(plan_add_transpose(p.get(), a, s));
     const size_t vs = p->o.vtype == AXSPMV_F32 ? 4 : 8;
     p->alg_bytes = (vs + 4) * uint64_t(a->nnz) + 4ull * (a->nrows + 1) + 8ull * a->ncols + 8ull * a->nrows;
     p->stream_bytes += 8ull * a->ncols + 8ull * a->nrows;
@@ -466,4 +517,140 @@ extern "C" int axspmv_spmv(axspmv_plan *p, const double *d_x, double *d_y, void 
     if (!p || (!d_x && p->ncols) || (!d_y && p->nrows)) return fail(AXSPMV_E_INVALID, "spmv: NULL argument");
     const hipError_t e = plan_launch(p, d_x, d_y, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "spmv launch");
+}
+
+// ---------------------------------------------------------------- adjoint
+namespace {
+
+launch::SegArgs fused_args(const Plan &p) {
+    launch::SegArgs g = seg_args(p, p.d_sec);
+    g.blocks = p.d_fblocks;
+    g.nblocks = p.nfblocks;
+    return g;
+}
+
+hipError_t launch_vbar(Plan *p, const double *ybar, const double *x, double *vbar, hipStream_t s) {
+    if (p->kind == Plan::AXT_UNC_H1)
+        return launch::axt_unc_h1_vbar(seg_args(*p, p->d_sec), p->nunits, x, ybar, vbar,
+                                       launch::VbarArgs{p->d_rowptr, p->d_first}, s);
+    return launch::csr_vbar(p->nrows, p->d_col, p->d_row, x, ybar, vbar, s);
+}
+
+hipError_t launch_fwd_adj(Plan *p, const double *x, const double *b, double *y, double *r, double *xbar,
+                          double *vbar, bool fused, hipStream_t s) {
+    hipError_t e;
+    if (fused) {
+        if (p->kind == Plan::AXT_UNC_H1)
+            e = launch::axt_unc_h1_fused(fused_args(*p), x, b, y, r, vbar, launch::VbarArgs{p->d_rowptr, p->d_first}, s);
+        else
+            e = launch::csr_fused(p->nrows, p->d_val, p->o.vtype, p->d_col, p->d_row, x, b, y, r, vbar, s);
+        if (e != hipSuccess) return e;
+        return plan_launch(p->T, r, xbar, s);
+    }
+    e = plan_launch(p, x, y, s);
+    if (e == hipSuccess) e = launch::residual(p->nrows, y, b, r, s);
+    if (e == hipSuccess) e = plan_launch(p->T, r, xbar, s);
+    if (e == hipSuccess) e = launch_vbar(p, r, x, vbar, s);
+    return e;
+}
+
+uint32_t pick_ktile(const Plan *p, uint32_t k) {
+    if (p->o.ktile) return p->o.ktile;
+    uint32_t kt = 8;
+    while (kt < 64 && kt < k) kt *= 2;
+    return kt;
+}
+
+bool adjoint_kind(const Plan *p) { return p->kind == Plan::AXT_UNC_H1 || p->kind == Plan::CSR_NAIVE; }
+
+}  // namespace
+
+namespace axspmv {
+
+int plan_add_transpose(Plan *p, const axspmv_csr *a, hipStream_t s) {
+    axspmv_csr *t = nullptr;
+    AX_TRY(axspmv_csr_transpose(a, &t));
+    axspmv_opts o = p->o;
+    o.flags &= ~AXSPMV_PLAN_TRANSPOSE;
+    axspmv_plan *pt = nullptr;
+    const int rc = axspmv_plan_create(t, &o, s, &pt);
+    axspmv_csr_free(t);
+    if (rc) return rc;
+    p->T = pt;
+    return AXSPMV_OK;
+}
+
+hipError_t plan_op(Plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s) {
+    switch (op) {
+        case AXSPMV_OP_SPMV: return plan_launch(p, o.x, o.y, s);
+        case AXSPMV_OP_ADJOINT: return plan_launch(p->T, o.ybar, o.xbar, s);
+        case AXSPMV_OP_VBAR: return launch_vbar(p, o.ybar, o.x, o.vbar, s);
+        case AXSPMV_OP_FWD_ADJ: return launch_fwd_adj(p, o.x, o.b, o.y, o.r, o.xbar, o.vbar, true, s);
+        case AXSPMV_OP_FWD_ADJ_SPLIT: return launch_fwd_adj(p, o.x, o.b, o.y, o.r, o.xbar, o.vbar, false, s);
+        case AXSPMV_OP_SPMM: return launch::spmm(p->spmm, o.k, pick_ktile(p, o.k), o.X, o.Y, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+int plan_op_check(const Plan *p, uint32_t op, const axspmv_operands &o) {
+    if (!p) return fail(AXSPMV_E_INVALID, "NULL plan");
+    const bool needT = op == AXSPMV_OP_ADJOINT || op == AXSPMV_OP_FWD_ADJ || op == AXSPMV_OP_FWD_ADJ_SPLIT;
+    if (needT && !p->T) return fail(AXSPMV_E_INVALID, "the adjoint needs a plan created with AXSPMV_PLAN_TRANSPOSE");
+    if ((op == AXSPMV_OP_VBAR || op == AXSPMV_OP_FWD_ADJ || op == AXSPMV_OP_FWD_ADJ_SPLIT) && !adjoint_kind(p))
+        return fail(AXSPMV_E_UNSUPPORTED, "value gradient: plan %s is not CSR naive or AXT UNC th=1", p->name);
+    if (op == AXSPMV_OP_SPMM) {
+        if (!p->rowwise) return fail(AXSPMV_E_UNSUPPORTED, "SpMM: plan %s is not CSR, AXC or AXT UNC th=1", p->name);
+        if (o.k == 0) return fail(AXSPMV_E_INVALID, "SpMM: k must be > 0");
+        const uint32_t kt = pick_ktile(p, o.k);
+        if (kt != 8 && kt != 16 && kt != 32 && kt != 64) return fail(AXSPMV_E_INVALID, "SpMM: ktile %u not in 8/16/32/64", kt);
+        if ((!o.X && p->ncols) || (!o.Y && p->nrows)) return fail(AXSPMV_E_INVALID, "SpMM: NULL operand");
+    }
+    if (op > AXSPMV_OP_SPMM) return fail(AXSPMV_E_INVALID, "unknown operation %u", op);
+    return AXSPMV_OK;
+}
+
+}  // namespace axspmv
+
+extern "C" int axspmv_adjoint(axspmv_plan *p, const double *d_ybar, double *d_xbar, void *stream) {
+    axspmv_operands o{};
+    o.ybar = d_ybar;
+    o.xbar = d_xbar;
+    AX_TRY(plan_op_check(p, AXSPMV_OP_ADJOINT, o));
+    const hipError_t e = plan_op(p, AXSPMV_OP_ADJOINT, o, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "adjoint launch");
+}
+
+extern "C" int axspmv_vbar(axspmv_plan *p, const double *d_ybar, const double *d_x, double *d_vbar, void *stream) {
+    axspmv_operands o{};
+    o.ybar = d_ybar;
+    o.x = d_x;
+    o.vbar = d_vbar;
+    AX_TRY(plan_op_check(p, AXSPMV_OP_VBAR, o));
+    const hipError_t e = plan_op(p, AXSPMV_OP_VBAR, o, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "vbar launch");
+}
+
+extern "C" int axspmv_fwd_adj(axspmv_plan *p, const double *d_x, const double *d_b, double *d_y, double *d_r,
+                              double *d_xbar, double *d_vbar, int fused, void *stream) {
+    axspmv_operands o{};
+    o.x = d_x;
+    o.b = d_b;
+    o.y = d_y;
+    o.r = d_r;
+    o.xbar = d_xbar;
+    o.vbar = d_vbar;
+    const uint32_t op = fused ? AXSPMV_OP_FWD_ADJ : AXSPMV_OP_FWD_ADJ_SPLIT;
+    AX_TRY(plan_op_check(p, op, o));
+    const hipError_t e = plan_op(p, op, o, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "fwd_adj launch");
+}
+
+extern "C" int axspmv_spmm(axspmv_plan *p, uint32_t k, const double *d_X, double *d_Y, void *stream) {
+    axspmv_operands o{};
+    o.k = k;
+    o.X = d_X;
+    o.Y = d_Y;
+    AX_TRY(plan_op_check(p, AXSPMV_OP_SPMM, o));
+    const hipError_t e = plan_op(p, AXSPMV_OP_SPMM, o, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "spmm launch");
 }

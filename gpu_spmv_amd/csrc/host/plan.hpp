@@ -32,6 +32,16 @@ struct axspmv_plan {
     axspmv::launch::K1Args k1{};
     axspmv::launch::AxcArgs axc{};
     axspmv::launch::ComArgs com{};
+    // Row-contiguous layouts (CSR, AXT UNC th == 1, AXC): CSR row pointer and
+    // first plane slot of each row, for SpMM and the value gradient.
+    uint32_t *d_rowptr = nullptr, *d_first = nullptr;
+    axspmv::launch::SpmmArgs spmm{};
+    bool rowwise = false;
+    // AXT UNC th == 1: blocks of <= 64 units for the fused adjoint pass
+    axspmv::launch::Block *d_fblocks = nullptr;
+    uint32_t nfblocks = 0, nunits = 0;
+    // A^T in the same format (AXSPMV_PLAN_TRANSPOSE)
+    axspmv_plan *T = nullptr;
 
     std::vector<void *> bufs;
     void *dalloc(size_t bytes);
@@ -43,5 +53,9 @@ namespace axspmv {
 using Plan = axspmv_plan;
 int hip_fail(hipError_t e, const char *what);
 hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s);
+// One operation of enum axspmv_op; plan_op_check validates it first.
+hipError_t plan_op(axspmv_plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s);
+int plan_op_check(const axspmv_plan *p, uint32_t op, const axspmv_operands &o);
+int plan_add_transpose(axspmv_plan *p, const axspmv_csr *a, hipStream_t s);
 std::vector<launch::Block> row_blocks(const std::vector<uint32_t> &units_per_row, uint32_t budget);
 }  // namespace axspmv

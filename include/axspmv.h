@@ -225,9 +225,11 @@ typedef struct axspmv_opts {
     uint32_t reorder; /* K1: 1 = sort rows by length (default), 0 = natural */
     uint32_t vtype;   /* enum axspmv_vtype: value storage */
     uint32_t block;   /* threads per workgroup (0 = kernel default) */
-    uint32_t budget;  /* slots per wave block for segmented kernels (0 = auto) */
+    uint32_t budget;  /* units per wave block for segmented kernels (0 = auto) */
     uint32_t flags;   /* AXSPMV_PLAN_* */
-    uint32_t reserved[3];
+    uint32_t ktile;   /* SpMM k-tile: right-hand sides per lane group, 8/16/32/64
+                         (0 = smallest of those >= min(k, 64)) */
+    uint32_t reserved[2];
 } axspmv_opts;
 
 #define AXSPMV_PLAN_TRANSPOSE 1u /* also build A^T (needed by the adjoint) */
@@ -261,6 +263,29 @@ int axspmv_plan_get_info(const axspmv_plan *p, axspmv_plan_info *info);
 /* y = A x.  y is fully overwritten (no zeroing needed by the caller). */
 int axspmv_spmv(axspmv_plan *p, const double *d_x, double *d_y, void *stream);
 
+/* ------------------------------------------------- adjoint (gradients) */
+/* x_bar = A^T y_bar, run by the same kernel on a precomputed A^T in the same
+ * format (plan created with AXSPMV_PLAN_TRANSPOSE): no atomics.
+ * No reference counterpart (the reference computes forward products only). */
+int axspmv_adjoint(axspmv_plan *p, const double *d_ybar, double *d_xbar, void *stream);
+/* Per-nonzero value gradient v_bar[k] = y_bar[row(k)] * x[col(k)], written
+ * in the CSR order of the matrix the plan was built from.  Plans: CSR naive
+ * and AXT UNC th == 1. */
+int axspmv_vbar(axspmv_plan *p, const double *d_ybar, const double *d_x, double *d_vbar, void *stream);
+/* Forward + adjoint of the loss 0.5*||A x - b||^2: y = A x, r = y - b,
+ * x_bar = A^T r, v_bar[k] = r[row(k)] * x[col(k)].  fused = 1: one sweep over
+ * A produces y, r and v_bar, then one sweep over A^T; fused = 0: SpMV,
+ * residual, adjoint and v_bar as four separate kernels.  Needs
+ * AXSPMV_PLAN_TRANSPOSE; plans: CSR naive and AXT UNC th == 1. */
+int axspmv_fwd_adj(axspmv_plan *p, const double *d_x, const double *d_b, double *d_y, double *d_r,
+                   double *d_xbar, double *d_vbar, int fused, void *stream);
+
+/* ---------------------------------------------------------------- SpMM */
+/* Y = A X for k right-hand sides, X (ncols x k) and Y (nrows x k) row-major
+ * ("interleaved": X[c*k + j]).  Every value/index load feeds ktile FMAs.
+ * Plans: CSR, AXC and AXT UNC th == 1.  No reference counterpart. */
+int axspmv_spmm(axspmv_plan *p, uint32_t k, const double *d_X, double *d_Y, void *stream);
+
 /* ---------------------------------------------------------- timing */
 typedef struct axspmv_timing {
     double mean_ms, median_ms, min_ms, max_ms; /* per operation */
@@ -277,6 +302,24 @@ typedef struct axspmv_timing {
 int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, uint32_t iters,
                      uint32_t warmup, uint32_t mode, uint32_t graph_batch, void *stream,
                      axspmv_timing *out);
+
+/* Operations axspmv_time_op can time, and their operands. */
+enum axspmv_op {
+    AXSPMV_OP_SPMV = 0,          /* y = A x                          */
+    AXSPMV_OP_ADJOINT = 1,       /* xbar = A^T ybar                  */
+    AXSPMV_OP_VBAR = 2,          /* vbar = ybar[row] * x[col]        */
+    AXSPMV_OP_FWD_ADJ = 3,       /* axspmv_fwd_adj, fused            */
+    AXSPMV_OP_FWD_ADJ_SPLIT = 4, /* axspmv_fwd_adj, four kernels     */
+    AXSPMV_OP_SPMM = 5           /* Y = A X, k right-hand sides      */
+};
+typedef struct axspmv_operands {
+    const double *x, *b, *ybar, *X;
+    double *y, *r, *xbar, *vbar, *Y;
+    uint32_t k, pad_;
+} axspmv_operands;
+int axspmv_time_op(axspmv_plan *p, uint32_t op, const axspmv_operands *ops, uint32_t iters,
+                   uint32_t warmup, uint32_t mode, uint32_t graph_batch, void *stream,
+                   axspmv_timing *out);
 
 /* ---------------------------------------------------- device helpers */
 int axspmv_dev_alloc(size_t bytes, void **ptr);

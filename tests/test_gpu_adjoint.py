@@ -1,0 +1,181 @@
+"""GPU: adjoint x_bar = A^T y_bar, value gradient v_bar, the fused forward +
+adjoint pass, SpMM, and the autograd wrappers.
+
+Checks: oracle (ncsr on A^T, vbar, spmm) at 1e-12; v_bar is one product per
+entry, so bit-exact; <A x, w> = <x, A^T w>; central finite differences
+(h = 1e-6) and a dense FP64 gradient on a 4,096-row random matrix of about
+10 entries per row, values N(0,1).
+"""
+import numpy as np
+import pytest
+
+import gpu_spmv_amd as ax
+from conftest import rel_err
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+ADJ_KERNELS = [
+    ("unc_h1_thw4", ax.FMT_AXT, dict(thw=4, th=1)),
+    ("unc_h1_thw8", ax.FMT_AXT, dict(thw=8, th=1)),
+    ("unc_h1_thw2", ax.FMT_AXT, dict(thw=2, th=1)),
+    ("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+]
+
+
+def mats():
+    rng = np.random.default_rng(5)
+    yield "random4096", ax.random_sparse(4096, 5, 15, seed=8)
+    yield "banded_empty", ax.banded(3000, 3, 0.3, seed=6)
+    yield "stencil16p", ax.stencil27(16, permute=True, seed=4)
+    # long rows (more than 64 units: the fused kernel's second sweep)
+    n = 2000
+    rl = rng.integers(0, 8, n)
+    rl[[7, 900, 1999]] = [700, 1500, 300]
+    row = np.concatenate([[0], np.cumsum(rl)]).astype(np.uint32)
+    col = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in rl]).astype(np.uint32)
+    yield "long_rows", ax.CsrMatrix(row, col, rng.standard_normal(col.size), ncols=n)
+
+
+MATS = dict(mats())
+
+
+@pytest.mark.parametrize("mname", list(MATS))
+@pytest.mark.parametrize("kname,fmt,kw", ADJ_KERNELS, ids=[k[0] for k in ADJ_KERNELS])
+def test_adjoint_vbar_fused(mname, kname, fmt, kw):
+    m = MATS[mname]
+    n, nc = m.nrows, m.ncols
+    x = ax.uniform(21, nc)
+    b = ax.uniform(22, n, stream=ax.VEC_STREAM + 1)
+    w = ax.uniform(23, n)
+    p = ax.Plan(m, fmt, flags=ax.PLAN_TRANSPOSE, **kw)
+    dx, db, dw = (ax.DeviceBuffer.from_array(v) for v in (x, b, w))
+    dxb, dvb = ax.DeviceBuffer(8 * nc), ax.DeviceBuffer(8 * max(m.nnz, 1))
+    p.adjoint(dw, dxb)
+    p.vbar(dw, dx, dvb)
+    xb = dxb.download(np.float64, nc)
+    assert rel_err(orc.ncsr_t(m, w, nc), xb) <= TOL
+    assert np.array_equal(dvb.download(np.float64, m.nnz), orc.vbar(m, w, x))
+    y_ref = orc.ncsr(m, x)
+    assert abs(np.dot(y_ref, w) - np.dot(x, xb)) <= 1e-11 * (np.abs(y_ref).sum() * np.abs(w).max() + 1)
+    # fused and split forward + adjoint
+    r_ref = y_ref - b
+    for fused in (True, False):
+        outs = [ax.DeviceBuffer.from_array(np.full(s, np.nan)) for s in (n, n, nc, max(m.nnz, 1))]
+        p.fwd_adj(dx, db, *outs, fused=fused)
+        y, r, xbar, vbar = (o.download(np.float64, s) for o, s in zip(outs, (n, n, nc, m.nnz)))
+        assert rel_err(y_ref, y) <= TOL
+        assert rel_err(r_ref, r) <= TOL
+        assert rel_err(orc.ncsr_t(m, r_ref, nc), xbar) <= 1e-11
+        assert rel_err(orc.vbar(m, r_ref, x), vbar) <= 1e-11
+        assert np.array_equal(vbar, orc.vbar(m, r, x))  # v_bar is r[row] * x[col] of the r it wrote
+
+
+def test_gradients_finite_differences_and_dense():
+    """Config 5 gradient check: 4,096 rows, ~10 entries per row, N(0,1)."""
+    m = ax.random_sparse(4096, 5, 15, seed=8)
+    x = ax.uniform(31, m.ncols)
+    b = ax.uniform(32, m.nrows, stream=ax.VEC_STREAM + 1)
+    p = ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_TRANSPOSE)
+    dx, db = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer.from_array(b)
+    n, nc, nnz = m.nrows, m.ncols, m.nnz
+    outs = [ax.DeviceBuffer(8 * s) for s in (n, n, nc, nnz)]
+    p.fwd_adj(dx, db, *outs, fused=True)
+    _, r, xbar, vbar = (o.download(np.float64, s) for o, s in zip(outs, (n, n, nc, nnz)))
+    # dense FP64 reference gradient
+    A = m.to_dense()
+    rd = A @ x - b
+    assert rel_err(A.T @ rd, xbar) < 1e-12
+    assert rel_err(rd[m.row_ids()] * x[m.col], vbar) < 1e-12
+
+    def loss(xv, vals):
+        mm = ax.CsrMatrix(m.row, m.col, vals, ncols=nc)
+        y = orc.ncsr(mm, xv)
+        return 0.5 * float(np.dot(y - b, y - b))
+
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    for i in rng.choice(nc, 24, replace=False):
+        e = np.zeros(nc)
+        e[i] = h
+        fd = (loss(x + e, m.val) - loss(x - e, m.val)) / (2 * h)
+        assert abs(fd - xbar[i]) <= 2e-5 * max(1.0, abs(xbar[i])), (i, fd, xbar[i])
+    for k in rng.choice(nnz, 24, replace=False):
+        e = np.zeros(nnz)
+        e[k] = h
+        fd = (loss(x, m.val + e) - loss(x, m.val - e)) / (2 * h)
+        assert abs(fd - vbar[k]) <= 2e-5 * max(1.0, abs(vbar[k])), (k, fd, vbar[k])
+
+
+@pytest.mark.timeout(900)  # the first import of torch on a fresh machine is slow
+def test_autograd_wrappers():
+    torch = pytest.importorskip("torch")
+    from gpu_spmv_amd import autograd as ag
+    m = ax.random_sparse(500, 2, 9, seed=3)
+    vals = torch.tensor(m.val, requires_grad=True)
+    x = torch.tensor(ax.uniform(41, m.ncols), requires_grad=True)
+    b = torch.tensor(ax.uniform(42, m.nrows))
+    # spmv: gradients against a dense torch reference
+    y = ag.spmv(vals, x, m, thw=4)
+    g = torch.tensor(ax.uniform(43, m.nrows))
+    (y * g).sum().backward()
+    A = torch.tensor(m.to_dense())
+    assert torch.allclose(y, A @ x.detach(), rtol=0, atol=1e-12)
+    assert torch.allclose(x.grad, A.T @ g, rtol=0, atol=1e-12)
+    rows = torch.tensor(m.row_ids().astype(np.int64))
+    cols = torch.tensor(m.col.astype(np.int64))
+    assert torch.allclose(vals.grad, g[rows] * x.detach()[cols], rtol=0, atol=1e-13)
+    # least squares, fused and split
+    for fused in (True, False):
+        vals.grad = None
+        x.grad = None
+        loss = ag.least_squares(vals, x, b, m, fused=fused, thw=4)
+        loss.backward()
+        r = A @ x.detach() - b
+        assert abs(loss.item() - 0.5 * float(r @ r)) <= 1e-12 * (1 + loss.item())
+        assert torch.allclose(x.grad, A.T @ r, rtol=0, atol=1e-11)
+        assert torch.allclose(vals.grad, r[rows] * x.detach()[cols], rtol=0, atol=1e-12)
+
+
+SPMM_CASES = [(1, 0), (3, 0), (8, 8), (8, 0), (32, 32), (32, 8), (64, 64), (64, 16), (20, 0), (96, 64)]
+
+
+@pytest.mark.parametrize("mname", ["random4096", "long_rows", "banded_empty"])
+@pytest.mark.parametrize("k,kt", SPMM_CASES)
+@pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+                                    (ax.FMT_AXC, dict(hbs=8))], ids=["axt", "csr", "axc"])
+def test_spmm(mname, k, kt, fmt, kw):
+    m = MATS[mname]
+    X = ax.uniform(50 + k, m.ncols * k).reshape(m.ncols, k)
+    ref = orc.spmm(m, X)
+    p = ax.Plan(m, fmt, ktile=kt, **kw)
+    dX = ax.DeviceBuffer.from_array(X)
+    dY = ax.DeviceBuffer.from_array(np.full(m.nrows * k, np.nan))
+    p.spmm(k, dX, dY)
+    Y = dY.download(np.float64, m.nrows * k).reshape(m.nrows, k)
+    assert rel_err(ref, Y) <= TOL
+
+
+def test_spmm_f32_values():
+    m = MATS["random4096"]
+    k = 32
+    X = ax.uniform(60, m.ncols * k).reshape(m.ncols, k)
+    m32 = ax.CsrMatrix(m.row, m.col, m.val.astype(np.float32).astype(np.float64), ncols=m.ncols)
+    p = ax.Plan(m, ax.FMT_AXT, thw=4, vtype=ax.F32)
+    dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * m.nrows * k)
+    p.spmm(k, dX, dY)
+    assert rel_err(orc.spmm(m32, X), dY.download(np.float64, m.nrows * k).reshape(m.nrows, k)) <= TOL
+
+
+def test_unsupported_combinations_raise():
+    m = MATS["random4096"]
+    p = ax.Plan(m, ax.FMT_AXT, thw=4)  # no transpose
+    d = ax.DeviceBuffer(8 * m.nrows)
+    with pytest.raises(ax.AxspmvError):
+        p.adjoint(d, d)
+    pk = ax.Plan(m, ax.FMT_K1, chunk=64)
+    with pytest.raises(ax.AxspmvError):
+        pk.spmm(8, d, d)
+    with pytest.raises(ax.AxspmvError):
+        pk.vbar(d, d, d)
