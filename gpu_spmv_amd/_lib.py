@@ -81,6 +81,11 @@ class PlanInfo(C.Structure):
                 ("alg_bytes", u64), ("beta", f64), ("launches", u32), ("nblocks", u32)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("nrows", u64), ("ncols", u64), ("nnz", u64), ("slots", u64), ("alg_bytes", u64),
+                ("device_bytes", u64), ("count", u32), ("nitems", u32), ("nwaves", u32), ("pad_", u32)]
+
+
 class Timing(C.Structure):
     _fields_ = [("mean_ms", f64), ("median_ms", f64), ("min_ms", f64), ("max_ms", f64),
                 ("iters", u32), ("mode", u32)]
@@ -133,6 +138,17 @@ SIGNATURES = {
     "axspmv_spmm": (C.c_int, [vp, u32, vp, vp, vp]),
     "axspmv_time_spmv": (C.c_int, [vp, vp, vp, u32, u32, u32, u32, vp, P(Timing)]),
     "axspmv_time_op": (C.c_int, [vp, u32, P(Operands), u32, u32, u32, u32, vp, P(Timing)]),
+    "axspmv_batch_create": (C.c_int, [P(P(Csr)), u32, P(Opts), C.c_int, vp, P(vp)]),
+    "axspmv_batch_destroy": (C.c_int, [vp]),
+    "axspmv_batch_get_info": (C.c_int, [vp, P(BatchInfo)]),
+    "axspmv_batch_spmv": (C.c_int, [vp, vp, vp, vp]),
+    "axspmv_batch_time": (C.c_int, [vp, vp, vp, u32, u32, u32, u32, vp, P(Timing)]),
+    "axspmv_plan_graph": (C.c_int, [vp, u32, P(Operands), u32, P(vp)]),
+    "axspmv_batch_graph": (C.c_int, [vp, vp, vp, u32, P(vp)]),
+    "axspmv_graph_launch": (C.c_int, [vp, vp]),
+    "axspmv_graph_destroy": (C.c_int, [vp]),
+    "axspmv_autotune": (C.c_int, [P(Csr), u32, u32, P(Opts), u32, u32, vp, P(u32), P(f64)]),
+    "axspmv_autotune_batch": (C.c_int, [P(P(Csr)), u32, P(Opts), u32, u32, vp, P(Opts), P(u32)]),
     "axspmv_dev_alloc": (C.c_int, [C.c_size_t, P(vp)]),
     "axspmv_dev_free": (C.c_int, [vp]),
     "axspmv_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t, vp]),

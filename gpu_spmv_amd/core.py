@@ -349,3 +349,109 @@ def spmv_host(plan: Plan, x: np.ndarray) -> np.ndarray:
     plan.spmv(dx, dy)
     L.call("axspmv_device_sync")
     return dy.download(np.float64, plan.nrows)
+
+
+# ----------------------------------------------------------------- batches
+class Batch:
+    """Many AXT UNC matrices, SpMV in one persistent launch (axspmv_batch).
+    x / y are the concatenations of the matrices' vectors in batch order."""
+
+    def __init__(self, mats, opts_list=None, **kw):
+        self.mats = list(mats)
+        n = len(self.mats)
+        arr = (C.POINTER(L.Csr) * n)(*[C.pointer(m._s) for m in self.mats])
+        if opts_list is None:
+            o = (L.Opts * 1)(opts(L.FMT_AXT, **kw))
+            per = 0
+        else:
+            o = (L.Opts * n)(*opts_list)
+            per = 1
+        p = C.c_void_p()
+        L.call("axspmv_batch_create", arr, n, o, per, None, C.byref(p))
+        self.ptr = p.value
+        self.xoff = np.concatenate([[0], np.cumsum([m.ncols for m in self.mats])]).astype(np.int64)
+        self.yoff = np.concatenate([[0], np.cumsum([m.nrows for m in self.mats])]).astype(np.int64)
+
+    def info(self) -> dict:
+        i = L.BatchInfo()
+        L.call("axspmv_batch_get_info", self.ptr, C.byref(i))
+        return {f: getattr(i, f) for f, _ in L.BatchInfo._fields_ if f != "pad_"}
+
+    def spmv(self, dx, dy, stream=None) -> None:
+        L.call("axspmv_batch_spmv", self.ptr, _dptr(dx), _dptr(dy), stream)
+
+    def time(self, dx, dy, iters=50, warmup=5, mode=TIME_REGION, batch=1, stream=None) -> TimingResult:
+        t = L.Timing()
+        L.call("axspmv_batch_time", self.ptr, _dptr(dx), _dptr(dy), iters, warmup, mode, batch, stream, C.byref(t))
+        return TimingResult(t.mean_ms, t.median_ms, t.min_ms, t.max_ms, t.iters, t.mode)
+
+    def close(self) -> None:
+        if self.ptr:
+            L.load().axspmv_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(m: CsrMatrix, op: int = L.OP_SPMV, k: int = 0, candidates=None, iters: int = 20):
+    """Index of the fastest candidate (list of Opts, or None for the built-in
+    set) and each candidate's time in ms (-1 = rejected)."""
+    n = len(candidates) if candidates is not None else 0
+    cand = (L.Opts * n)(*candidates) if candidates is not None else None
+    times = np.full(max(n, 16), -1.0)
+    best = C.c_uint32()
+    L.call("axspmv_autotune", m.ref, op, k, cand, n, iters, None, C.byref(best), _ptr(times, L.f64))
+    return best.value, times[:n] if n else times
+
+
+def autotune_batch(mats, candidates=None, iters: int = 20):
+    """Per-matrix options for a Batch; returns (list of Opts, number of groups)."""
+    mats = list(mats)
+    n = len(mats)
+    arr = (C.POINTER(L.Csr) * n)(*[C.pointer(m._s) for m in mats])
+    nc = len(candidates) if candidates is not None else 0
+    cand = (L.Opts * nc)(*candidates) if candidates is not None else None
+    out = (L.Opts * n)()
+    ng = C.c_uint32()
+    L.call("axspmv_autotune_batch", arr, n, cand, nc, iters, None, out, C.byref(ng))
+    return list(out), ng.value
+
+
+class Graph:
+    """A captured hipGraph of repeated launches (axspmv_graph)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    @classmethod
+    def of_plan(cls, plan: Plan, op: int, repeats: int = 1, **operands) -> "Graph":
+        o = L.Operands()
+        for name, v in operands.items():
+            setattr(o, name, v if name == "k" else _dptr(v))
+        p = C.c_void_p()
+        L.call("axspmv_plan_graph", plan.ptr, op, C.byref(o), repeats, C.byref(p))
+        return cls(p.value)
+
+    @classmethod
+    def of_batch(cls, batch: Batch, dx, dy, repeats: int = 1) -> "Graph":
+        p = C.c_void_p()
+        L.call("axspmv_batch_graph", batch.ptr, _dptr(dx), _dptr(dy), repeats, C.byref(p))
+        return cls(p.value)
+
+    def launch(self, stream=None) -> None:
+        L.call("axspmv_graph_launch", self.ptr, stream)
+
+    def close(self) -> None:
+        if self.ptr:
+            L.load().axspmv_graph_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

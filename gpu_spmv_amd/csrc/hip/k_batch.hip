@@ -1,0 +1,60 @@
+// Batched SpMV: many matrices of different sizes in one persistent launch.
+//
+// Every matrix is stored in the AXT UNC layout its own tile parameters call
+// for (tile half width / height chosen per matrix), all planes concatenated
+// in one device buffer; x and y of the batch are concatenated too.  The host
+// cuts every matrix into row-aligned wave blocks and lists all blocks of the
+// batch longest first.  A persistent grid (a fixed number of waves sized to
+// the chip) walks that list with a grid stride -- every wave leaves after
+// its last item -- and runs the same row-block engine as the single-matrix
+// kernels on each item.
+#include "rowblock.hpp"
+
+namespace {
+using namespace axspmv::dev;
+using axspmv::launch::BatchArgs;
+using axspmv::launch::BatchItem;
+using axspmv::launch::BatchMat;
+
+template <class VT>
+__global__ void __launch_bounds__(256) batch_kernel(BatchArgs a, const double *__restrict__ x, double *__restrict__ y) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = wave * 64 + lane; i < a.nempty; i += nwaves * 64) y[a.empty[i]] = 0.0;
+    const VT *vbase = static_cast<const VT *>(a.vplane);
+    for (uint32_t it = wave; it < a.nitems; it += nwaves) {
+        const BatchItem w = a.items[it];
+        const BatchMat m = a.mats[w.mat];
+        const VT *vp = vbase + m.voff;
+        const uint32_t *cp = a.cplane + m.voff;
+        const uint32_t *sec = a.sec + m.soff;
+        const double *xm = x + m.xoff;
+        double *ym = y + m.yoff;
+        const Block b{w.beg, w.end};
+        switch (m.kind) {
+            case 0: rowblock_process(UncH1Src<VT, 4>{vp, cp, sec, m.ushift}, b, xm, ym); break;
+            case 1: rowblock_process(UncH1Src<VT, 2>{vp, cp, sec, 0}, b, xm, ym); break;
+            case 2: rowblock_process(UncH1Src<VT, 1>{vp, cp, sec, 0}, b, xm, ym); break;
+            default: rowblock_process(UncHSrc<VT, 0>{vp, cp, sec, m.thw_log, m.th}, b, xm, ym); break;
+        }
+    }
+}
+
+}  // namespace
+
+namespace axspmv {
+namespace launch {
+
+hipError_t batch(const BatchArgs &a, const double *x, double *y, hipStream_t s) {
+    const uint32_t waves = a.nwaves ? a.nwaves : 1;
+    const unsigned grid = (waves + 3) / 4;
+    if (a.vtype == 1)
+        batch_kernel<float><<<grid, 256, 0, s>>>(a, x, y);
+    else
+        batch_kernel<double><<<grid, 256, 0, s>>>(a, x, y);
+    return hipGetLastError();
+}
+
+}  // namespace launch
+}  // namespace axspmv

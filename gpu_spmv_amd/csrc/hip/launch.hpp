@@ -57,6 +57,29 @@ struct SpmmArgs {
 // kt (k-tile) in {8, 16, 32, 64}
 hipError_t spmm(const SpmmArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s);
 
+// Batched persistent SpMV over AXT UNC matrices.
+struct BatchMat {
+    uint64_t voff;      // first slot of the matrix in the concatenated planes
+    uint64_t soff;      // first entry of its sec[] in the concatenated sec
+    uint32_t xoff, yoff;  // offsets of its x / y in the concatenated vectors
+    uint32_t kind;      // 0/1/2: th == 1 with 4/2/1 slots per unit; 3: th > 1
+    uint32_t ushift;    // th == 1: log2(thw / slots per unit)
+    uint32_t thw_log, th;
+};
+struct BatchItem {
+    uint32_t mat, beg, end, pad;
+};
+struct BatchArgs {
+    const void *vplane;
+    const uint32_t *cplane, *sec;
+    const BatchMat *mats;
+    const BatchItem *items;
+    uint32_t nitems, nwaves, vtype;
+    const uint32_t *empty;  // global y indices of rows without entries
+    uint32_t nempty;
+};
+hipError_t batch(const BatchArgs &a, const double *x, double *y, hipStream_t s);
+
 hipError_t csr_naive(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,
                      const double *x, double *y, uint32_t block, hipStream_t s);
 

@@ -1,0 +1,182 @@
+// Row-aligned wave-block engine shared by the AXT UNC kernels and the
+// batched persistent kernel.
+//
+// A wave owns a contiguous range of units [b.beg, b.end) that starts and ends
+// on row boundaries, and walks it 64 units at a time.  Each lane sums its
+// unit; lane sums are combined across lanes by a DPP segmented scan keyed on
+// the row id; the last lane of each row segment stores y.  A segment still
+// open at lane 63 is carried to the next 64 units in scalar registers.
+#pragma once
+
+#include "launch.hpp"
+#include "wave64.hpp"
+
+namespace axspmv {
+namespace dev {
+using axspmv::launch::Block;
+
+template <class VT, int K>
+struct VecLoad;
+template <>
+struct VecLoad<double, 4> {
+    __device__ static void get(const double *p, double *v) {
+        const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+};
+template <>
+struct VecLoad<double, 2> {
+    __device__ static void get(const double *p, double *v) {
+        const double2 a = *reinterpret_cast<const double2 *>(p);
+        v[0] = a.x; v[1] = a.y;
+    }
+};
+template <>
+struct VecLoad<double, 1> {
+    __device__ static void get(const double *p, double *v) { v[0] = *p; }
+};
+template <>
+struct VecLoad<float, 4> {
+    __device__ static void get(const float *p, float *v) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+};
+template <>
+struct VecLoad<float, 2> {
+    __device__ static void get(const float *p, float *v) {
+        const float2 a = *reinterpret_cast<const float2 *>(p);
+        v[0] = a.x; v[1] = a.y;
+    }
+};
+template <>
+struct VecLoad<float, 1> {
+    __device__ static void get(const float *p, float *v) { v[0] = *p; }
+};
+template <int K>
+__device__ __forceinline__ void load_cols(const uint32_t *p, uint32_t *c) {
+    if constexpr (K == 4) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(p);
+        c[0] = a.x; c[1] = a.y; c[2] = a.z; c[3] = a.w;
+    } else if constexpr (K == 2) {
+        const uint2 a = *reinterpret_cast<const uint2 *>(p);
+        c[0] = a.x; c[1] = a.y;
+    } else {
+        c[0] = *p;
+    }
+}
+
+// th == 1: unit u covers plane slots [u*K, u*K + K), tile (u*K)/thw.
+template <class VT, int K>
+struct UncH1Src {
+    const VT *__restrict__ vp;
+    const uint32_t *__restrict__ cp;
+    const uint32_t *__restrict__ sec;
+    uint32_t ushift;  // log2(thw / K)
+    __device__ __forceinline__ double sum(uint32_t u, const double *__restrict__ x) const {
+        const size_t q = size_t(u) * K;
+        VT v[K];
+        uint32_t c[K];
+        VecLoad<VT, K>::get(vp + q, v);
+        load_cols<K>(cp + q, c);
+        double xv[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) xv[j] = x[c[j]];
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc = fma(double(v[j]), xv[j], acc);
+        return acc;
+    }
+    __device__ __forceinline__ uint32_t row(uint32_t u) const { return sec[u >> ushift]; }
+    // As sum(), keeping the gathered x values.
+    __device__ __forceinline__ double sum_keep(uint32_t u, const double *__restrict__ x, double *xv) const {
+        const size_t q = size_t(u) * K;
+        VT v[K];
+        uint32_t c[K];
+        VecLoad<VT, K>::get(vp + q, v);
+        load_cols<K>(cp + q, c);
+#pragma unroll
+        for (int j = 0; j < K; ++j) xv[j] = x[c[j]];
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc = fma(double(v[j]), xv[j], acc);
+        return acc;
+    }
+    __device__ __forceinline__ void gather_x(uint32_t u, const double *__restrict__ x, double *xv) const {
+        uint32_t c[K];
+        load_cols<K>(cp + size_t(u) * K, c);
+#pragma unroll
+        for (int j = 0; j < K; ++j) xv[j] = x[c[j]];
+    }
+};
+
+// th > 1: unit g is global tile column g; level f sits at
+// (g/thw)*thw*th + f*thw + g%thw.  TH > 0 fixes th at compile time.
+template <class VT, int TH>
+struct UncHSrc {
+    const VT *__restrict__ vp;
+    const uint32_t *__restrict__ cp;
+    const uint32_t *__restrict__ sec;
+    uint32_t thw_log, th;
+    __device__ __forceinline__ double sum(uint32_t g, const double *__restrict__ x) const {
+        const uint32_t thw = 1u << thw_log;
+        const uint32_t h = TH > 0 ? uint32_t(TH) : th;
+        const size_t base = size_t(g >> thw_log) * (size_t(h) << thw_log) + (g & (thw - 1));
+        double acc = 0.0;
+        if constexpr (TH > 0) {
+            double xv[TH];
+            VT v[TH];
+#pragma unroll
+            for (int f = 0; f < TH; ++f) {
+                v[f] = vp[base + size_t(f) * thw];
+                xv[f] = x[cp[base + size_t(f) * thw]];
+            }
+#pragma unroll
+            for (int f = 0; f < TH; ++f) acc = fma(double(v[f]), xv[f], acc);
+        } else {
+#pragma unroll 4
+            for (uint32_t f = 0; f < h; ++f) acc = fma(double(vp[base + size_t(f) * thw]), x[cp[base + size_t(f) * thw]], acc);
+        }
+        return acc;
+    }
+    __device__ __forceinline__ uint32_t row(uint32_t g) const { return sec[g]; }
+};
+
+template <class Src>
+__device__ __forceinline__ void rowblock_process(const Src &src, const Block b, const double *__restrict__ x,
+                                                 double *__restrict__ y) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t crow = NOROW;  // row of the segment still open at the end of the previous 64 units
+    double carry = 0.0;
+    for (uint32_t u0 = b.beg; u0 < b.end; u0 += 64) {
+        const uint32_t u = u0 + lane;
+        const bool act = u < b.end;
+        double s = 0.0;
+        uint32_t r = NOROW;
+        if (act) {
+            s = src.sum(u, x);
+            r = src.row(u);
+        }
+        // The carried segment ends here if these units start with another row.
+        const uint32_t r0 = readfirst_u32(r);
+        if (crow != NOROW && r0 != crow) {
+            if (lane == 0) y[crow] = carry;
+            crow = NOROW;
+            carry = 0.0;
+        }
+        uint32_t head = r != shift_up_u32(r, crow);
+        double v = seg_scan_f64(s, head);
+        if (!head) v += carry;  // still inside the carried row
+        const bool last = u0 + 64 >= b.end;
+        bool end = act && shift_down_u32(r, NOROW) != r;
+        if (!last && lane == 63) end = false;
+        if (end) y[r] = v;
+        if (!last) {
+            crow = readlane_u32(r, 63);
+            carry = readlane_f64(v, 63);
+        }
+    }
+}
+
+}  // namespace dev
+}  // namespace axspmv

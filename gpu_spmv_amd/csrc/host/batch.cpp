@@ -1,0 +1,299 @@
+// Batched plans (many matrices, one persistent launch) and the autotuner.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <tuple>
+
+#include "plan.hpp"
+
+using namespace axspmv;
+
+struct axspmv_batch {
+    std::vector<void *> bufs;
+    launch::BatchArgs args{};
+    axspmv_batch_info info{};
+    void *dalloc(size_t bytes) {
+        void *p = nullptr;
+        if (bytes == 0) bytes = 256;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            fail(AXSPMV_E_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+            return nullptr;
+        }
+        bufs.push_back(p);
+        info.device_bytes += bytes;
+        return p;
+    }
+    ~axspmv_batch() {
+        for (void *p : bufs) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+// Waves of the persistent grid: 16 per CU.
+uint32_t persistent_waves() {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    return uint32_t(cus) * 16;
+}
+
+int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (!bytes) return AXSPMV_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "batch upload");
+}
+
+}  // namespace
+
+extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *opts,
+                                   int per_matrix, void *stream, axspmv_batch **out) {
+    if (!mats || !opts || !out || count == 0) return fail(AXSPMV_E_INVALID, "batch_create: bad argument");
+    *out = nullptr;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto b = std::make_unique<axspmv_batch>();
+    const uint32_t vtype = opts[0].vtype;
+    const size_t vs = vtype == AXSPMV_F32 ? 4 : 8;
+    std::vector<launch::BatchMat> desc(count);
+    std::vector<launch::BatchItem> items;
+    std::vector<uint32_t> empty;
+    std::vector<double> vcat;
+    std::vector<float> vcat32;
+    std::vector<uint32_t> ccat, scat;
+    uint64_t xoff = 0, yoff = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const axspmv_csr *a = mats[i];
+        axspmv_opts o = opts[per_matrix ? i : 0];
+        if (!a || !a->row || a->row[a->nrows] != a->nnz) return fail(AXSPMV_E_INVALID, "batch_create: matrix %u inconsistent", i);
+        if (o.format != AXSPMV_FMT_AXT || o.mode != AXSPMV_AXT_UNC)
+            return fail(AXSPMV_E_UNSUPPORTED, "batch_create: matrix %u: batches use the AXT UNC layout", i);
+        if (o.vtype != vtype) return fail(AXSPMV_E_INVALID, "batch_create: one value type per batch");
+        if (o.th == 0) o.th = 1;
+        if (o.thw == 0) o.thw = o.th == 1 ? 4 : 64;
+        if (!is_pow2(o.thw) || o.thw > 64) return fail(AXSPMV_E_UNSUPPORTED, "batch_create: thw must be a power of two <= 64");
+        axspmv_axt *m = nullptr;
+        AX_TRY(axspmv_build_axt(a, nullptr, 256, o.thw, o.th, AXSPMV_AXT_UNC, AXSPMV_BUILD_PLANES, &m, nullptr));
+        std::unique_ptr<axspmv_axt, void (*)(axspmv_axt *)> guard(m, axspmv_axt_free);
+        // keep every matrix's planes 64-slot aligned (vector loads)
+        const uint64_t voff = vtype == AXSPMV_F32 ? vcat32.size() : vcat.size();
+        const uint64_t nslots = m->lenAX / 2;
+        if (vtype == AXSPMV_F32)
+            vcat32.insert(vcat32.end(), m->vplane, m->vplane + nslots);
+        else
+            vcat.insert(vcat.end(), m->vplane, m->vplane + nslots);
+        ccat.insert(ccat.end(), m->cplane, m->cplane + nslots);
+        const uint64_t pad = (64 - nslots % 64) % 64;
+        if (vtype == AXSPMV_F32)
+            vcat32.insert(vcat32.end(), pad, 0.0f);
+        else
+            vcat.insert(vcat.end(), pad, 0.0);
+        ccat.insert(ccat.end(), pad, 0u);
+        const uint64_t soff = scat.size();
+        scat.insert(scat.end(), m->sec, m->sec + m->lenSEC);
+        launch::BatchMat d{};
+        d.voff = voff;
+        d.soff = soff;
+        if (xoff + a->ncols >= (1ull << 32) || yoff + a->nrows >= (1ull << 32))
+            return fail(AXSPMV_E_OVERFLOW, "batch_create: concatenated vectors exceed 32-bit offsets");
+        d.xoff = uint32_t(xoff);
+        d.yoff = uint32_t(yoff);
+        const uint32_t K = o.th == 1 ? std::min<uint32_t>(4, o.thw) : 1;
+        d.kind = o.th == 1 ? (K == 4 ? 0u : K == 2 ? 1u : 2u) : 3u;
+        d.ushift = o.th == 1 ? ilog2(o.thw / K) : 0;
+        d.thw_log = ilog2(o.thw);
+        d.th = o.th;
+        desc[i] = d;
+        std::vector<uint32_t> units(a->nrows);
+        for (uint32_t r = 0; r < a->nrows; ++r) {
+            const uint32_t rl = a->row[r + 1] - a->row[r];
+            if (rl == 0) empty.push_back(uint32_t(yoff) + r);
+            units[r] = o.th == 1 ? uint32_t(ceil_div(rl, o.thw) * (o.thw / K)) : uint32_t(ceil_div(rl, o.th));
+        }
+        for (const launch::Block &bl : row_blocks(units, o.budget ? o.budget : 256))
+            items.push_back({i, bl.beg, bl.end, 0});
+        b->info.nrows += a->nrows;
+        b->info.ncols += a->ncols;
+        b->info.nnz += a->nnz;
+        b->info.slots += nslots;
+        b->info.alg_bytes += (vs + 4) * uint64_t(a->nnz) + 4ull * (a->nrows + 1) + 8ull * a->ncols + 8ull * a->nrows;
+        xoff += a->ncols;
+        yoff += a->nrows;
+    }
+    // longest blocks first across the whole batch
+    std::stable_sort(items.begin(), items.end(), [](const launch::BatchItem &p, const launch::BatchItem &q) {
+        return p.end - p.beg > q.end - q.beg;
+    });
+    const uint64_t nsl = ccat.size();
+    void *dv = b->dalloc(vs * nsl);
+    auto *dc = static_cast<uint32_t *>(b->dalloc(4 * nsl));
+    auto *ds = static_cast<uint32_t *>(b->dalloc(4 * scat.size()));
+    auto *dm = static_cast<launch::BatchMat *>(b->dalloc(sizeof(launch::BatchMat) * desc.size()));
+    auto *di = static_cast<launch::BatchItem *>(b->dalloc(sizeof(launch::BatchItem) * items.size()));
+    auto *de = static_cast<uint32_t *>(b->dalloc(4 * empty.size()));
+    if (!dv || !dc || !ds || !dm || !di || !de) return AXSPMV_E_NOMEM;
+    AX_TRY(h2d(dv, vtype == AXSPMV_F32 ? static_cast<const void *>(vcat32.data()) : vcat.data(), vs * nsl, s));
+    AX_TRY(h2d(dc, ccat.data(), 4 * nsl, s));
+    AX_TRY(h2d(ds, scat.data(), 4 * scat.size(), s));
+    AX_TRY(h2d(dm, desc.data(), sizeof(launch::BatchMat) * desc.size(), s));
+    AX_TRY(h2d(di, items.data(), sizeof(launch::BatchItem) * items.size(), s));
+    AX_TRY(h2d(de, empty.data(), 4 * empty.size(), s));
+    launch::BatchArgs &g = b->args;
+    g.vplane = dv;
+    g.cplane = dc;
+    g.sec = ds;
+    g.mats = dm;
+    g.items = di;
+    g.nitems = uint32_t(items.size());
+    g.nwaves = std::max<uint32_t>(1, std::min<uint32_t>(g.nitems, persistent_waves()));
+    g.vtype = vtype;
+    g.empty = de;
+    g.nempty = uint32_t(empty.size());
+    b->info.count = count;
+    b->info.nitems = g.nitems;
+    b->info.nwaves = g.nwaves;
+    *out = b.release();
+    return AXSPMV_OK;
+}
+
+extern "C" int axspmv_batch_destroy(axspmv_batch *b) {
+    delete b;
+    return AXSPMV_OK;
+}
+
+extern "C" int axspmv_batch_get_info(const axspmv_batch *b, axspmv_batch_info *info) {
+    if (!b || !info) return fail(AXSPMV_E_INVALID, "batch_get_info: NULL argument");
+    *info = b->info;
+    return AXSPMV_OK;
+}
+
+extern "C" int axspmv_batch_spmv(axspmv_batch *b, const double *d_x, double *d_y, void *stream) {
+    if (!b || !d_x || !d_y) return fail(AXSPMV_E_INVALID, "batch_spmv: NULL argument");
+    const hipError_t e = launch::batch(b->args, d_x, d_y, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "batch launch");
+}
+
+extern "C" int axspmv_batch_time(axspmv_batch *b, const double *d_x, double *d_y, uint32_t iters, uint32_t warmup,
+                                 uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
+    if (!b || !d_x || !d_y) return fail(AXSPMV_E_INVALID, "batch_time: NULL argument");
+    const launch::BatchArgs g = b->args;
+    return time_launches([g, d_x, d_y](hipStream_t st) { return launch::batch(g, d_x, d_y, st); }, iters, warmup,
+                         mode, graph_batch, stream, out);
+}
+
+// ---------------------------------------------------------------- tuner
+namespace {
+
+std::vector<axspmv_opts> default_candidates(uint32_t op) {
+    std::vector<axspmv_opts> c;
+    axspmv_opts o;
+    axspmv_opts_default(AXSPMV_FMT_AXT, &o);
+    if (op == AXSPMV_OP_SPMM) {
+        for (uint32_t kt : {8u, 16u, 32u, 64u}) {
+            o.ktile = kt;
+            c.push_back(o);
+        }
+        return c;
+    }
+    for (uint32_t budget : {64u, 256u}) {
+        o.budget = budget;
+        for (uint32_t thw : {2u, 4u, 8u, 16u}) {
+            o.th = 1;
+            o.thw = thw;
+            c.push_back(o);
+        }
+        for (uint32_t th : {2u, 4u, 8u}) {
+            o.th = th;
+            o.thw = 64;
+            c.push_back(o);
+        }
+    }
+    return c;
+}
+
+}  // namespace
+
+extern "C" int axspmv_autotune(const axspmv_csr *a, uint32_t op, uint32_t k, const axspmv_opts *cand, uint32_t ncand,
+                               uint32_t iters, void *stream, uint32_t *best, double *times_ms) {
+    if (!a || !best || (op != AXSPMV_OP_SPMV && op != AXSPMV_OP_SPMM) || (op == AXSPMV_OP_SPMM && k == 0))
+        return fail(AXSPMV_E_INVALID, "autotune: bad argument");
+    std::vector<axspmv_opts> cs = cand ? std::vector<axspmv_opts>(cand, cand + ncand) : default_candidates(op);
+    if (cs.empty()) return fail(AXSPMV_E_INVALID, "autotune: no candidates");
+    if (iters == 0) iters = 20;
+    const uint64_t kk = op == AXSPMV_OP_SPMM ? k : 1;
+    void *dx = nullptr, *dy = nullptr;
+    if (hipMalloc(&dx, 8 * std::max<uint64_t>(1, uint64_t(a->ncols) * kk)) != hipSuccess ||
+        hipMalloc(&dy, 8 * std::max<uint64_t>(1, uint64_t(a->nrows) * kk)) != hipSuccess) {
+        if (dx) (void)hipFree(dx);
+        return fail(AXSPMV_E_NOMEM, "autotune: vector allocation failed");
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // x = 1 (the timing does not depend on the values)
+    if (launch::fill_f64(static_cast<double *>(dx), std::max<uint64_t>(1, uint64_t(a->ncols) * kk), 1.0, s) !=
+        hipSuccess) {
+        (void)hipFree(dx);
+        (void)hipFree(dy);
+        return fail(AXSPMV_E_HIP, "autotune: fill launch failed");
+    }
+    double best_t = INFINITY;
+    uint32_t best_i = 0;
+    for (uint32_t i = 0; i < cs.size(); ++i) {
+        double t = -1.0;
+        axspmv_plan *p = nullptr;
+        if (axspmv_plan_create(a, &cs[i], stream, &p) == AXSPMV_OK) {
+            axspmv_operands o{};
+            o.x = o.X = static_cast<const double *>(dx);
+            o.y = o.Y = static_cast<double *>(dy);
+            o.k = k;
+            axspmv_timing tm;
+            if (axspmv_time_op(p, op, &o, iters, 2, AXSPMV_TIME_REGION, 1, stream, &tm) == AXSPMV_OK) t = tm.mean_ms;
+            axspmv_plan_destroy(p);
+        }
+        if (times_ms) times_ms[i] = t;
+        if (t >= 0 && t < best_t) {
+            best_t = t;
+            best_i = i;
+        }
+    }
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    clear_error();
+    if (!(best_t < INFINITY)) return fail(AXSPMV_E_UNSUPPORTED, "autotune: no candidate ran");
+    *best = best_i;
+    return AXSPMV_OK;
+}
+
+extern "C" int axspmv_autotune_batch(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *cand,
+                                     uint32_t ncand, uint32_t iters, void *stream, axspmv_opts *out_opts,
+                                     uint32_t *ngroups) {
+    if (!mats || !out_opts || count == 0) return fail(AXSPMV_E_INVALID, "autotune_batch: bad argument");
+    std::vector<axspmv_opts> cs = cand ? std::vector<axspmv_opts>(cand, cand + ncand) : default_candidates(AXSPMV_OP_SPMV);
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> chosen;  // shape -> candidate index
+    for (uint32_t i = 0; i < count; ++i) {
+        const axspmv_csr *a = mats[i];
+        if (!a) return fail(AXSPMV_E_INVALID, "autotune_batch: matrix %u is NULL", i);
+        const uint32_t lg = a->nrows ? ilog2(a->nrows) : 0;
+        const double mean = a->nrows ? double(a->nnz) / a->nrows : 0.0;
+        const auto key = std::make_pair(lg, uint32_t(mean / 4.0));
+        auto it = chosen.find(key);
+        if (it == chosen.end()) {
+            uint32_t best = 0;
+            AX_TRY(axspmv_autotune(a, AXSPMV_OP_SPMV, 0, cs.data(), uint32_t(cs.size()), iters, stream, &best, nullptr));
+            it = chosen.emplace(key, best).first;
+        }
+        out_opts[i] = cs[it->second];
+    }
+    if (ngroups) *ngroups = uint32_t(chosen.size());
+    return AXSPMV_OK;
+}
+
+extern "C" int axspmv_batch_graph(axspmv_batch *b, const double *d_x, double *d_y, uint32_t repeats,
+                                  axspmv_graph **out) {
+    if (!b || !d_x || !d_y) return fail(AXSPMV_E_INVALID, "batch_graph: NULL argument");
+    const launch::BatchArgs g = b->args;
+    return capture_graph([g, d_x, d_y](hipStream_t st) { return launch::batch(g, d_x, d_y, st); }, repeats, out);
+}

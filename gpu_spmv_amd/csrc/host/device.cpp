@@ -1,5 +1,6 @@
 // Device helpers and the timing loop.
 #include <algorithm>
+#include <functional>
 #include <vector>
 
 #include "plan.hpp"
@@ -82,61 +83,58 @@ void summarize(std::vector<double> &ms, double scale, axspmv_timing *out) {
 
 // Timing loop.  EVENTS: mean of per-launch event times as in the reference
 // (cudaSpmv.cu:563-581), plus the median.  GRAPH: hipGraph replays of
-// `graph_batch` SpMVs each, reported per SpMV.  REGION: one event pair
-// around `iters` back-to-back launches.
-extern "C" int axspmv_time_op(axspmv_plan *p, uint32_t op, const axspmv_operands *ops, uint32_t iters,
-                              uint32_t warmup, uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
-    if (!p || !ops || !out || iters == 0 || mode > AXSPMV_TIME_REGION) return fail(AXSPMV_E_INVALID, "time_op: bad argument");
-    AX_TRY(plan_op_check(p, op, *ops));
+// `graph_batch` operations each, reported per operation.  REGION: one event
+// pair around `iters` back-to-back launches.
+int axspmv::time_launches(const std::function<hipError_t(hipStream_t)> &fn, uint32_t iters, uint32_t warmup,
+                          uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
+    if (!out || iters == 0 || mode > AXSPMV_TIME_REGION) return fail(AXSPMV_E_INVALID, "timing: bad argument");
     std::memset(out, 0, sizeof *out);
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipStream_t own = nullptr;
     Events ev;
     if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess)
-        return fail(AXSPMV_E_HIP, "time_spmv: event creation failed");
+        return fail(AXSPMV_E_HIP, "timing: event creation failed");
     std::vector<double> ms;
     ms.reserve(iters);
-    int rc = AXSPMV_OK;
     out->mode = mode;
     out->iters = iters;
     if (mode != AXSPMV_TIME_GRAPH) {
         for (uint32_t i = 0; i < warmup; ++i)
-            if (plan_op(p, op, *ops, s) != hipSuccess) return fail(AXSPMV_E_HIP, "time_spmv: launch failed");
+            if (fn(s) != hipSuccess) return fail(AXSPMV_E_HIP, "timing: launch failed");
         if (mode == AXSPMV_TIME_REGION) {
             hipError_t e = hipStreamSynchronize(s);
             if (e == hipSuccess) e = hipEventRecord(ev.a, s);
-            for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) e = plan_op(p, op, *ops, s);
+            for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) e = fn(s);
             if (e == hipSuccess) e = hipEventRecord(ev.b, s);
             if (e == hipSuccess) e = hipEventSynchronize(ev.b);
             float t = 0.f;
             if (e == hipSuccess) e = hipEventElapsedTime(&t, ev.a, ev.b);
-            if (e != hipSuccess) return hip_fail(e, "time_spmv (region)");
+            if (e != hipSuccess) return hip_fail(e, "timing (region)");
             out->mean_ms = out->median_ms = out->min_ms = out->max_ms = double(t) / iters;
             return AXSPMV_OK;
         }
-        for (uint32_t i = 0; i < iters && rc == AXSPMV_OK; ++i) {
+        for (uint32_t i = 0; i < iters; ++i) {
             hipError_t e = hipEventRecord(ev.a, s);
-            if (e == hipSuccess) e = plan_op(p, op, *ops, s);
+            if (e == hipSuccess) e = fn(s);
             if (e == hipSuccess) e = hipEventRecord(ev.b, s);
             if (e == hipSuccess) e = hipEventSynchronize(ev.b);
             float t = 0.f;
             if (e == hipSuccess) e = hipEventElapsedTime(&t, ev.a, ev.b);
-            if (e != hipSuccess) rc = hip_fail(e, "time_spmv");
+            if (e != hipSuccess) return hip_fail(e, "timing");
             ms.push_back(t);
         }
-        if (rc) return rc;
         summarize(ms, 1.0, out);
         return AXSPMV_OK;
     }
     if (graph_batch == 0) graph_batch = 1;
     if (!s) {
-        if (hipStreamCreate(&own) != hipSuccess) return fail(AXSPMV_E_HIP, "time_spmv: stream creation failed");
+        if (hipStreamCreate(&own) != hipSuccess) return fail(AXSPMV_E_HIP, "timing: stream creation failed");
         s = own;
     }
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-    for (uint32_t i = 0; i < graph_batch && e == hipSuccess; ++i) e = plan_op(p, op, *ops, s);
+    for (uint32_t i = 0; i < graph_batch && e == hipSuccess; ++i) e = fn(s);
     const hipError_t e2 = hipStreamEndCapture(s, &g);
     if (e == hipSuccess) e = e2;
     if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -154,9 +152,18 @@ extern "C" int axspmv_time_op(axspmv_plan *p, uint32_t op, const axspmv_operands
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
     if (own) (void)hipStreamDestroy(own);
-    if (e != hipSuccess) return hip_fail(e, "time_spmv (graph)");
+    if (e != hipSuccess) return hip_fail(e, "timing (graph)");
     summarize(ms, 1.0 / graph_batch, out);
     return AXSPMV_OK;
+}
+
+extern "C" int axspmv_time_op(axspmv_plan *p, uint32_t op, const axspmv_operands *ops, uint32_t iters,
+                              uint32_t warmup, uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
+    if (!p || !ops) return fail(AXSPMV_E_INVALID, "time_op: NULL argument");
+    AX_TRY(plan_op_check(p, op, *ops));
+    const axspmv_operands o = *ops;
+    return time_launches([p, op, o](hipStream_t st) { return plan_op(p, op, o, st); }, iters, warmup, mode,
+                         graph_batch, stream, out);
 }
 
 extern "C" int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, uint32_t iters, uint32_t warmup,
@@ -165,4 +172,58 @@ extern "C" int axspmv_time_spmv(axspmv_plan *p, const double *d_x, double *d_y, 
     o.x = d_x;
     o.y = d_y;
     return axspmv_time_op(p, AXSPMV_OP_SPMV, &o, iters, warmup, mode, graph_batch, stream, out);
+}
+
+// ---------------------------------------------------------------- graphs
+struct axspmv_graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    ~axspmv_graph() {
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+namespace axspmv {
+
+// Captures `repeats` calls of fn on a private stream into an executable graph.
+int capture_graph(const std::function<hipError_t(hipStream_t)> &fn, uint32_t repeats, axspmv_graph **out) {
+    if (!out || repeats == 0) return fail(AXSPMV_E_INVALID, "graph: bad argument");
+    *out = nullptr;
+    hipStream_t s = nullptr;
+    if (hipStreamCreate(&s) != hipSuccess) return fail(AXSPMV_E_HIP, "graph: stream creation failed");
+    auto gr = new axspmv_graph;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    for (uint32_t i = 0; i < repeats && e == hipSuccess; ++i) e = fn(s);
+    const hipError_t e2 = hipStreamEndCapture(s, &gr->g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&gr->ge, gr->g, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) {
+        delete gr;
+        return hip_fail(e, "graph capture");
+    }
+    *out = gr;
+    return AXSPMV_OK;
+}
+
+}  // namespace axspmv
+
+extern "C" int axspmv_plan_graph(axspmv_plan *p, uint32_t op, const axspmv_operands *ops, uint32_t repeats,
+                                 axspmv_graph **out) {
+    if (!p || !ops) return fail(AXSPMV_E_INVALID, "plan_graph: NULL argument");
+    AX_TRY(plan_op_check(p, op, *ops));
+    const axspmv_operands o = *ops;
+    return capture_graph([p, op, o](hipStream_t st) { return plan_op(p, op, o, st); }, repeats, out);
+}
+
+extern "C" int axspmv_graph_launch(axspmv_graph *g, void *stream) {
+    if (!g || !g->ge) return fail(AXSPMV_E_INVALID, "graph_launch: NULL graph");
+    const hipError_t e = hipGraphLaunch(g->ge, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "graph launch");
+}
+
+extern "C" int axspmv_graph_destroy(axspmv_graph *g) {
+    delete g;
+    return AXSPMV_OK;
 }

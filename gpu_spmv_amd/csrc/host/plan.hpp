@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <functional>
 #include <vector>
 
 #include "../hip/launch.hpp"
@@ -57,5 +58,10 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
 hipError_t plan_op(axspmv_plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s);
 int plan_op_check(const axspmv_plan *p, uint32_t op, const axspmv_operands &o);
 int plan_add_transpose(axspmv_plan *p, const axspmv_csr *a, hipStream_t s);
+// Times launches of `fn` (see axspmv_time_op for the modes).
+int time_launches(const std::function<hipError_t(hipStream_t)> &fn, uint32_t iters, uint32_t warmup, uint32_t mode,
+                  uint32_t graph_batch, void *stream, axspmv_timing *out);
+// Captures `repeats` calls of fn into an executable graph.
+int capture_graph(const std::function<hipError_t(hipStream_t)> &fn, uint32_t repeats, axspmv_graph **out);
 std::vector<launch::Block> row_blocks(const std::vector<uint32_t> &units_per_row, uint32_t budget);
 }  // namespace axspmv

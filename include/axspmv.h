@@ -321,6 +321,59 @@ int axspmv_time_op(axspmv_plan *p, uint32_t op, const axspmv_operands *ops, uint
                    uint32_t warmup, uint32_t mode, uint32_t graph_batch, void *stream,
                    axspmv_timing *out);
 
+/* ------------------------------------------------------------ batches */
+/* Many matrices of different sizes, SpMV in one persistent launch.  Each
+ * matrix uses the AXT UNC layout of its own options (opts[i] if per_matrix,
+ * else opts[0] for all; format must be AXSPMV_FMT_AXT, mode UNC).  x and y
+ * are the concatenations of the matrices' x (ncols each) and y (nrows each)
+ * in batch order.  No reference counterpart. */
+typedef struct axspmv_batch axspmv_batch;
+int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *opts,
+                        int per_matrix, void *stream, axspmv_batch **out);
+int axspmv_batch_destroy(axspmv_batch *b);
+typedef struct axspmv_batch_info {
+    uint64_t nrows, ncols, nnz, slots, alg_bytes, device_bytes;
+    uint32_t count, nitems, nwaves, pad_;
+} axspmv_batch_info;
+int axspmv_batch_get_info(const axspmv_batch *b, axspmv_batch_info *info);
+int axspmv_batch_spmv(axspmv_batch *b, const double *d_x, double *d_y, void *stream);
+/* Same modes as axspmv_time_op (AXSPMV_TIME_GRAPH captures graph_batch
+ * persistent launches in one hipGraph). */
+int axspmv_batch_time(axspmv_batch *b, const double *d_x, double *d_y, uint32_t iters, uint32_t warmup,
+                      uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out);
+
+/* ------------------------------------------------------------- graphs */
+/* A hipGraph holding `repeats` back-to-back launches of one operation of a
+ * plan (enum axspmv_op, operands as for axspmv_time_op) or of a batch SpMV,
+ * captured once and replayed with axspmv_graph_launch.  The captured
+ * kernels write every output element, so replays need no clearing.  No
+ * reference counterpart. */
+typedef struct axspmv_graph axspmv_graph;
+int axspmv_plan_graph(axspmv_plan *p, uint32_t op, const axspmv_operands *ops, uint32_t repeats,
+                      axspmv_graph **out);
+int axspmv_batch_graph(axspmv_batch *b, const double *d_x, double *d_y, uint32_t repeats,
+                       axspmv_graph **out);
+int axspmv_graph_launch(axspmv_graph *g, void *stream);
+int axspmv_graph_destroy(axspmv_graph *g);
+
+/* ---------------------------------------------------------- autotuner */
+/* Times every candidate option set (a plan built on the device, then `iters`
+ * back-to-back launches of `op`, AXSPMV_OP_SPMV or AXSPMV_OP_SPMM with k
+ * right-hand sides) and returns the index of the fastest in *best.
+ * times_ms (ncand entries, may be NULL) gets each candidate's time per
+ * operation, or -1 for a candidate the library rejects.  cand = NULL uses a
+ * built-in set (AXT UNC th = 1 with thw 2..16, th = 2..8 with thw 64, two
+ * block budgets; for SpMM the four k-tiles). */
+int axspmv_autotune(const axspmv_csr *a, uint32_t op, uint32_t k, const axspmv_opts *cand,
+                    uint32_t ncand, uint32_t iters, void *stream, uint32_t *best, double *times_ms);
+/* Per-matrix choice for a batch: matrices are grouped by shape (rows rounded
+ * to a power of two, mean row length in steps of 4); the first matrix of
+ * each group is tuned with axspmv_autotune (SpMV) and its winner is written
+ * to out_opts[i] for every member.  *ngroups gets the number of groups. */
+int axspmv_autotune_batch(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *cand,
+                          uint32_t ncand, uint32_t iters, void *stream, axspmv_opts *out_opts,
+                          uint32_t *ngroups);
+
 /* ---------------------------------------------------- device helpers */
 int axspmv_dev_alloc(size_t bytes, void **ptr);
 int axspmv_dev_free(void *ptr);
