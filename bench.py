@@ -45,6 +45,12 @@ OTHERS = [
     ("k1_c64_natural", ax.FMT_K1, dict(chunk=64, reorder=0)),
     ("axc_hbs4", ax.FMT_AXC, dict(hbs=4)),
     ("axc_hbs64", ax.FMT_AXC, dict(hbs=64)),
+    ("rct_auto", ax.FMT_RCT, dict()),
+    ("rct_r16384_s4", ax.FMT_RCT, dict(tile_rows=16384, slices=4)),
+    ("rct_r8192_s4", ax.FMT_RCT, dict(tile_rows=8192, slices=4)),
+    ("rct_r8192_s8", ax.FMT_RCT, dict(tile_rows=8192, slices=8)),
+    ("rct_r16384_s2", ax.FMT_RCT, dict(tile_rows=16384, slices=2)),
+    ("rct_r4096_s1", ax.FMT_RCT, dict(tile_rows=4096, slices=1)),
 ]
 
 

@@ -30,7 +30,7 @@ class AxspmvError(RuntimeError):
 # status codes (enum axspmv_status)
 OK, E_INVALID, E_HIP, E_NOMEM, E_UNSUPPORTED, E_OVERFLOW, E_IO = 0, -1, -2, -3, -4, -5, -6
 # formats / kernels / value types / AXT modes / flags
-FMT_CSR, FMT_K1, FMT_AXC, FMT_AXT = 0, 1, 2, 3
+FMT_CSR, FMT_K1, FMT_AXC, FMT_AXT, FMT_RCT = 0, 1, 2, 3, 4
 KER_DEFAULT, KER_NAIVE, KER_STREAM = 0, 1, 2
 F64, F32 = 0, 1
 AXT_UNC, AXT_COM = 0, 1
@@ -72,7 +72,8 @@ class Axt(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("format", u32), ("kernel", u32), ("mode", u32), ("thw", u32), ("th", u32),
                 ("bs", u32), ("hbs", u32), ("chunk", u32), ("reorder", u32), ("vtype", u32),
-                ("block", u32), ("budget", u32), ("flags", u32), ("ktile", u32), ("reserved", u32 * 2)]
+                ("block", u32), ("budget", u32), ("flags", u32), ("ktile", u32), ("tile_rows", u32),
+                ("slices", u32)]
 
 
 class PlanInfo(C.Structure):

@@ -57,6 +57,19 @@ struct SpmmArgs {
 // kt (k-tile) in {8, 16, 32, 64}
 hipError_t spmm(const SpmmArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s);
 
+// RCT tiles: tile t = rb * S + s holds entries tile_ptr[t] .. tile_ptr[t+1]-1
+// (column col, row rb*R + lrow), sorted by column; part holds S vectors of
+// nrows partial sums.
+struct RctArgs {
+    const void *val;
+    const uint32_t *col;
+    const uint16_t *lrow;
+    const uint32_t *tile_ptr;
+    double *part;
+    uint32_t nrows, R, S, nrb, vtype, block;
+};
+hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s);
+
 // Batched persistent SpMV over AXT UNC matrices.
 struct BatchMat {
     uint64_t voff;      // first slot of the matrix in the concatenated planes

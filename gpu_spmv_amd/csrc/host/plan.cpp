@@ -400,6 +400,90 @@ int plan_axc(Plan &p, const axspmv_csr *a, hipStream_t s) {
     return setup_rowwise(p, a, d_first, vp, mapx, s);
 }
 
+// RCT: tiles of R rows x C columns, entries of a tile sorted by (column, row).
+int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
+    hipDeviceProp_t prop;
+    size_t lds_max = 65536;
+    if (hipGetDeviceProperties(&prop, p.device) == hipSuccess && prop.sharedMemPerBlock > 0)
+        lds_max = prop.sharedMemPerBlock;
+    uint32_t R = p.o.tile_rows ? p.o.tile_rows : uint32_t(std::min<size_t>(16384, lds_max / 8));
+    if (R == 0 || R > 65536 || size_t(R) * 8 > lds_max)
+        return fail(AXSPMV_E_UNSUPPORTED, "RCT: %u rows per tile do not fit %zu bytes of LDS", R, lds_max);
+    uint32_t S = p.o.slices;
+    if (S == 0) {
+        S = 1;
+        while (S < 8 && uint64_t(a->ncols) * 8 / S > (2ull << 20)) S *= 2;
+    }
+    if (S != 1 && S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_INVALID, "RCT: slices must be 1, 2, 4 or 8");
+    const uint32_t n = a->nrows;
+    const uint32_t nrb = uint32_t(ceil_div(n, R));
+    const uint32_t C = uint32_t(std::max<uint64_t>(1, ceil_div(a->ncols, S)));
+    const uint64_t ntile = uint64_t(nrb) * S;
+    // count entries per tile
+    std::vector<uint64_t> cnt(ntile + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint32_t rb = 0; rb < nrb; ++rb) {
+        const uint32_t r1 = std::min<uint64_t>(uint64_t(rb + 1) * R, n);
+        for (uint32_t j = a->row[rb * R]; j < a->row[r1]; ++j) cnt[uint64_t(rb) * S + a->col[j] / C + 1]++;
+    }
+    for (uint64_t t = 0; t < ntile; ++t) cnt[t + 1] += cnt[t];
+    const uint64_t nnz = a->nnz;
+    std::vector<uint32_t> col(nnz ? nnz : 1);
+    std::vector<uint16_t> lrow(nnz ? nnz : 1);
+    std::vector<double> val(nnz ? nnz : 1);
+#pragma omp parallel
+    {
+        std::vector<std::pair<uint64_t, uint32_t>> key;  // (col << 16 | lrow, CSR position)
+#pragma omp for schedule(dynamic, 1)
+        for (uint32_t rb = 0; rb < nrb; ++rb) {
+            const uint32_t r0 = rb * R, r1 = std::min<uint64_t>(uint64_t(rb + 1) * R, n);
+            key.clear();
+            for (uint32_t r = r0; r < r1; ++r)
+                for (uint32_t j = a->row[r]; j < a->row[r + 1]; ++j)
+                    key.emplace_back((uint64_t(a->col[j]) << 16) | (r - r0), j);
+            std::sort(key.begin(), key.end());
+            // sorted by column, so the slices come out in order
+            uint64_t q = cnt[uint64_t(rb) * S];
+            for (const auto &k : key) {
+                col[q] = uint32_t(k.first >> 16);
+                lrow[q] = uint16_t(k.first & 0xffff);
+                val[q] = a->val[k.second];
+                ++q;
+            }
+        }
+    }
+    if (cnt[ntile] >= (1ull << 32)) return fail(AXSPMV_E_OVERFLOW, "RCT: too many entries");
+    std::vector<uint32_t> tptr(ntile + 1);
+    for (uint64_t t = 0; t <= ntile; ++t) tptr[t] = uint32_t(cnt[t]);
+    const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
+    void *dv = p.dalloc(vs * nnz);
+    auto *dc = static_cast<uint32_t *>(p.dalloc(4 * nnz));
+    auto *dl = static_cast<uint16_t *>(p.dalloc(2 * nnz));
+    auto *dt = static_cast<uint32_t *>(p.dalloc(4 * tptr.size()));
+    auto *part = static_cast<double *>(p.dalloc(S > 1 ? 8ull * S * n : 8));
+    if (!dv || !dc || !dl || !dt || !part) return AXSPMV_E_NOMEM;
+    if (p.o.vtype == AXSPMV_F32) {
+        std::vector<float> v(val.begin(), val.begin() + nnz);
+        AX_TRY(p.upload(dv, v.data(), 4 * nnz, s));
+    } else {
+        AX_TRY(p.upload(dv, val.data(), 8 * nnz, s));
+    }
+    AX_TRY(p.upload(dc, col.data(), 4 * nnz, s));
+    AX_TRY(p.upload(dl, lrow.data(), 2 * nnz, s));
+    AX_TRY(p.upload(dt, tptr.data(), 4 * tptr.size(), s));
+    uint32_t block = p.o.block;
+    if (block == 0 || block > 1024 || block % 64) block = 1024;
+    p.rct = launch::RctArgs{dv, dc, dl, dt, part, n, R, S, nrb, p.o.vtype, block};
+    p.slots = nnz;
+    // matrix stream + per-slice partial write/read when S > 1
+    p.stream_bytes = (vs + 6) * nnz + 4 * tptr.size() + (S > 1 ? 16ull * S * n : 0);
+    p.launches = S > 1 ? 2 : 1;
+    p.nblocks = uint32_t(ntile);
+    p.kind = Plan::RCT;
+    std::snprintf(p.name, sizeof p.name, "rct_r%u_s%u", R, S);
+    return AXSPMV_OK;
+}
+
 launch::SegArgs seg_args(const Plan &p, const uint32_t *rowsrc) {
     launch::SegArgs g{};
     g.vplane = p.d_vplane;
@@ -433,7 +517,7 @@ extern "C" int axspmv_opts_default(uint32_t format, axspmv_opts *o) {
     o->vtype = AXSPMV_F64;
     o->block = 256;
     o->budget = 0;
-    if (format > AXSPMV_FMT_AXT) return fail(AXSPMV_E_INVALID, "opts_default: unknown format %u", format);
+    if (format > AXSPMV_FMT_RCT) return fail(AXSPMV_E_INVALID, "opts_default: unknown format %u", format);
     return AXSPMV_OK;
 }
 
@@ -461,6 +545,7 @@ extern "C" int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, voi
         case AXSPMV_FMT_K1: rc = plan_k1(*p, a, s); break;
         case AXSPMV_FMT_AXC: rc = plan_axc(*p, a, s); break;
         case AXSPMV_FMT_AXT: rc = plan_axt(*p, a, s); break;
+        case AXSPMV_FMT_RCT: rc = plan_rct(*p, a, s); break;
         default: rc = fail(AXSPMV_E_INVALID, "plan_create: unknown format %u", o->format);
     }
     if (rc) return rc;
@@ -507,6 +592,7 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
         case Plan::AXT_COM: return launch::axt_com(p->com, x, y, s);
         case Plan::K1: return launch::k1(p->k1, x, y, s);
         case Plan::AXC: return launch::axc(p->axc, x, y, s);
+        case Plan::RCT: return launch::rct(p->rct, x, y, s);
         default: return hipErrorInvalidValue;
     }
 }

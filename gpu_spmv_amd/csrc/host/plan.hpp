@@ -11,7 +11,7 @@
 #include "common.hpp"
 
 struct axspmv_plan {
-    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC };
+    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC, RCT };
     Kind kind = NONE;
     int device = 0;
     axspmv_opts o{};
@@ -33,6 +33,7 @@ struct axspmv_plan {
     axspmv::launch::K1Args k1{};
     axspmv::launch::AxcArgs axc{};
     axspmv::launch::ComArgs com{};
+    axspmv::launch::RctArgs rct{};
     // Row-contiguous layouts (CSR, AXT UNC th == 1, AXC): CSR row pointer and
     // first plane slot of each row, for SpMM and the value gradient.
     uint32_t *d_rowptr = nullptr, *d_first = nullptr;

@@ -204,7 +204,10 @@ enum axspmv_format {
     AXSPMV_FMT_CSR = 0, /* val/col/row                                  */
     AXSPMV_FMT_K1 = 1,  /* sliced ELL, chunk rows per wavefront          */
     AXSPMV_FMT_AXC = 2, /* row bricks of hbs slots                       */
-    AXSPMV_FMT_AXT = 3  /* tiled: UNC/COM x th==1 / th>1                 */
+    AXSPMV_FMT_AXT = 3, /* tiled: UNC/COM x th==1 / th>1                 */
+    AXSPMV_FMT_RCT = 4  /* row-block x column-slice tiles, entries sorted
+                           by column, one workgroup per tile (no reference
+                           counterpart: built for gather-bound matrices)  */
 };
 enum axspmv_kernel {
     AXSPMV_KER_DEFAULT = 0, /* the format's own kernel                  */
@@ -229,7 +232,8 @@ typedef struct axspmv_opts {
     uint32_t flags;   /* AXSPMV_PLAN_* */
     uint32_t ktile;   /* SpMM k-tile: right-hand sides per lane group, 8/16/32/64
                          (0 = smallest of those >= min(k, 64)) */
-    uint32_t reserved[2];
+    uint32_t tile_rows; /* RCT: rows per tile (0 = as many as the LDS holds) */
+    uint32_t slices;    /* RCT: column slices 1, 2, 4 or 8 (0 = x slice <= 2 MB) */
 } axspmv_opts;
 
 #define AXSPMV_PLAN_TRANSPOSE 1u /* also build A^T (needed by the adjoint) */

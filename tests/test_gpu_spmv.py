@@ -47,6 +47,11 @@ KERNELS = [
     ("com_h16_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=16)),
     ("com_h4_thw32", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=32, th=4)),
     ("com_h32_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=32)),
+    ("rct_auto", ax.FMT_RCT, dict()),
+    ("rct_r1024_s4", ax.FMT_RCT, dict(tile_rows=1024, slices=4)),
+    ("rct_r4096_s8", ax.FMT_RCT, dict(tile_rows=4096, slices=8)),
+    ("rct_r8192_s2_b256", ax.FMT_RCT, dict(tile_rows=8192, slices=2, block=256)),
+    ("rct_r512_s1", ax.FMT_RCT, dict(tile_rows=512, slices=1)),
 ]
 
 
@@ -102,7 +107,7 @@ def test_spmv_f64(mname, kname, fmt, kw):
 
 
 F32_KERNELS = [k for k in KERNELS if k[0] in ("csr_naive", "unc_h1_thw4", "unc_h4_thw64", "k1_c64", "axc_h8",
-                                                "com_h1_thw64_bs256", "com_h8_thw64")]
+                                                "com_h1_thw64_bs256", "com_h8_thw64", "rct_r1024_s4")]
 
 
 @pytest.mark.parametrize("mname", ["rand200", "powerlaw64k", "long_rows"])
@@ -124,7 +129,7 @@ def test_powerlaw_config2_full_size():
     for fmt, kw in ((ax.FMT_AXT, dict(thw=4, th=1)), (ax.FMT_AXT, dict(thw=64, th=4)),
                     (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=256)),
                     (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)), (ax.FMT_K1, dict(chunk=64)),
-                    (ax.FMT_AXC, dict(hbs=4))):
+                    (ax.FMT_AXC, dict(hbs=4)), (ax.FMT_RCT, dict())):
         y = ax.spmv_host(ax.Plan(m, fmt, **kw), x)
         assert rel_err(ref, y) <= TOL, kw
 
