@@ -103,7 +103,10 @@ def test_spmv_f64(mname, kname, fmt, kw):
     dy = ax.DeviceBuffer.from_array(np.full(max(m.nrows, 1), np.nan))
     plan.spmv(dx, dy)
     y2 = dy.download(np.float64, m.nrows)
-    assert np.array_equal(y, y2)
+    if fmt == ax.FMT_RCT:  # LDS atomics: the order of additions varies between runs
+        assert rel_err(ref, y2) <= TOL
+    else:
+        assert np.array_equal(y, y2)
 
 
 F32_KERNELS = [k for k in KERNELS if k[0] in ("csr_naive", "unc_h1_thw4", "unc_h4_thw64", "k1_c64", "axc_h8",
