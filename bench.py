@@ -1,17 +1,23 @@
 """SpMV benchmark on one MI355X.
 
-Workload (one step = one SpMV y = A x, inputs resident in HBM):
-  synthetic power-law matrix, n = 2^20 rows, row lengths Zipf(alpha = 2) on
-  {1..2048} rescaled to mean 20 (nnz ~ 21.0M), uniform sorted distinct
-  columns, values N(0,1), x ~ U[-1,1], seed 1 -- FP64, in the AXT format
-  (uncompacted, tile height 1, tile half width 4).
+Workload (one step = one SpMV y = A x, inputs resident in HBM): the
+synthetic power-law matrix, n = 2^20 rows, row lengths Zipf(alpha = 2) on
+{1..2048} rescaled to mean 20 (nnz ~ 21.0M), uniform sorted distinct
+columns, values N(0,1), x ~ U[-1,1], seed 1 -- FP64.
 
 Prints one JSON line: GFLOP/s of the flagship kernel (2*nnz per SpMV), its
 roofline against 8 TB/s of HBM (algorithmic bytes per SpMV / kernel time
 from HIP events), the CPU baseline (the oracle's CSR product on this host's
-cores), and every other format on the same matrix.
+cores), every format on the same matrix, and the adjoint / SpMM / FP32-value
+forms against plain SpMV.
 
   python bench.py [--steps K] [--warmup W] [--formats flagship|all]
+  python bench.py --suite full     also the Laplacian, the permuted 27-point
+                                   stencil, the 256-matrix batch and the
+                                   forward + adjoint on the stencil structure
+  python bench.py --calibrate      copy bandwidth and the x-footprint sweep
+  python bench.py --pmc-probe      a 256 MB copy then 5 flagship SpMVs (for
+                                   rocprofv3 --pmc runs)
 """
 from __future__ import annotations
 
@@ -27,17 +33,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import gpu_spmv_amd as ax  # noqa: E402
+from gpu_spmv_amd import _lib as L  # noqa: E402
 from gpu_spmv_amd.core import TIME_GRAPH, TIME_REGION  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
-FLAGSHIP = ("axt_unc_h1_thw4", ax.FMT_AXT, dict(thw=4, th=1))
+FLAGSHIP = ("rct_auto", ax.FMT_RCT, dict())
+FLAGSHIP_AXT = ("axt_unc_h1_thw4", ax.FMT_AXT, dict(thw=4, th=1))
 OTHERS = [
+    FLAGSHIP_AXT,
     ("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
     ("axt_unc_h1_thw8", ax.FMT_AXT, dict(thw=8, th=1)),
     ("axt_unc_h1_thw16", ax.FMT_AXT, dict(thw=16, th=1)),
     ("axt_unc_h4_thw64", ax.FMT_AXT, dict(thw=64, th=4)),
     ("axt_unc_h8_thw64", ax.FMT_AXT, dict(thw=64, th=8)),
-    ("axt_unc_h1_thw4_f32val", ax.FMT_AXT, dict(thw=4, th=1, vtype=ax.F32)),
     ("axt_com_h1_thw64_bs256", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=256)),
     ("axt_com_h4_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)),
     ("axt_com_h8_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=8)),
@@ -45,12 +53,9 @@ OTHERS = [
     ("k1_c64_natural", ax.FMT_K1, dict(chunk=64, reorder=0)),
     ("axc_hbs4", ax.FMT_AXC, dict(hbs=4)),
     ("axc_hbs64", ax.FMT_AXC, dict(hbs=64)),
-    ("rct_auto", ax.FMT_RCT, dict()),
-    ("rct_r16384_s4", ax.FMT_RCT, dict(tile_rows=16384, slices=4)),
     ("rct_r8192_s4", ax.FMT_RCT, dict(tile_rows=8192, slices=4)),
     ("rct_r8192_s8", ax.FMT_RCT, dict(tile_rows=8192, slices=8)),
     ("rct_r16384_s2", ax.FMT_RCT, dict(tile_rows=16384, slices=2)),
-    ("rct_r4096_s1", ax.FMT_RCT, dict(tile_rows=4096, slices=1)),
 ]
 
 
@@ -59,13 +64,18 @@ def log(*a):
 
 
 def rel_err(ref, y):
-    return float(np.max(np.abs(y - ref)) / np.max(np.abs(ref)))
+    den = float(np.max(np.abs(ref))) if ref.size else 0.0
+    return float(np.max(np.abs(y - ref)) / den) if den else float(np.max(np.abs(y - ref)))
+
+
+def omp_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", 0)) or (os.cpu_count() or 1)
 
 
 def cpu_baseline(m, x, seconds: float):
-    """The oracle's CSR product (the reference's ncsr) on all host threads."""
+    """The oracle's CSR product (the reference's ncsr) on the host threads."""
     from oracle import oracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or (os.cpu_count() or 1)
+    threads = omp_threads()
     y = orc.ncsr(m, x, nthreads=threads)
     calls, t0 = 0, time.perf_counter()
     while True:
@@ -75,26 +85,232 @@ def cpu_baseline(m, x, seconds: float):
         if el >= seconds:
             break
     per = el / calls
+    alg = 12.0 * m.nnz + 4.0 * (m.nrows + 1) + 8.0 * m.ncols + 8.0 * m.nrows
     return y, dict(value=round(2.0 * m.nnz / per / 1e9, 3), unit="GFLOP/s", cores=threads, kind="port",
                    sample=f"{calls} calls of ncsr (OpenMP static over rows) on the full matrix, {el:.1f} s",
-                   ms_per_spmv=round(per * 1e3, 3))
+                   ms_per_spmv=round(per * 1e3, 3), alg_gbs=round(alg / per / 1e9, 1))
 
 
-def measure(m, name, fmt, kw, dx, dy, iters, warmup, yref):
+def summarize(info, nnz, sec, name, err, extra=None):
+    gbs = info["alg_bytes"] / sec / 1e9
+    r = dict(kernel=info["name"], label=name, us=round(sec * 1e6, 2), gflops=round(2.0 * nnz / sec / 1e9, 2),
+             alg_gbs=round(gbs, 1), frac=round(gbs / PEAK_HBM_GBS, 4), beta=round(info["beta"], 4),
+             stream_gbs=round(info["stream_bytes"] / sec / 1e9, 1), launches=info["launches"], rel_err=err)
+    if extra:
+        r.update(extra)
+    return r
+
+
+def measure(m, name, fmt, kw, dx, dy, iters, warmup, yref, mode=TIME_REGION, batch=1):
     p = ax.Plan(m, fmt, **kw)
     info = p.info()
     p.spmv(dx, dy)
-    ax._lib.call("axspmv_device_sync")
+    L.call("axspmv_device_sync")
     err = rel_err(yref, dy.download(np.float64, m.nrows)) if yref is not None else None
-    p.time(dx, dy, iters=max(warmup, 1), warmup=0, mode=TIME_REGION)
-    t = p.time(dx, dy, iters=iters, warmup=0, mode=TIME_REGION)
-    sec = t.mean_ms * 1e-3
-    gbs = info["alg_bytes"] / sec / 1e9
-    return p, info, dict(kernel=info["name"], label=name, us=round(t.mean_ms * 1e3, 2),
-                         gflops=round(2.0 * m.nnz / sec / 1e9, 2), alg_gbs=round(gbs, 1),
-                         frac=round(gbs / PEAK_HBM_GBS, 4), beta=round(info["beta"], 4),
-                         stream_gbs=round(info["stream_bytes"] / sec / 1e9, 1),
-                         rel_err=err)
+    p.time(dx, dy, iters=max(warmup, 1), warmup=0, mode=mode, batch=batch)
+    t = p.time(dx, dy, iters=iters, warmup=0, mode=mode, batch=batch)
+    return p, info, summarize(info, m.nnz, t.mean_ms * 1e-3, name, err)
+
+
+def extras_same_matrix(m, x, dx, yref, plain_us):
+    """Forward + adjoint, SpMM and FP32 values on the same matrix, against
+    plain FP64 SpMV (plain_us) of the flagship AXT kernel."""
+    from oracle import oracle as orc
+    out = {}
+    n, nc, nnz = m.nrows, m.ncols, m.nnz
+    # forward + adjoint of 0.5*||Ax - b||^2
+    b = ax.uniform(4, n, stream=ax.VEC_STREAM + 1)
+    p = ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_TRANSPOSE)
+    db = ax.DeviceBuffer.from_array(b)
+    bufs = dict(y=ax.DeviceBuffer(8 * n), r=ax.DeviceBuffer(8 * n), xbar=ax.DeviceBuffer(8 * nc),
+                vbar=ax.DeviceBuffer(8 * nnz))
+    fa = {}
+    for label, op in (("fused", L.OP_FWD_ADJ), ("split", L.OP_FWD_ADJ_SPLIT)):
+        t = p.time_op(op, iters=30, warmup=3, x=dx, b=db, **bufs)
+        fa[label] = round(t.mean_ms * 1e3, 2)
+    r = bufs["r"].download(np.float64, n)
+    xbar = bufs["xbar"].download(np.float64, nc)
+    r_ref = yref - b if yref is not None else None
+    t_adj = p.time_op(L.OP_ADJOINT, iters=30, warmup=3, ybar=bufs["r"], xbar=bufs["xbar"])
+    out["fwd_adj"] = dict(plain_spmv_us=plain_us, fused_us=fa["fused"], split_us=fa["split"],
+                          adjoint_only_us=round(t_adj.mean_ms * 1e3, 2),
+                          fused_over_plain=round(fa["fused"] / plain_us, 3),
+                          r_rel_err=rel_err(r_ref, r) if r_ref is not None else None,
+                          xbar_rel_err=rel_err(orc.ncsr_t(m, r_ref, nc), xbar) if r_ref is not None else None)
+    log(f"fwd_adj: {out['fwd_adj']}")
+    del bufs
+    # SpMM, FP64 and FP32 values
+    sp = []
+    for k in (8, 32, 64):
+        X = ax.uniform(3, nc * k).reshape(nc, k)
+        dX = ax.DeviceBuffer.from_array(X)
+        dY = ax.DeviceBuffer(8 * n * k)
+        rows = np.arange(0, n, max(1, n // 2000))
+        for vt in (ax.F64, ax.F32):
+            best, times = ax.autotune(m, op=L.OP_SPMM, k=k,
+                                      candidates=[ax.opts(ax.FMT_AXT, thw=4, vtype=vt, ktile=kt)
+                                                  for kt in (8, 16, 32, 64)], iters=5)
+            kt = (8, 16, 32, 64)[best]
+            pm = ax.Plan(m, ax.FMT_AXT, thw=4, vtype=vt, ktile=kt)
+            t = pm.time_op(L.OP_SPMM, iters=10, warmup=2, k=k, X=dX, Y=dY)
+            Y = dY.download(np.float64, n * k).reshape(n, k)
+            # accuracy on sampled rows against the FP64 oracle
+            Yref = orc.spmm(m, X, nthreads=omp_threads())[rows] if k == 8 else None
+            err = rel_err(Yref, Y[rows]) if Yref is not None else None
+            alg = (12 if vt == ax.F64 else 8) * nnz + 4 * (n + 1) + 16 * n * k
+            us = t.mean_ms * 1e3
+            sp.append(dict(k=k, vtype="f64" if vt == ax.F64 else "f32val", ktile=kt, us=round(us, 1),
+                           gflops=round(2.0 * nnz * k / (us * 1e-6) / 1e9, 1),
+                           alg_gbs=round(alg / (us * 1e-6) / 1e9, 1),
+                           per_rhs_vs_plain=round(us / k / plain_us, 3), rel_err_vs_f64_oracle=err))
+            log(f"spmm: {sp[-1]}")
+        del dX, dY
+    out["spmm"] = sp
+    return out
+
+
+def calibrate():
+    """Copy bandwidth, and the flagship AXT kernel on the power-law structure
+    with columns folded into a smaller x (footprint sweep): the gather bound."""
+    res = {}
+    n = 32 << 20  # 256 MB of doubles
+    src, dst = ax.DeviceBuffer(8 * n), ax.DeviceBuffer(8 * n)
+    for _ in range(3):
+        L.call("axspmv_copy_f64", src.ptr, dst.ptr, n, None)
+    L.call("axspmv_device_sync")
+    t0 = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        L.call("axspmv_copy_f64", src.ptr, dst.ptr, n, None)
+    L.call("axspmv_device_sync")
+    dt = (time.perf_counter() - t0) / reps
+    res["copy_256MB"] = dict(us=round(dt * 1e6, 1), gbs_read_plus_write=round(16.0 * n / dt / 1e9, 1))
+    del src, dst
+    m = ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
+    sweep = []
+    for fold in (1, 2, 4, 8, 64):
+        ncols = m.ncols // fold
+        # same rows and row lengths, columns folded into [0, ncols / fold)
+        mm = ax.CsrMatrix(m.row, (m.col % ncols).astype(np.uint32), m.val, ncols=m.ncols)
+        x = ax.uniform(1, m.ncols)
+        dx, dy = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer(8 * m.nrows)
+        for name, fmt, kw in (FLAGSHIP_AXT, FLAGSHIP):
+            p = ax.Plan(mm, fmt, **kw)
+            p.time(dx, dy, iters=5, warmup=0, mode=TIME_REGION)
+            t = p.time(dx, dy, iters=30, warmup=0, mode=TIME_REGION)
+            sweep.append(dict(x_footprint_mb=round(8 * ncols / 2 ** 20, 3), kernel=p.info()["name"],
+                              us=round(t.mean_ms * 1e3, 1),
+                              ggathers=round(m.nnz / (t.mean_ms * 1e-3) / 1e9, 1)))
+            log(f"calibrate: {sweep[-1]}")
+    res["x_footprint_sweep"] = sweep
+    return res
+
+
+def pmc_probe():
+    """A 256 MB copy (known bytes: the counter scale) then 5 flagship SpMVs
+    of each of the two flagship kernels."""
+    n = 32 << 20
+    src, dst = ax.DeviceBuffer(8 * n), ax.DeviceBuffer(8 * n)
+    L.call("axspmv_copy_f64", src.ptr, dst.ptr, n, None)
+    L.call("axspmv_device_sync")
+    m = ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
+    x = ax.uniform(1, m.ncols)
+    dx, dy = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer(8 * m.nrows)
+    for _, fmt, kw in (FLAGSHIP_AXT, FLAGSHIP):
+        p = ax.Plan(m, fmt, **kw)
+        for _ in range(5):
+            p.spmv(dx, dy)
+        L.call("axspmv_device_sync")
+        log(f"pmc-probe: {p.info()}")
+
+
+def suite_full(res):
+    """Laplacian (graph-timed), permuted stencil, 256-matrix batch, forward +
+    adjoint on the stencil structure."""
+    from oracle import oracle as orc
+    # config 1: 2D Laplacian 256^2, below launch latency -> graph batches
+    m1 = ax.laplace2d(256)
+    x1 = ax.uniform(0, m1.ncols)
+    y1 = orc.ncsr(m1, x1)
+    d1, e1 = ax.DeviceBuffer.from_array(x1), ax.DeviceBuffer(8 * m1.nrows)
+    c1 = []
+    for name, fmt, kw in (("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+                          ("axt_unc_h1_thw4", ax.FMT_AXT, dict(thw=4)), ("axt_unc_h4_thw64", ax.FMT_AXT, dict(thw=64, th=4)),
+                          ("axc_hbs4", ax.FMT_AXC, dict(hbs=4)), ("k1_c64", ax.FMT_K1, dict(chunk=64)),
+                          ("axt_com_h4_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)),
+                          ("rct_auto", ax.FMT_RCT, dict())):
+        _, _, r = measure(m1, name, fmt, kw, d1, e1, 20, 2, y1, mode=TIME_GRAPH, batch=50)
+        c1.append(r)
+        log(f"laplace256: {r}")
+    res["laplace256"] = c1
+    # config 3: 27-point stencil 128^3, permuted
+    m3 = ax.stencil27(128, permute=True, seed=2)
+    x3 = ax.uniform(2, m3.ncols)
+    y3 = orc.ncsr(m3, x3, nthreads=omp_threads())
+    d3, e3 = ax.DeviceBuffer.from_array(x3), ax.DeviceBuffer(8 * m3.nrows)
+    c3 = []
+    for name, fmt, kw in [FLAGSHIP, FLAGSHIP_AXT] + OTHERS[1:13]:
+        _, _, r = measure(m3, name, fmt, kw, d3, e3, 20, 3, y3)
+        c3.append(r)
+        log(f"stencil128p: {r}")
+    res["stencil128p"] = c3
+    # config 5: forward + adjoint on the stencil structure, values N(1, 0.1)
+    ax.fill_values_normal(m3, 1.0, 0.1, 4)
+    y5 = orc.ncsr(m3, x3, nthreads=omp_threads())
+    pl = ax.Plan(m3, ax.FMT_AXT, thw=4)
+    pl.time(d3, e3, iters=3, warmup=0, mode=TIME_REGION)
+    plain = pl.time(d3, e3, iters=20, warmup=0, mode=TIME_REGION).mean_ms * 1e3
+    del pl
+    res["stencil128p_fwd_adj"] = fwd_adj_only(m3, x3, d3, y5, plain)
+    del m3, d3, e3
+    # config 4B: 256 uneven banded matrices in one persistent launch
+    rng = np.random.default_rng(5)
+    mats = []
+    for i in range(256):
+        mats.append(ax.banded(int(rng.integers(1000, 200001)), int(rng.integers(1, 65)), 0.3, seed=5000 + i))
+    tuned, ngroups = ax.autotune_batch(mats, iters=5)
+    res["batch256"] = {}
+    for label, bt in (("uniform_thw4", ax.Batch(mats, thw=4)), ("autotuned", ax.Batch(mats, opts_list=tuned))):
+        info = bt.info()
+        xb = ax.uniform(6, int(bt.xoff[-1]))
+        dxb, dyb = ax.DeviceBuffer.from_array(xb), ax.DeviceBuffer(8 * int(bt.yoff[-1]))
+        bt.spmv(dxb, dyb)
+        L.call("axspmv_device_sync")
+        yb = dyb.download(np.float64, int(bt.yoff[-1]))
+        errs = [rel_err(orc.ncsr(mats[i], xb[bt.xoff[i]:bt.xoff[i + 1]]), yb[bt.yoff[i]:bt.yoff[i + 1]])
+                for i in range(0, 256, 16)]
+        t = bt.time(dxb, dyb, iters=10, warmup=2, mode=TIME_REGION)
+        tg = bt.time(dxb, dyb, iters=5, warmup=1, mode=TIME_GRAPH, batch=10)
+        sec = t.mean_ms * 1e-3
+        res["batch256"][label] = dict(matrices=256, nnz=info["nnz"], items=info["nitems"], waves=info["nwaves"],
+                                      us=round(sec * 1e6, 1), graph_us=round(tg.mean_ms * 1e3, 1),
+                                      alg_gbs=round(info["alg_bytes"] / sec / 1e9, 1),
+                                      frac=round(info["alg_bytes"] / sec / 1e9 / PEAK_HBM_GBS, 4),
+                                      max_rel_err_sampled=max(errs), tuner_groups=ngroups)
+        log(f"batch256 {label}: {res['batch256'][label]}")
+        del dxb, dyb
+
+
+def fwd_adj_only(m, x, dx, yref, plain_us):
+    from oracle import oracle as orc
+    n, nc, nnz = m.nrows, m.ncols, m.nnz
+    b = ax.uniform(4, n, stream=ax.VEC_STREAM + 1)
+    p = ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_TRANSPOSE)
+    db = ax.DeviceBuffer.from_array(b)
+    bufs = dict(y=ax.DeviceBuffer(8 * n), r=ax.DeviceBuffer(8 * n), xbar=ax.DeviceBuffer(8 * nc),
+                vbar=ax.DeviceBuffer(8 * nnz))
+    fa = {}
+    for label, op in (("fused", L.OP_FWD_ADJ), ("split", L.OP_FWD_ADJ_SPLIT)):
+        t = p.time_op(op, iters=20, warmup=2, x=dx, b=db, **bufs)
+        fa[label] = round(t.mean_ms * 1e3, 2)
+    r = bufs["r"].download(np.float64, n)
+    xbar = bufs["xbar"].download(np.float64, nc)
+    r_ref = yref - b
+    out = dict(plain_spmv_us=round(plain_us, 2), fused_us=fa["fused"], split_us=fa["split"],
+               fused_over_plain=round(fa["fused"] / plain_us, 3), r_rel_err=rel_err(r_ref, r),
+               xbar_rel_err=rel_err(orc.ncsr_t(m, r_ref, nc), xbar))
+    log(f"stencil fwd_adj: {out}")
+    return out
 
 
 def main():
@@ -103,13 +319,22 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--formats", choices=["flagship", "all"], default="all")
+    ap.add_argument("--suite", choices=["headline", "full"], default="headline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: SpMV here runs on a single MI355X; "
               "multi-GPU row-block sharding is a design note, not an implemented path.", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate:
+        print(json.dumps(calibrate()), flush=True)
+        return
+    if args.pmc_probe:
+        pmc_probe()
+        return
 
     t0 = time.perf_counter()
     m = ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
@@ -130,16 +355,18 @@ def main():
     # The timed region: K SpMVs back to back, device-synchronized on both
     # sides; HIP events on the launch stream bracket the same region.
     plan.time(dx, dy, iters=max(args.warmup, 1), warmup=0, mode=TIME_REGION)
-    ax._lib.call("axspmv_device_sync")
+    L.call("axspmv_device_sync")
     w0 = time.perf_counter()
     t = plan.time(dx, dy, iters=args.steps, warmup=0, mode=TIME_REGION)
-    ax._lib.call("axspmv_device_sync")
+    L.call("axspmv_device_sync")
     wall = time.perf_counter() - w0
     kern_s = t.mean_ms * 1e-3
     achieved = info["alg_bytes"] / kern_s / 1e9
     gflops = 2.0 * m.nnz / (wall / args.steps) / 1e9
+    del plan
 
     formats = [flag]
+    extras = {}
     if args.formats == "all":
         for (n2, f2, k2) in OTHERS:
             try:
@@ -148,12 +375,18 @@ def main():
                 r = dict(label=n2, error=str(e))
             log(f"{n2}: {r}")
             formats.append(r)
+        plain = next(f["us"] for f in formats if f.get("label") == FLAGSHIP_AXT[0])
+        extras = extras_same_matrix(m, x, dx, yref, plain)
+    if args.suite == "full":
+        suite_full(extras)
 
     traffic, traffic_src = None, None
-    tpath = os.path.join(ROOT, "profiles", "traffic_axt_unc_h1_thw4_powerlaw.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_powerlaw_flagship.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
-        traffic, traffic_src = tj.get("hbm_bytes_per_launch"), os.path.relpath(tpath, ROOT)
+        entry = tj.get(info["name"], {})
+        traffic = entry.get("hbm_bytes_per_launch")
+        traffic_src = os.path.relpath(tpath, ROOT) if traffic is not None else None
 
     out = {
         "metric": "SpMV GFLOP/s + HBM GB/s (% of 8 TB/s) per format; fwd+adjoint vs plain",
@@ -168,7 +401,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded power-law matrix generated on the host)",
-        "config": {"workload": "powerlaw n=2^20 Zipf(2) rl<=2048 mean 20 seed 1, FP64 SpMV, AXT UNC th=1 thw=4",
+        "config": {"workload": "powerlaw n=2^20 Zipf(2) rl<=2048 mean 20 seed 1, FP64 SpMV",
                    "nrows": m.nrows, "nnz": m.nnz, "kernel": info["name"],
                    "alg_bytes_per_spmv": info["alg_bytes"], "parity_rel_err": flag["rel_err"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -176,6 +409,7 @@ def main():
                      "kernel_us": round(t.mean_ms * 1e3, 2), "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "formats": formats,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
 
