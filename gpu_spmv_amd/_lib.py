@@ -36,6 +36,7 @@ F64, F32 = 0, 1
 AXT_UNC, AXT_COM = 0, 1
 BUILD_AX, BUILD_PLANES = 1, 2
 PLAN_TRANSPOSE = 1
+PLAN_RCT_SPLIT = 2
 
 
 class Csr(C.Structure):

@@ -20,10 +20,38 @@
 #include "launch.hpp"
 #include "wave64.hpp"
 
+// Entries per lane per round (each of the two register sets holds this many).
+#ifndef RCT_UNROLL
+#define RCT_UNROLL 4
+#endif
+
 namespace {
 using axspmv::launch::RctArgs;
 
-template <class VT>
+// Entry e of a tile: value val[e]; column and tile row either packed in one
+// word key[e] = (column - slice start) << rbits | row (PACKED) or in col[] /
+// lrow[].
+template <class VT, bool PACKED>
+struct RctEntries {
+    const VT *__restrict__ val;
+    const uint32_t *__restrict__ key;
+    const uint32_t *__restrict__ col;
+    const uint16_t *__restrict__ lrow;
+    uint32_t rbits, cbase;
+    __device__ __forceinline__ void load(uint32_t e, uint32_t &c, uint32_t &r, double &v) const {
+        v = double(val[e]);
+        if constexpr (PACKED) {
+            const uint32_t k = key[e];
+            c = cbase + (k >> rbits);
+            r = k & ((1u << rbits) - 1);
+        } else {
+            c = col[e];
+            r = lrow[e];
+        }
+    }
+};
+
+template <class VT, bool PACKED>
 __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__restrict__ x) {
     extern __shared__ double yl[];
     const uint32_t w = blockIdx.x;
@@ -36,20 +64,60 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     __syncthreads();
     const uint32_t t = rb * a.S + s;  // tile index in storage order
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const VT *__restrict__ val = static_cast<const VT *>(a.val);
-    uint32_t e = beg + tid;
-    for (; e + 3 * nt < end; e += 4 * nt) {
-        const uint32_t e1 = e + nt, e2 = e + 2 * nt, e3 = e + 3 * nt;
-        const uint32_t c0 = a.col[e], c1 = a.col[e1], c2 = a.col[e2], c3 = a.col[e3];
-        const uint32_t r0 = a.lrow[e], r1 = a.lrow[e1], r2 = a.lrow[e2], r3 = a.lrow[e3];
-        const double v0 = double(val[e]) * x[c0], v1 = double(val[e1]) * x[c1];
-        const double v2 = double(val[e2]) * x[c2], v3 = double(val[e3]) * x[c3];
-        atomicAdd(&yl[r0], v0);
-        atomicAdd(&yl[r1], v1);
-        atomicAdd(&yl[r2], v2);
-        atomicAdd(&yl[r3], v3);
+    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.rbits, s * a.C};
+    // Rounds of U entries per lane, in two register sets A and B: a round
+    // issues its x gathers, then the loads of the next round into the other
+    // set, then waits for the gathers only and does its LDS adds.  Loads are
+    // never predicated (a lane past the end re-reads the tile's last entry)
+    // and the product of such a lane is zeroed.  The round count is the same
+    // for the whole workgroup, and the last round (which fetches nothing) is
+    // peeled, so every fetch is used on every path and the compiler has no
+    // reason to sink it behind the LDS adds.
+    constexpr int U = RCT_UNROLL;
+    struct Set {
+        uint32_t c[U], r[U];
+        double v[U];
+    };
+    if (beg < end) {
+        const uint32_t last = end - 1, span = U * nt;
+        const uint32_t nr = (end - beg + span - 1) / span;
+        auto fetch = [&](Set &S, uint32_t e0) {
+#pragma unroll
+            for (int j = 0; j < U; ++j) ent.load(min(e0 + j * nt, last), S.c[j], S.r[j], S.v[j]);
+        };
+        auto add = [&](const Set &S, const double *xv, uint32_t e0) {
+#pragma unroll
+            for (int j = 0; j < U; ++j) atomicAdd(&yl[S.r[j]], e0 + j * nt < end ? S.v[j] * xv[j] : 0.0);
+        };
+        auto round = [&](const Set &S, Set &N, uint32_t e0) {
+            double xv[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) xv[j] = x[S.c[j]];
+            fetch(N, e0 + span);
+            add(S, xv, e0);
+        };
+        auto final_round = [&](const Set &S, uint32_t e0) {
+            double xv[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) xv[j] = x[S.c[j]];
+            add(S, xv, e0);
+        };
+        Set A, B;
+        uint32_t e = beg + tid;
+        fetch(A, e);
+        uint32_t k = 1;  // rounds k - 1 and k run in one pass; both have a successor
+        for (; k + 1 < nr; k += 2) {
+            round(A, B, e);
+            round(B, A, e + span);
+            e += 2 * span;
+        }
+        if (k < nr) {
+            round(A, B, e);
+            final_round(B, e + span);
+        } else {
+            final_round(A, e);
+        }
     }
-    for (; e < end; e += nt) atomicAdd(&yl[a.lrow[e]], double(val[e]) * x[a.col[e]]);
     __syncthreads();
     const uint32_t row0 = rb * R;
     const uint32_t rows = min(R, a.nrows - row0);
@@ -81,10 +149,18 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s) {
     // With S == 1 the tile's sums are the rows' totals: write y directly.
     RctArgs b = a;
     if (a.S == 1) b.part = y;
-    if (a.vtype == 1)
-        rct_kernel<float><<<grid, a.block, lds, s>>>(b, x);
-    else
-        rct_kernel<double><<<grid, a.block, lds, s>>>(b, x);
+    const bool packed = a.key != nullptr;
+    if (a.vtype == 1) {
+        if (packed)
+            rct_kernel<float, true><<<grid, a.block, lds, s>>>(b, x);
+        else
+            rct_kernel<float, false><<<grid, a.block, lds, s>>>(b, x);
+    } else {
+        if (packed)
+            rct_kernel<double, true><<<grid, a.block, lds, s>>>(b, x);
+        else
+            rct_kernel<double, false><<<grid, a.block, lds, s>>>(b, x);
+    }
     if (a.S > 1) rct_merge<<<(a.nrows + 255) / 256, 256, 0, s>>>(a.part, a.S, a.nrows, y);
     return hipGetLastError();
 }

@@ -62,11 +62,13 @@ hipError_t spmm(const SpmmArgs &a, uint32_t k, uint32_t kt, const double *X, dou
 // nrows partial sums.
 struct RctArgs {
     const void *val;
-    const uint32_t *col;
-    const uint16_t *lrow;
+    const uint32_t *col;    // unpacked: column of each entry
+    const uint16_t *lrow;   // unpacked: row within the tile
+    const uint32_t *key;    // packed: (column - slice start) << rbits | row
     const uint32_t *tile_ptr;
     double *part;
-    uint32_t nrows, R, S, nrb, vtype, block;
+    uint32_t *arrived;      // per row block: tiles done (0 between calls)
+    uint32_t nrows, R, S, nrb, vtype, block, rbits, C;
 };
 hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s);
 

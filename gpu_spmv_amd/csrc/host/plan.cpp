@@ -452,35 +452,62 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
             }
         }
     }
-    if (cnt[ntile] >= (1ull << 32)) return fail(AXSPMV_E_OVERFLOW, "RCT: too many entries");
+    // the kernel indexes entries in 32 bits with up to 2 * 8 * 1024 past the end
+    if (cnt[ntile] >= (1ull << 32) - (1u << 16)) return fail(AXSPMV_E_OVERFLOW, "RCT: too many entries");
     std::vector<uint32_t> tptr(ntile + 1);
     for (uint64_t t = 0; t <= ntile; ++t) tptr[t] = uint32_t(cnt[t]);
     const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
+    // Pack (column within the slice, row within the tile) into one word when
+    // both fit in 32 bits.
+    auto bits = [](uint64_t v) {  // bits to hold values 0 .. v-1
+        uint32_t b = 1;
+        while ((1ull << b) < v) ++b;
+        return b;
+    };
+    const uint32_t rbits = bits(R), cbits = bits(C);
+    const bool packed = rbits + cbits <= 32 && !(p.o.flags & AXSPMV_PLAN_RCT_SPLIT);
     void *dv = p.dalloc(vs * nnz);
-    auto *dc = static_cast<uint32_t *>(p.dalloc(4 * nnz));
-    auto *dl = static_cast<uint16_t *>(p.dalloc(2 * nnz));
     auto *dt = static_cast<uint32_t *>(p.dalloc(4 * tptr.size()));
     auto *part = static_cast<double *>(p.dalloc(S > 1 ? 8ull * S * n : 8));
-    if (!dv || !dc || !dl || !dt || !part) return AXSPMV_E_NOMEM;
+    auto *arrived = static_cast<uint32_t *>(p.dalloc(4ull * nrb));
+    if (!dv || !dt || !part || !arrived) return AXSPMV_E_NOMEM;
+    if (hipMemsetAsync(arrived, 0, 4ull * nrb, s) != hipSuccess) return fail(AXSPMV_E_HIP, "RCT: counter reset");
     if (p.o.vtype == AXSPMV_F32) {
         std::vector<float> v(val.begin(), val.begin() + nnz);
         AX_TRY(p.upload(dv, v.data(), 4 * nnz, s));
     } else {
         AX_TRY(p.upload(dv, val.data(), 8 * nnz, s));
     }
-    AX_TRY(p.upload(dc, col.data(), 4 * nnz, s));
-    AX_TRY(p.upload(dl, lrow.data(), 2 * nnz, s));
     AX_TRY(p.upload(dt, tptr.data(), 4 * tptr.size(), s));
+    uint32_t *dc = nullptr, *dk = nullptr;
+    uint16_t *dl = nullptr;
+    if (packed) {
+        std::vector<uint32_t> key(nnz ? nnz : 1);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (uint64_t t = 0; t < ntile; ++t) {
+            const uint32_t cb = uint32_t(t % S) * C;
+            for (uint64_t q = cnt[t]; q < cnt[t + 1]; ++q) key[q] = ((col[q] - cb) << rbits) | lrow[q];
+        }
+        dk = static_cast<uint32_t *>(p.dalloc(4 * nnz));
+        if (!dk) return AXSPMV_E_NOMEM;
+        AX_TRY(p.upload(dk, key.data(), 4 * nnz, s));
+    } else {
+        dc = static_cast<uint32_t *>(p.dalloc(4 * nnz));
+        dl = static_cast<uint16_t *>(p.dalloc(2 * nnz));
+        if (!dc || !dl) return AXSPMV_E_NOMEM;
+        AX_TRY(p.upload(dc, col.data(), 4 * nnz, s));
+        AX_TRY(p.upload(dl, lrow.data(), 2 * nnz, s));
+    }
     uint32_t block = p.o.block;
     if (block == 0 || block > 1024 || block % 64) block = 1024;
-    p.rct = launch::RctArgs{dv, dc, dl, dt, part, n, R, S, nrb, p.o.vtype, block};
+    p.rct = launch::RctArgs{dv, dc, dl, dk, dt, part, arrived, n, R, S, nrb, p.o.vtype, block, rbits, C};
     p.slots = nnz;
     // matrix stream + per-slice partial write/read when S > 1
-    p.stream_bytes = (vs + 6) * nnz + 4 * tptr.size() + (S > 1 ? 16ull * S * n : 0);
+    p.stream_bytes = (vs + (packed ? 4 : 6)) * nnz + 4 * tptr.size() + (S > 1 ? 16ull * S * n : 0);
     p.launches = S > 1 ? 2 : 1;
     p.nblocks = uint32_t(ntile);
     p.kind = Plan::RCT;
-    std::snprintf(p.name, sizeof p.name, "rct_r%u_s%u", R, S);
+    std::snprintf(p.name, sizeof p.name, "rct_r%u_s%u%s", R, S, packed ? "" : "_split");
     return AXSPMV_OK;
 }
 

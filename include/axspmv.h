@@ -237,6 +237,9 @@ typedef struct axspmv_opts {
 } axspmv_opts;
 
 #define AXSPMV_PLAN_TRANSPOSE 1u /* also build A^T (needed by the adjoint) */
+#define AXSPMV_PLAN_RCT_SPLIT 2u /* RCT: keep column (4 B) and tile row (2 B) of
+                                    each entry apart even when both fit one
+                                    32-bit key */
 
 /* Fill `o` with the defaults of a format (every field defined). */
 int axspmv_opts_default(uint32_t format, axspmv_opts *o);

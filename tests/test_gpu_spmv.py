@@ -52,6 +52,8 @@ KERNELS = [
     ("rct_r4096_s8", ax.FMT_RCT, dict(tile_rows=4096, slices=8)),
     ("rct_r8192_s2_b256", ax.FMT_RCT, dict(tile_rows=8192, slices=2, block=256)),
     ("rct_r512_s1", ax.FMT_RCT, dict(tile_rows=512, slices=1)),
+    ("rct_r2048_s4_split", ax.FMT_RCT, dict(tile_rows=2048, slices=4, flags=ax.PLAN_RCT_SPLIT)),
+    ("rct_r16384_s1_split_b512", ax.FMT_RCT, dict(tile_rows=16384, slices=1, block=512, flags=ax.PLAN_RCT_SPLIT)),
 ]
 
 
@@ -112,7 +114,8 @@ def test_spmv_f64(mname, kname, fmt, kw):
 
 
 F32_KERNELS = [k for k in KERNELS if k[0] in ("csr_naive", "unc_h1_thw4", "unc_h4_thw64", "k1_c64", "axc_h8",
-                                                "com_h1_thw64_bs256", "com_h8_thw64", "rct_r1024_s4")]
+                                                "com_h1_thw64_bs256", "com_h8_thw64", "rct_r1024_s4",
+                                                "rct_r2048_s4_split")]
 
 
 @pytest.mark.parametrize("mname", ["rand200", "powerlaw64k", "long_rows"])
