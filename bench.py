@@ -56,6 +56,7 @@ OTHERS = [
     ("rct_r8192_s4", ax.FMT_RCT, dict(tile_rows=8192, slices=4)),
     ("rct_r8192_s8", ax.FMT_RCT, dict(tile_rows=8192, slices=8)),
     ("rct_r16384_s2", ax.FMT_RCT, dict(tile_rows=16384, slices=2)),
+    ("axt_unc_h1_thw4_ax", ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_AX_LAYOUT)),
 ]
 
 
@@ -166,6 +167,33 @@ def extras_same_matrix(m, x, dx, yref, plain_us):
             log(f"spmm: {sp[-1]}")
         del dX, dY
     out["spmm"] = sp
+    return out
+
+
+def conversion(m, x, dx):
+    """Format conversion time (the reference's `ct`): host builders against
+    the device builders, and the per-SpMV x refresh (`ot`) of the AX layout."""
+    out = []
+    for label, host, dev in (
+            ("faxtuh1hw4", lambda: ax.build_axt(m, x, 256, 4, 1, ax.AXT_UNC),
+             lambda: ax.dev_build_axt(m, x, 256, 4, 1, ax.AXT_UNC)),
+            ("faxtuh4hw64", lambda: ax.build_axt(m, x, 256, 64, 4, ax.AXT_UNC),
+             lambda: ax.dev_build_axt(m, x, 256, 64, 4, ax.AXT_UNC)),
+            ("faxtch1hw64bs256", lambda: ax.build_axt(m, x, 256, 64, 1, ax.AXT_COM),
+             lambda: ax.dev_build_axt(m, x, 256, 64, 1, ax.AXT_COM)),
+            ("faxc_hbs4", lambda: ax.build_axc(m, x, 4), lambda: ax.dev_build_axc(m, x, 4))):
+        h = host()
+        dev()  # first call pays for code loading
+        d = dev()
+        same = bool(np.array_equal(h["ax"], d["ax"]))
+        out.append(dict(format=label, host_ms=round(h["fd"]["ct"] * 1e3, 2), host_threads=omp_threads(),
+                        device_ms=round(d["ms"], 3), ax_identical=same))
+        log(f"conversion: {out[-1]}")
+        del h, d
+    p = ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_AX_LAYOUT)
+    t = p.time_op(L.OP_XREFRESH, iters=30, warmup=3, x=dx)
+    out.append(dict(format="axt_unc_h1_thw4_ax", x_refresh_us=round(t.mean_ms * 1e3, 2)))
+    log(f"conversion: {out[-1]}")
     return out
 
 
@@ -377,6 +405,7 @@ def main():
             formats.append(r)
         plain = next(f["us"] for f in formats if f.get("label") == FLAGSHIP_AXT[0])
         extras = extras_same_matrix(m, x, dx, yref, plain)
+        extras["conversion"] = conversion(m, x, dx)
     if args.suite == "full":
         suite_full(extras)
 

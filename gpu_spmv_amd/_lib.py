@@ -37,6 +37,7 @@ AXT_UNC, AXT_COM = 0, 1
 BUILD_AX, BUILD_PLANES = 1, 2
 PLAN_TRANSPOSE = 1
 PLAN_RCT_SPLIT = 2
+PLAN_AX_LAYOUT = 4
 
 
 class Csr(C.Structure):
@@ -70,6 +71,19 @@ class Axt(C.Structure):
                 ("cplane", P(u32))]
 
 
+class Dcsr(C.Structure):
+    _fields_ = [("nrows", u32), ("ncols", u32), ("nnz", u32), ("pad_", u32),
+                ("row", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p)]
+
+
+class Dfmt(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("format", u32), ("mode", u32), ("hw", u32), ("th", u32),
+                ("bs", u32), ("log", u32), ("nrows", u32), ("nnz", u32), ("tileN", u32), ("lenAX", u32),
+                ("lenSEC", u32), ("lenCON", u32), ("lenMAPX", u32), ("pad_", u32),
+                ("ax", C.c_void_p), ("sec", C.c_void_p), ("con", C.c_void_p), ("plane", C.c_void_p),
+                ("ms", f64)]
+
+
 class Opts(C.Structure):
     _fields_ = [("format", u32), ("kernel", u32), ("mode", u32), ("thw", u32), ("th", u32),
                 ("bs", u32), ("hbs", u32), ("chunk", u32), ("reorder", u32), ("vtype", u32),
@@ -94,7 +108,7 @@ class Timing(C.Structure):
 
 
 # enum axspmv_op
-OP_SPMV, OP_ADJOINT, OP_VBAR, OP_FWD_ADJ, OP_FWD_ADJ_SPLIT, OP_SPMM = 0, 1, 2, 3, 4, 5
+OP_SPMV, OP_ADJOINT, OP_VBAR, OP_FWD_ADJ, OP_FWD_ADJ_SPLIT, OP_SPMM, OP_XREFRESH = 0, 1, 2, 3, 4, 5, 6
 
 
 class Operands(C.Structure):
@@ -129,6 +143,9 @@ SIGNATURES = {
     "axspmv_axc_free": (None, [P(Axc)]),
     "axspmv_build_axt": (C.c_int, [P(Csr), P(f64), u32, u32, u32, u32, u32, P(P(Axt)), P(FmtData)]),
     "axspmv_axt_free": (None, [P(Axt)]),
+    "axspmv_dev_build_axt": (C.c_int, [P(Dcsr), vp, u32, u32, u32, u32, vp, P(P(Dfmt))]),
+    "axspmv_dev_build_axc": (C.c_int, [P(Dcsr), vp, u32, vp, P(P(Dfmt))]),
+    "axspmv_dfmt_free": (None, [P(Dfmt)]),
     "axspmv_opts_default": (C.c_int, [u32, P(Opts)]),
     "axspmv_plan_create": (C.c_int, [P(Csr), P(Opts), vp, P(vp)]),
     "axspmv_plan_destroy": (C.c_int, [vp]),
@@ -159,6 +176,7 @@ SIGNATURES = {
     "axspmv_device_sync": (C.c_int, []),
     "axspmv_device_count": (C.c_int, [P(C.c_int)]),
     "axspmv_copy_f64": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "axspmv_x_refresh": (C.c_int, [vp, u64, u32, u32, vp, vp, vp]),
 }
 
 _lib = None

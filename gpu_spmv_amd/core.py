@@ -342,6 +342,49 @@ def _dptr(d):
     return int(d)
 
 
+def _dev_build(m: CsrMatrix, x, fn: str, *args) -> dict:
+    """Upload m (and x), run a device builder, download its arrays."""
+    drow = DeviceBuffer.from_array(m.row)
+    dcol = DeviceBuffer.from_array(m.col if m.nnz else np.zeros(1, np.uint32))
+    dval = DeviceBuffer.from_array(m.val if m.nnz else np.zeros(1))
+    dx = DeviceBuffer.from_array(np.ascontiguousarray(x, np.float64)) if x is not None else None
+    a = L.Dcsr(m.nrows, m.ncols, m.nnz, 0, drow.ptr, dcol.ptr, dval.ptr)
+    out = C.POINTER(L.Dfmt)()
+    L.call(fn, C.byref(a), dx.ptr if dx is not None else None, *args, None, C.byref(out))
+    f = out.contents
+
+    def down(ptr, n, dt):
+        h = np.empty(n, dt)
+        if n:
+            L.call("axspmv_memcpy_d2h", h.ctypes.data, ptr, h.nbytes, None)
+        return h
+    try:
+        r = dict(name=f.name.decode(), tileN=f.tileN, lenAX=f.lenAX, lenSEC=f.lenSEC, lenCON=f.lenCON,
+                 lenMAPX=f.lenMAPX, log=f.log, ms=f.ms, ax=down(f.ax, f.lenAX, np.float64),
+                 con=down(f.con, f.nrows + 1, np.uint32), plane=down(f.plane, f.lenMAPX, np.uint32),
+                 sec=down(f.sec, f.lenSEC, np.uint32) if f.sec else None)
+        L.call("axspmv_stream_sync", None)
+    finally:
+        L.load().axspmv_dfmt_free(out)
+    return r
+
+
+def dev_build_axt(m: CsrMatrix, x: np.ndarray | None, bs: int, thw: int, th: int, mode: int) -> dict:
+    """axspmv_dev_build_axt: the AXT arrays built on the device (downloaded)."""
+    return _dev_build(m, x, "axspmv_dev_build_axt", bs, thw, th, mode)
+
+
+def dev_build_axc(m: CsrMatrix, x: np.ndarray | None, hbs: int) -> dict:
+    """axspmv_dev_build_axc: the AXC arrays built on the device (downloaded)."""
+    return _dev_build(m, x, "axspmv_dev_build_axc", hbs)
+
+
+def x_refresh(dmap, nslots: int, hw: int, nx: int, dx, dax, stream=None) -> None:
+    """Fill the x-halves of an interleaved ax array on the device:
+    ax[(q // hw) * 2hw + hw + q % hw] = x[map[q]] (0.0 where map[q] >= nx)."""
+    L.call("axspmv_x_refresh", _dptr(dmap), nslots, hw, nx, _dptr(dx), _dptr(dax), stream)
+
+
 def spmv_host(plan: Plan, x: np.ndarray) -> np.ndarray:
     """Upload x, run one SpMV, download y (test convenience)."""
     dx = DeviceBuffer.from_array(np.ascontiguousarray(x, np.float64))

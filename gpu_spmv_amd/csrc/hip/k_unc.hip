@@ -180,6 +180,14 @@ hipError_t axt_unc_h(const SegArgs &a, const double *x, double *y, hipStream_t s
     return a.vtype == 1 ? unc_h_vt<float>(a, x, y, s) : unc_h_vt<double>(a, x, y, s);
 }
 
+hipError_t axt_unc_h1_ax(const SegArgs &a, double *y, hipStream_t s) {
+    const uint32_t thw_log = __builtin_ctz(a.thw);
+    const double *ax = static_cast<const double *>(a.vplane);
+    if (a.thw >= 4) return run(AxH1Src<4>{ax, a.rowsrc, thw_log - 2, thw_log}, a, nullptr, y, s);
+    if (a.thw == 2) return run(AxH1Src<2>{ax, a.rowsrc, 0, thw_log}, a, nullptr, y, s);
+    return run(AxH1Src<1>{ax, a.rowsrc, 0, thw_log}, a, nullptr, y, s);
+}
+
 namespace {
 template <class VT, int K>
 hipError_t vbar_vk(const SegArgs &a, uint32_t ushift, uint32_t nunits, const double *x, const double *ybar,

@@ -1,5 +1,6 @@
 // Utility kernels: a plain copy (the bandwidth reference used to calibrate
-// the HBM byte counters) and a fill.
+// the HBM byte counters), fills, and the x-half refresh of the interleaved
+// ax layouts.
 #include "launch.hpp"
 #include "wave64.hpp"
 
@@ -24,6 +25,19 @@ __global__ void __launch_bounds__(256) fill_rows_kernel(const uint32_t *__restri
     if (i < n) y[rows[i]] = 0.0;
 }
 
+// One lane per value slot: map loads and ax stores are contiguous within each
+// group of hw slots; x is gathered.
+__global__ void __launch_bounds__(256) x_refresh_kernel(const uint32_t *__restrict__ map, size_t n, uint32_t hw_log,
+                                                        uint32_t nx, const double *__restrict__ x,
+                                                        double *__restrict__ ax) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    const size_t hw = size_t(1) << hw_log;
+    for (size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride) {
+        const uint32_t c = map[q];
+        ax[((q >> hw_log) << (hw_log + 1)) + hw + (q & (hw - 1))] = c < nx ? x[c] : 0.0;
+    }
+}
+
 unsigned grid_for(size_t work, unsigned block) {
     size_t g = (work + block - 1) / block;
     const size_t cap = 256 * 64;  // 256 CUs x 64 workgroups: enough to fill the chip
@@ -46,6 +60,12 @@ hipError_t copy_f64(const double *src, double *dst, size_t n, hipStream_t s) {
 
 hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s) {
     if (n) fill_kernel<<<grid_for(n, 256), 256, 0, s>>>(dst, n, v);
+    return hipGetLastError();
+}
+
+hipError_t x_refresh(const uint32_t *map, uint64_t n, uint32_t hw_log, uint32_t nx, const double *x, double *ax,
+                     hipStream_t s) {
+    if (n) x_refresh_kernel<<<grid_for(n, 256), 256, 0, s>>>(map, n, hw_log, nx, x, ax);
     return hipGetLastError();
 }
 

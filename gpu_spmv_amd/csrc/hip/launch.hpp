@@ -27,6 +27,12 @@ struct SegArgs {
     uint32_t block;           // threads per workgroup
 };
 hipError_t axt_unc_h1(const SegArgs &a, const double *x, double *y, hipStream_t s);
+// As axt_unc_h1 on the interleaved layout: a.vplane is the FP64 ax array
+// (tiles of thw values then thw x-values, x-halves already refreshed).
+hipError_t axt_unc_h1_ax(const SegArgs &a, double *y, hipStream_t s);
+// ax[(q >> hw_log) * 2hw + hw + q % hw] = map[q] < nx ? x[map[q]] : 0 for q < n.
+hipError_t x_refresh(const uint32_t *map, uint64_t n, uint32_t hw_log, uint32_t nx, const double *x, double *ax,
+                     hipStream_t s);
 hipError_t axt_unc_h(const SegArgs &a, const double *x, double *y, hipStream_t s);
 
 // Plane slot -> CSR position of the AXT UNC th == 1 layout.
@@ -145,6 +151,30 @@ struct ComArgs {
     uint32_t nempty;
 };
 hipError_t axt_com(const ComArgs &a, const double *x, double *y, hipStream_t s);
+
+// Format builders on the device (CSR arrays already in device memory).
+struct DevCsr {
+    uint32_t nrows, ncols, nnz;
+    const uint32_t *row, *col;
+    const double *val;
+};
+struct BuildGeo {
+    uint32_t hw, th, bs, log;  // tile half width (AXC: hbs), tile height, COM block size, header shift
+};
+struct BuildOut {
+    double *ax;        // interleaved values / x values (zero filled beforehand)
+    uint32_t *plane;   // column of every value slot (AXC: mapx)
+    uint32_t *sec;     // AXT row / header words (unused by AXC)
+    uint32_t *con;     // AXT con, AXC brp
+};
+// cnt[r] = ceil(rl_r / div) for r < nrows
+hipError_t build_count(const DevCsr &a, uint32_t div, uint64_t *cnt, hipStream_t s);
+// rocPRIM exclusive scan; tmp == nullptr queries tmp_bytes
+hipError_t build_scan(const uint64_t *in, uint64_t *out, size_t n, void *tmp, size_t &tmp_bytes, hipStream_t s);
+// kind 0: AXC / AXT UNC th == 1, 1: AXT UNC th > 1, 2: COM th == 1, 3: COM th > 1;
+// start = scanned counts (nrows + 1 entries) for kinds 0 and 1
+hipError_t build_fill(const DevCsr &a, int kind, const uint64_t *start, const BuildGeo &g, const double *x,
+                      const BuildOut &o, hipStream_t s);
 
 hipError_t copy_f64(const double *src, double *dst, size_t n, hipStream_t s);
 hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s);

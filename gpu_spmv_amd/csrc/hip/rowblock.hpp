@@ -110,6 +110,29 @@ struct UncH1Src {
     }
 };
 
+// th == 1 on the interleaved ax array: unit u covers value slots [u*K,
+// u*K + K) of tile t = (u*K) / thw, stored at ax[t*2thw + (u*K) % thw] with
+// their x values thw further on.
+template <int K>
+struct AxH1Src {
+    const double *__restrict__ ax;
+    const uint32_t *__restrict__ sec;
+    uint32_t ushift;   // log2(thw / K)
+    uint32_t thw_log;
+    __device__ __forceinline__ double sum(uint32_t u, const double *__restrict__) const {
+        const size_t q = size_t(u) * K;
+        const size_t p = ((q >> thw_log) << (thw_log + 1)) + (q & ((size_t(1) << thw_log) - 1));
+        double v[K], xv[K];
+        VecLoad<double, K>::get(ax + p, v);
+        VecLoad<double, K>::get(ax + p + (size_t(1) << thw_log), xv);
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc = fma(v[j], xv[j], acc);
+        return acc;
+    }
+    __device__ __forceinline__ uint32_t row(uint32_t u) const { return sec[u >> ushift]; }
+};
+
 // th > 1: unit g is global tile column g; level f sits at
 // (g/thw)*thw*th + f*thw + g%thw.  TH > 0 fixes th at compile time.
 template <class VT, int TH>

@@ -1,0 +1,205 @@
+// Format builders on the device: parameter checks, sizing and allocation on
+// the host; count, scan and fill kernels on the device (k_build.hip).  The
+// arrays equal the host builders' (builders.cpp) bit for bit.
+#include <cstdio>
+#include <vector>
+
+#include "../hip/launch.hpp"
+#include "common.hpp"
+#include "plan.hpp"
+
+using namespace axspmv;
+
+namespace {
+
+int check_dcsr(const axspmv_dcsr *a, const char *who) {
+    if (!a) return fail(AXSPMV_E_INVALID, "%s: NULL matrix", who);
+    if (!a->row || (a->nnz && (!a->col || !a->val))) return fail(AXSPMV_E_INVALID, "%s: NULL array", who);
+    return AXSPMV_OK;
+}
+
+struct Events {
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~Events() {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    hipError_t create() {
+        for (hipEvent_t &x : e) {
+            const hipError_t r = hipEventCreate(&x);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+};
+
+struct Scratch {
+    std::vector<void *> p;
+    ~Scratch() {
+        for (void *q : p) (void)hipFree(q);
+    }
+    void *alloc(size_t bytes) {
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes ? bytes : 8) != hipSuccess) return nullptr;
+        p.push_back(q);
+        return q;
+    }
+};
+
+#define HIP_TRY(expr, what)                              \
+    do {                                                 \
+        const hipError_t e_ = (expr);                    \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+// kind 0: AXC / AXT UNC th == 1 (count = tiles of hw), 1: AXT UNC th > 1
+// (count = tile columns of th), 2 / 3: COM th == 1 / th > 1 (no count).
+int run_build(const axspmv_dcsr *a, const double *d_x, int kind, const launch::BuildGeo &g, bool axc,
+              hipStream_t s, axspmv_dfmt *f) {
+    const uint32_t n = a->nrows;
+    const launch::DevCsr dc{n, a->ncols, a->nnz, a->row, a->col, a->val};
+    const uint64_t ths = uint64_t(g.hw) * g.th;
+    Events ev;
+    HIP_TRY(ev.create(), "dev build: events");
+    Scratch scratch;
+    uint64_t *start = nullptr;
+    uint64_t total = 0;
+    float ms_scan = 0.0f;
+    if (kind <= 1) {
+        auto *cnt = static_cast<uint64_t *>(scratch.alloc(8ull * (n + 1)));
+        start = static_cast<uint64_t *>(scratch.alloc(8ull * (n + 1)));
+        size_t tmp_bytes = 0;
+        HIP_TRY(launch::build_scan(cnt, start, size_t(n) + 1, nullptr, tmp_bytes, s), "dev build: scan size");
+        void *tmp = scratch.alloc(tmp_bytes);
+        if (!cnt || !start || !tmp) return fail(AXSPMV_E_NOMEM, "dev build: scratch allocation failed");
+        HIP_TRY(hipMemsetAsync(cnt + n, 0, 8, s), "dev build: memset");
+        HIP_TRY(hipEventRecord(ev.e[0], s), "dev build: event");
+        HIP_TRY(launch::build_count(dc, kind == 0 ? g.hw : g.th, cnt, s), "dev build: count");
+        HIP_TRY(launch::build_scan(cnt, start, size_t(n) + 1, tmp, tmp_bytes, s), "dev build: scan");
+        HIP_TRY(hipEventRecord(ev.e[1], s), "dev build: event");
+        HIP_TRY(hipMemcpyAsync(&total, start + n, 8, hipMemcpyDeviceToHost, s), "dev build: total");
+        HIP_TRY(hipStreamSynchronize(s), "dev build: sync");
+        HIP_TRY(hipEventElapsedTime(&ms_scan, ev.e[0], ev.e[1]), "dev build: elapsed");
+    }
+    uint64_t tileN, lenSEC, lenAX;
+    if (kind == 0) {
+        tileN = total;
+        lenSEC = total;
+        lenAX = total * 2 * g.hw;
+    } else if (kind == 1) {
+        tileN = ceil_div(total, g.hw);
+        lenSEC = tileN * g.hw;
+        lenAX = tileN * 2 * ths;
+    } else {
+        tileN = ceil_div(a->nnz, ths);
+        lenSEC = tileN * ths;
+        lenAX = tileN * 2 * ths;
+    }
+    if (lenAX >= (1ull << 32) || lenSEC >= (1ull << 32))
+        return fail(AXSPMV_E_OVERFLOW, "dev build: lenAX %llu exceeds 32 bits", (unsigned long long)lenAX);
+    f->nrows = n;
+    f->nnz = a->nnz;
+    f->tileN = uint32_t(tileN);
+    f->lenAX = uint32_t(lenAX);
+    f->lenSEC = axc ? 0 : uint32_t(lenSEC);
+    f->lenCON = n;
+    f->lenMAPX = axc ? uint32_t(lenAX / 2 + 8) : uint32_t(lenAX / 2);
+    const size_t nplane = axc ? lenAX / 2 + 8 : lenAX / 2;
+    if (hipMalloc(&f->ax, 8 * (lenAX ? lenAX : 1)) != hipSuccess ||
+        hipMalloc(&f->plane, 4 * (nplane ? nplane : 1)) != hipSuccess ||
+        hipMalloc(&f->con, 4ull * (n + 1)) != hipSuccess ||
+        (!axc && hipMalloc(&f->sec, 4 * (lenSEC ? lenSEC : 1)) != hipSuccess))
+        return fail(AXSPMV_E_NOMEM, "dev build: device allocation failed");
+    // zero fill: padding slots, headers and con[nrows] of the th > 1 layouts
+    HIP_TRY(hipMemsetAsync(f->ax, 0, 8 * lenAX, s), "dev build: memset");
+    HIP_TRY(hipMemsetAsync(f->plane, 0, 4 * nplane, s), "dev build: memset");
+    HIP_TRY(hipMemsetAsync(f->con, 0, 4ull * (n + 1), s), "dev build: memset");
+    if (!axc) HIP_TRY(hipMemsetAsync(f->sec, 0, 4 * lenSEC, s), "dev build: memset");
+    HIP_TRY(hipEventRecord(ev.e[2], s), "dev build: event");
+    const launch::BuildOut o{f->ax, f->plane, f->sec, f->con};
+    HIP_TRY(launch::build_fill(dc, kind, start, g, d_x, o, s), "dev build: fill");
+    HIP_TRY(hipEventRecord(ev.e[3], s), "dev build: event");
+    HIP_TRY(hipStreamSynchronize(s), "dev build: sync");
+    float ms_fill = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms_fill, ev.e[2], ev.e[3]), "dev build: elapsed");
+    f->ms = double(ms_scan) + double(ms_fill);
+    return AXSPMV_OK;
+}
+
+}  // namespace
+
+extern "C" void axspmv_dfmt_free(axspmv_dfmt *f) {
+    if (!f) return;
+    if (f->ax) (void)hipFree(f->ax);
+    if (f->sec) (void)hipFree(f->sec);
+    if (f->con) (void)hipFree(f->con);
+    if (f->plane) (void)hipFree(f->plane);
+    std::free(f);
+}
+
+extern "C" int axspmv_dev_build_axt(const axspmv_dcsr *a, const double *d_x, uint32_t bs, uint32_t thw, uint32_t th,
+                                    uint32_t mode, void *stream, axspmv_dfmt **out) {
+    AX_TRY(check_dcsr(a, "dev_build_axt"));
+    if (!out) return fail(AXSPMV_E_INVALID, "dev_build_axt: out is NULL");
+    *out = nullptr;
+    if (thw == 0 || th == 0 || th > 1023) return fail(AXSPMV_E_INVALID, "dev_build_axt: need thw > 0 and 0 < th < 1024");
+    if (mode != AXSPMV_AXT_UNC && mode != AXSPMV_AXT_COM)
+        return fail(AXSPMV_E_INVALID, "dev_build_axt: mode must be UNC or COM");
+    const bool com = mode == AXSPMV_AXT_COM;
+    uint32_t log = 0;
+    if (com && th == 1) {
+        if (!is_pow2(bs) || bs < 2 || bs > 1024)
+            return fail(AXSPMV_E_INVALID, "dev_build_axt: COM th==1 needs bs a power of two in [2,1024], got %u", bs);
+        log = ilog2(bs);
+    } else if (com) {
+        if (!is_pow2(th)) return fail(AXSPMV_E_INVALID, "dev_build_axt: COM needs th a power of two, got %u", th);
+        log = ilog2(th);
+    }
+    if (com && a->nrows && (uint64_t(a->nrows - 1) << log) >= (1ull << 32))
+        return fail(AXSPMV_E_OVERFLOW, "dev_build_axt: row << %u overflows the 32-bit header", log);
+    auto *f = host_calloc<axspmv_dfmt>(1);
+    if (!f) return AXSPMV_E_NOMEM;
+    f->format = AXSPMV_FMT_AXT;
+    f->mode = mode;
+    f->hw = thw;
+    f->th = th;
+    f->bs = bs;
+    f->log = log;
+    const int kind = com ? (th == 1 ? 2 : 3) : (th == 1 ? 0 : 1);
+    if (!com && th == 1)
+        std::snprintf(f->name, sizeof f->name, "faxtuh1hw%u", thw);
+    else if (!com)
+        std::snprintf(f->name, sizeof f->name, "faxtuh%uhw%u", th, thw);
+    else if (th == 1)
+        std::snprintf(f->name, sizeof f->name, "faxtch1hw%ubs%u", thw, bs);
+    else
+        std::snprintf(f->name, sizeof f->name, "faxtch%uhw%u", th, thw);
+    const int rc = run_build(a, d_x, kind, launch::BuildGeo{thw, th, bs, log}, false,
+                             static_cast<hipStream_t>(stream), f);
+    if (rc) {
+        axspmv_dfmt_free(f);
+        return rc;
+    }
+    *out = f;
+    return AXSPMV_OK;
+}
+
+extern "C" int axspmv_dev_build_axc(const axspmv_dcsr *a, const double *d_x, uint32_t hbs, void *stream,
+                                    axspmv_dfmt **out) {
+    AX_TRY(check_dcsr(a, "dev_build_axc"));
+    if (!out || hbs == 0) return fail(AXSPMV_E_INVALID, "dev_build_axc: hbs must be > 0");
+    *out = nullptr;
+    auto *f = host_calloc<axspmv_dfmt>(1);
+    if (!f) return AXSPMV_E_NOMEM;
+    f->format = AXSPMV_FMT_AXC;
+    f->hw = hbs;
+    f->th = 1;
+    std::snprintf(f->name, sizeof f->name, "faxc");
+    const int rc = run_build(a, d_x, 0, launch::BuildGeo{hbs, 1, 0, 0}, true, static_cast<hipStream_t>(stream), f);
+    if (rc) {
+        axspmv_dfmt_free(f);
+        return rc;
+    }
+    *out = f;
+    return AXSPMV_OK;
+}

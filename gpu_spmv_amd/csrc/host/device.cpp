@@ -57,6 +57,16 @@ extern "C" int axspmv_copy_f64(const double *d_src, double *d_dst, size_t n, voi
     return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "copy launch");
 }
 
+extern "C" int axspmv_x_refresh(const uint32_t *d_map, uint64_t nslots, uint32_t hw, uint32_t nx, const double *d_x,
+                                double *d_ax, void *stream) {
+    if (!is_pow2(hw) || hw > 1024) return fail(AXSPMV_E_INVALID, "x_refresh: hw must be a power of two <= 1024, got %u", hw);
+    if (nslots % hw) return fail(AXSPMV_E_INVALID, "x_refresh: %llu slots are not whole groups of %u",
+                                 (unsigned long long)nslots, hw);
+    if (nslots && (!d_map || !d_ax || (nx && !d_x))) return fail(AXSPMV_E_INVALID, "x_refresh: NULL argument");
+    const hipError_t e = launch::x_refresh(d_map, nslots, ilog2(hw), nx, d_x, d_ax, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "x_refresh launch");
+}
+
 namespace {
 
 struct Events {

@@ -258,14 +258,42 @@ int plan_axt_com(Plan &p, const axspmv_csr *a, hipStream_t s) {
     return AXSPMV_OK;
 }
 
+// The reference's interleaved ax of AXT UNC th == 1 (values, x-halves left
+// zero: every SpMV refreshes them) and the column of every value slot, with
+// 0xffffffff on padding slots so the refresh writes 0 there as the builder.
+int upload_ax_layout(Plan &p, const axspmv_csr *a, const axspmv_axt *m, hipStream_t s) {
+    const uint32_t thw = m->tileHW;
+    const uint64_t ns = uint64_t(m->lenAX) / 2;
+    std::vector<double> ax(2 * ns, 0.0);
+    std::vector<uint32_t> map(ns ? ns : 1, 0xffffffffu);
+    for (uint64_t q = 0; q < ns; ++q) ax[(q / thw) * 2 * thw + q % thw] = m->vplane[q];
+    for (uint32_t r = 0; r < a->nrows; ++r) {
+        const uint64_t first = m->con[r] / 2, rl = a->row[r + 1] - a->row[r];
+        for (uint64_t k = 0; k < rl; ++k) map[first + k] = m->cplane[first + k];
+    }
+    p.d_ax = static_cast<double *>(p.dalloc(16 * ns));
+    p.d_axmap = static_cast<uint32_t *>(p.dalloc(4 * ns));
+    if (!p.d_ax || !p.d_axmap) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(p.d_ax, ax.data(), 16 * ns, s));
+    AX_TRY(p.upload(p.d_axmap, map.data(), 4 * ns, s));
+    p.slots = ns;
+    return AXSPMV_OK;
+}
+
 int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
     const uint32_t thw = p.o.thw, th = p.o.th;
     if (!is_pow2(thw) || thw > 64) return fail(AXSPMV_E_UNSUPPORTED, "AXT kernels need thw a power of two <= 64, got %u", thw);
+    const bool axl = p.o.flags & AXSPMV_PLAN_AX_LAYOUT;
+    if (axl && (th != 1 || p.o.mode != AXSPMV_AXT_UNC || p.o.vtype != AXSPMV_F64))
+        return fail(AXSPMV_E_UNSUPPORTED, "AX layout: AXT UNC th = 1 with FP64 values only");
     if (p.o.mode == AXSPMV_AXT_COM) return plan_axt_com(p, a, s);
     axspmv_axt *m = nullptr;
     AX_TRY(axspmv_build_axt(a, nullptr, p.o.bs, thw, th, p.o.mode, AXSPMV_BUILD_PLANES, &m, nullptr));
     std::unique_ptr<axspmv_axt, void (*)(axspmv_axt *)> guard(m, axspmv_axt_free);
-    AX_TRY(upload_planes(p, m->vplane, m->cplane, uint64_t(m->lenAX) / 2, s));
+    if (axl)
+        AX_TRY(upload_ax_layout(p, a, m, s));
+    else
+        AX_TRY(upload_planes(p, m->vplane, m->cplane, uint64_t(m->lenAX) / 2, s));
     p.d_sec = static_cast<uint32_t *>(p.dalloc(4ull * m->lenSEC));
     if (!p.d_sec) return AXSPMV_E_NOMEM;
     AX_TRY(p.upload(p.d_sec, m->sec, 4ull * m->lenSEC, s));
@@ -282,6 +310,14 @@ int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
     AX_TRY(upload_blocks(p, row_blocks(units, budget), empty, s));
     p.thw = thw;
     p.th = th;
+    if (axl) {
+        // refresh: map (4 B) in, x-half (8 B) out per slot; SpMV: 16 B per slot
+        p.stream_bytes = 28ull * p.slots + 4ull * m->lenSEC + 8ull * p.nblocks;
+        p.launches = 2;
+        p.kind = Plan::AXT_UNC_H1_AX;
+        std::snprintf(p.name, sizeof p.name, "axt_unc_h1_thw%u_ax", thw);
+        return AXSPMV_OK;
+    }
     p.stream_bytes = (vs + 4) * p.slots + 4ull * m->lenSEC + 8ull * p.nblocks;
     if (th != 1) {
         p.kind = Plan::AXT_UNC_H;
@@ -620,6 +656,13 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
         case Plan::K1: return launch::k1(p->k1, x, y, s);
         case Plan::AXC: return launch::axc(p->axc, x, y, s);
         case Plan::RCT: return launch::rct(p->rct, x, y, s);
+        case Plan::AXT_UNC_H1_AX: {
+            const hipError_t e = launch::x_refresh(p->d_axmap, p->slots, ilog2(p->thw), p->ncols, x, p->d_ax, s);
+            if (e != hipSuccess) return e;
+            launch::SegArgs g = seg_args(*p, p->d_sec);
+            g.vplane = p->d_ax;
+            return launch::axt_unc_h1_ax(g, y, s);
+        }
         default: return hipErrorInvalidValue;
     }
 }
@@ -701,6 +744,8 @@ hipError_t plan_op(Plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s
         case AXSPMV_OP_FWD_ADJ: return launch_fwd_adj(p, o.x, o.b, o.y, o.r, o.xbar, o.vbar, true, s);
         case AXSPMV_OP_FWD_ADJ_SPLIT: return launch_fwd_adj(p, o.x, o.b, o.y, o.r, o.xbar, o.vbar, false, s);
         case AXSPMV_OP_SPMM: return launch::spmm(p->spmm, o.k, pick_ktile(p, o.k), o.X, o.Y, s);
+        case AXSPMV_OP_XREFRESH:
+            return launch::x_refresh(p->d_axmap, p->slots, ilog2(p->thw), p->ncols, o.x, p->d_ax, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -718,7 +763,12 @@ int plan_op_check(const Plan *p, uint32_t op, const axspmv_operands &o) {
         if (kt != 8 && kt != 16 && kt != 32 && kt != 64) return fail(AXSPMV_E_INVALID, "SpMM: ktile %u not in 8/16/32/64", kt);
         if ((!o.X && p->ncols) || (!o.Y && p->nrows)) return fail(AXSPMV_E_INVALID, "SpMM: NULL operand");
     }
-    if (op > AXSPMV_OP_SPMM) return fail(AXSPMV_E_INVALID, "unknown operation %u", op);
+    if (op == AXSPMV_OP_XREFRESH) {
+        if (p->kind != Plan::AXT_UNC_H1_AX)
+            return fail(AXSPMV_E_UNSUPPORTED, "x refresh: plan %s was not created with AXSPMV_PLAN_AX_LAYOUT", p->name);
+        if (!o.x && p->ncols) return fail(AXSPMV_E_INVALID, "x refresh: NULL x");
+    }
+    if (op > AXSPMV_OP_XREFRESH) return fail(AXSPMV_E_INVALID, "unknown operation %u", op);
     return AXSPMV_OK;
 }
 

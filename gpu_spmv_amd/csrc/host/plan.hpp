@@ -11,7 +11,7 @@
 #include "common.hpp"
 
 struct axspmv_plan {
-    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC, RCT };
+    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC, RCT, AXT_UNC_H1_AX };
     Kind kind = NONE;
     int device = 0;
     axspmv_opts o{};
@@ -42,6 +42,10 @@ struct axspmv_plan {
     // AXT UNC th == 1: blocks of <= 64 units for the fused adjoint pass
     axspmv::launch::Block *d_fblocks = nullptr;
     uint32_t nfblocks = 0, nunits = 0;
+    // AX layout (AXSPMV_PLAN_AX_LAYOUT): interleaved ax array and the column
+    // of every value slot (0xffffffff for padding)
+    double *d_ax = nullptr;
+    uint32_t *d_axmap = nullptr;
     // A^T in the same format (AXSPMV_PLAN_TRANSPOSE)
     axspmv_plan *T = nullptr;
 

@@ -240,6 +240,14 @@ typedef struct axspmv_opts {
 #define AXSPMV_PLAN_RCT_SPLIT 2u /* RCT: keep column (4 B) and tile row (2 B) of
                                     each entry apart even when both fit one
                                     32-bit key */
+#define AXSPMV_PLAN_AX_LAYOUT 4u /* AXT UNC th == 1, FP64 values: keep the
+                                    reference's interleaved ax array on the
+                                    device (tiles of thw values then thw
+                                    x-values, getArraysAxSecAXT_UNC_H1,
+                                    cudaSpmv.cu:1239-1271).  Every SpMV first
+                                    refreshes the x-halves from x
+                                    (axspmv_x_refresh), then streams ax like
+                                    gaxtuh1hw*w (cudaSpmv.cu:1570-1850). */
 
 /* Fill `o` with the defaults of a format (every field defined). */
 int axspmv_opts_default(uint32_t format, axspmv_opts *o);
@@ -317,7 +325,10 @@ enum axspmv_op {
     AXSPMV_OP_VBAR = 2,          /* vbar = ybar[row] * x[col]        */
     AXSPMV_OP_FWD_ADJ = 3,       /* axspmv_fwd_adj, fused            */
     AXSPMV_OP_FWD_ADJ_SPLIT = 4, /* axspmv_fwd_adj, four kernels     */
-    AXSPMV_OP_SPMM = 5           /* Y = A X, k right-hand sides      */
+    AXSPMV_OP_SPMM = 5,          /* Y = A X, k right-hand sides      */
+    AXSPMV_OP_XREFRESH = 6       /* AX-layout plan: only the x-half refresh
+                                    of its ax array from x (the overhead
+                                    time of the layout)               */
 };
 typedef struct axspmv_operands {
     const double *x, *b, *ybar, *X;
@@ -327,6 +338,47 @@ typedef struct axspmv_operands {
 int axspmv_time_op(axspmv_plan *p, uint32_t op, const axspmv_operands *ops, uint32_t iters,
                    uint32_t warmup, uint32_t mode, uint32_t graph_batch, void *stream,
                    axspmv_timing *out);
+
+/* ------------------------------------------------- builders on the device */
+/* A CSR matrix whose arrays are in device memory (caller-owned); it must
+ * satisfy the rules of axspmv_csr_check (the device builders trust the row
+ * pointer; a column >= ncols only leaves its x value 0). */
+typedef struct axspmv_dcsr {
+    uint32_t nrows, ncols, nnz, pad_;
+    const uint32_t *row;  /* nrows + 1 */
+    const uint32_t *col;  /* nnz */
+    const double *val;    /* nnz */
+} axspmv_dcsr;
+
+/* A format built on the device.  Arrays are device memory owned by the
+ * library (axspmv_dfmt_free); lengths and parameters as in axspmv_axc /
+ * axspmv_axt.  AXC: con = brp (nrows + 1), plane = mapx (lenMAPX = lenAX/2 +
+ * 8), sec unused.  AXT: con (lenCON + 1), sec (lenSEC), plane = column plane
+ * (lenAX / 2).  ms = device time of the conversion (count, scan and fill
+ * kernels; allocation excluded), the `ct` of the reference's formats table
+ * (cudaSpmv.cu:2420-2436) measured on the GPU. */
+typedef struct axspmv_dfmt {
+    char name[48];
+    uint32_t format, mode, hw, th, bs, log;
+    uint32_t nrows, nnz, tileN, lenAX, lenSEC, lenCON, lenMAPX, pad_;
+    double *ax;       /* lenAX: values, and x values when x was given (else 0) */
+    uint32_t *sec;
+    uint32_t *con;
+    uint32_t *plane;
+    double ms;
+} axspmv_dfmt;
+
+/* Device counterparts of axspmv_build_axt / axspmv_build_axc (the
+ * reference's getFormatDataAXT / getFormatDataAXC, cudaSpmv.cu:1391-1566 and
+ * 1004-1062, run on the host 250 times in the reference): the same arrays,
+ * bit for bit, built by kernels from a device CSR.  d_x may be NULL (the
+ * x-halves stay 0; axspmv_x_refresh fills them later).  Same parameter
+ * rules and errors as the host builders. */
+int axspmv_dev_build_axt(const axspmv_dcsr *a, const double *d_x, uint32_t bs, uint32_t thw, uint32_t th,
+                         uint32_t mode, void *stream, axspmv_dfmt **out);
+int axspmv_dev_build_axc(const axspmv_dcsr *a, const double *d_x, uint32_t hbs, void *stream,
+                         axspmv_dfmt **out);
+void axspmv_dfmt_free(axspmv_dfmt *f);
 
 /* ------------------------------------------------------------ batches */
 /* Many matrices of different sizes, SpMV in one persistent launch.  Each
@@ -392,6 +444,20 @@ int axspmv_device_sync(void);
 int axspmv_device_count(int *count);
 /* Plain device copy dst[i] = src[i] (a bandwidth reference kernel). */
 int axspmv_copy_f64(const double *d_src, double *d_dst, size_t n, void *stream);
+
+/* The x-refresh step of the interleaved layouts: for q < nslots,
+ *   d_ax[(q / hw) * 2hw + hw + q % hw] = d_map[q] < nx ? d_x[d_map[q]] : 0.0
+ * i.e. fills the x-half of every (hw values, hw x-values) group of an AXC
+ * (hw = hbs) or AXT (hw = thw; th > 1 levels and COM tiles are such groups
+ * too) ax array from the column of each value slot.  d_map is the AXC mapx
+ * (get_mapxAXC, cudaSpmv.cu:983-1000) or an AXT column plane; their padding
+ * slots hold column 0 and so get x[0], as the reference's own gather of the
+ * x-halves does (the AVX-512 x_ax, gpuSpmv_header.h:1750-1779), while a map
+ * entry >= nx (e.g. 0xffffffff) marks padding that gets 0.0, which makes
+ * the result equal the builders' ax.  hw is a power of two <= 1024 and
+ * nslots a multiple of hw (d_ax holds 2 * nslots doubles). */
+int axspmv_x_refresh(const uint32_t *d_map, uint64_t nslots, uint32_t hw, uint32_t nx, const double *d_x,
+                     double *d_ax, void *stream);
 
 #ifdef __cplusplus
 }
