@@ -4,15 +4,17 @@
 // L2 request for x, and once x outgrows an XCD's 4 MB L2 the requests go to
 // the Infinity Cache.  RCT attacks both:
 //
-//  * the matrix is cut into tiles of R rows x C columns (S = ncols / C column
-//    slices); a workgroup owns one tile and accumulates its R rows in LDS;
+//  * the matrix is cut into tiles of (at most) R rows x C columns (S = ncols /
+//    C column slices); row blocks hold about equal numbers of nonzeros (the
+//    host cuts them), and a workgroup owns one tile and accumulates its rows
+//    in LDS;
 //  * entries of a tile are stored sorted by column, so neighbouring lanes
 //    gather neighbouring x entries -- one cache line serves several lanes;
 //  * workgroups are dispatched round-robin to the 8 XCDs (workgroup w runs on
 //    XCD w % 8), and tile w belongs to column slice (w % 8) % S: each XCD only
 //    ever reads its own slice of x, which stays resident in its L2.
 //
-// Each tile writes its R partial sums (zero for rows it does not touch) to
+// Each tile writes its rows' partial sums (zero for rows it does not touch) to
 // the partial vector of its slice; a second kernel adds the S partials of
 // every row in slice order.  The LDS adds (ds_add_f64) make the order of the
 // additions inside a row nondeterministic; the result stays within the FP64
@@ -59,8 +61,9 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     const uint32_t per = 8 / a.S;  // row-block groups per slice
     const uint32_t s = xcd % a.S, rb = i * per + xcd / a.S;
     if (rb >= a.nrb) return;  // the whole workgroup leaves together
-    const uint32_t R = a.R, tid = threadIdx.x, nt = blockDim.x;
-    for (uint32_t k = tid; k < R; k += nt) yl[k] = 0.0;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t row0 = a.rb_start[rb], rows = a.rb_start[rb + 1] - row0;  // rows <= a.R
+    for (uint32_t k = tid; k < rows; k += nt) yl[k] = 0.0;
     __syncthreads();
     const uint32_t t = rb * a.S + s;  // tile index in storage order
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
@@ -119,8 +122,6 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
         }
     }
     __syncthreads();
-    const uint32_t row0 = rb * R;
-    const uint32_t rows = min(R, a.nrows - row0);
     double *out = a.part + size_t(s) * a.nrows + row0;
     for (uint32_t k = tid; k < rows; k += nt) out[k] = yl[k];
 }

@@ -63,17 +63,18 @@ struct SpmmArgs {
 // kt (k-tile) in {8, 16, 32, 64}
 hipError_t spmm(const SpmmArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s);
 
-// RCT tiles: tile t = rb * S + s holds entries tile_ptr[t] .. tile_ptr[t+1]-1
-// (column col, row rb*R + lrow), sorted by column; part holds S vectors of
-// nrows partial sums.
+// RCT tiles: row block rb = rows rb_start[rb] .. rb_start[rb+1]-1 (at most R
+// rows); tile t = rb * S + s holds entries tile_ptr[t] .. tile_ptr[t+1]-1
+// (column col, row rb_start[rb] + lrow), sorted by column; part holds S
+// vectors of nrows partial sums.
 struct RctArgs {
     const void *val;
     const uint32_t *col;    // unpacked: column of each entry
     const uint16_t *lrow;   // unpacked: row within the tile
     const uint32_t *key;    // packed: (column - slice start) << rbits | row
     const uint32_t *tile_ptr;
+    const uint32_t *rb_start;  // nrb + 1 row-block starts
     double *part;
-    uint32_t *arrived;      // per row block: tiles done (0 between calls)
     uint32_t nrows, R, S, nrb, vtype, block, rbits, C;
 };
 hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s);

@@ -436,15 +436,34 @@ int plan_axc(Plan &p, const axspmv_csr *a, hipStream_t s) {
     return setup_rowwise(p, a, d_first, vp, mapx, s);
 }
 
-// RCT: tiles of R rows x C columns, entries of a tile sorted by (column, row).
+// Row-block starts for RCT: nrb blocks of about nnz / nrb nonzeros each,
+// every block 1..rmax rows (nrb * rmax >= nrows).
+std::vector<uint32_t> rct_row_blocks(const axspmv_csr *a, uint32_t nrb, uint32_t rmax) {
+    const uint32_t n = a->nrows;
+    std::vector<uint32_t> rbs(size_t(nrb) + 1, 0);
+    for (uint32_t k = 1; k < nrb; ++k) {
+        const uint64_t target = uint64_t(a->nnz) * k / nrb;
+        uint64_t r = uint64_t(std::lower_bound(a->row, a->row + n + 1, target) - a->row);
+        const uint64_t lo = std::max<uint64_t>(uint64_t(rbs[k - 1]) + 1,
+                                               n > uint64_t(nrb - k) * rmax ? n - uint64_t(nrb - k) * rmax : 0);
+        const uint64_t hi = std::min<uint64_t>(uint64_t(rbs[k - 1]) + rmax, n - (nrb - k));
+        rbs[k] = uint32_t(std::min(std::max(r, lo), hi));
+    }
+    rbs[nrb] = n;
+    return rbs;
+}
+
+// RCT: row blocks of at most tile_rows rows and about equal nonzeros, column
+// slices, entries of a tile sorted by (column, row).
 int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     hipDeviceProp_t prop;
     size_t lds_max = 65536;
     if (hipGetDeviceProperties(&prop, p.device) == hipSuccess && prop.sharedMemPerBlock > 0)
         lds_max = prop.sharedMemPerBlock;
-    uint32_t R = p.o.tile_rows ? p.o.tile_rows : uint32_t(std::min<size_t>(16384, lds_max / 8));
-    if (R == 0 || R > 65536 || size_t(R) * 8 > lds_max)
-        return fail(AXSPMV_E_UNSUPPORTED, "RCT: %u rows per tile do not fit %zu bytes of LDS", R, lds_max);
+    const uint32_t lds_rows = uint32_t(std::min<size_t>(65536, lds_max / 8));
+    const uint32_t Rt = p.o.tile_rows ? p.o.tile_rows : std::min<uint32_t>(16384, lds_rows);
+    if (Rt == 0 || Rt > lds_rows)
+        return fail(AXSPMV_E_UNSUPPORTED, "RCT: %u rows per tile do not fit %zu bytes of LDS", Rt, lds_max);
     uint32_t S = p.o.slices;
     if (S == 0) {
         S = 1;
@@ -452,16 +471,19 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     }
     if (S != 1 && S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_INVALID, "RCT: slices must be 1, 2, 4 or 8");
     const uint32_t n = a->nrows;
-    const uint32_t nrb = uint32_t(ceil_div(n, R));
+    const uint32_t nrb = uint32_t(ceil_div(n, Rt));
+    // Blocks keep at most Rt rows (so the packed key keeps its bits); the
+    // balancing uses whatever slack ceil(n / Rt) * Rt - n leaves.
+    const std::vector<uint32_t> rbs = rct_row_blocks(a, nrb, Rt);
+    uint32_t R = 1;  // rows of the largest block: the LDS accumulator size
+    for (uint32_t k = 0; k < nrb; ++k) R = std::max(R, rbs[k + 1] - rbs[k]);
     const uint32_t C = uint32_t(std::max<uint64_t>(1, ceil_div(a->ncols, S)));
     const uint64_t ntile = uint64_t(nrb) * S;
     // count entries per tile
     std::vector<uint64_t> cnt(ntile + 1, 0);
 #pragma omp parallel for schedule(dynamic, 1)
-    for (uint32_t rb = 0; rb < nrb; ++rb) {
-        const uint32_t r1 = std::min<uint64_t>(uint64_t(rb + 1) * R, n);
-        for (uint32_t j = a->row[rb * R]; j < a->row[r1]; ++j) cnt[uint64_t(rb) * S + a->col[j] / C + 1]++;
-    }
+    for (uint32_t rb = 0; rb < nrb; ++rb)
+        for (uint32_t j = a->row[rbs[rb]]; j < a->row[rbs[rb + 1]]; ++j) cnt[uint64_t(rb) * S + a->col[j] / C + 1]++;
     for (uint64_t t = 0; t < ntile; ++t) cnt[t + 1] += cnt[t];
     const uint64_t nnz = a->nnz;
     std::vector<uint32_t> col(nnz ? nnz : 1);
@@ -472,7 +494,7 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         std::vector<std::pair<uint64_t, uint32_t>> key;  // (col << 16 | lrow, CSR position)
 #pragma omp for schedule(dynamic, 1)
         for (uint32_t rb = 0; rb < nrb; ++rb) {
-            const uint32_t r0 = rb * R, r1 = std::min<uint64_t>(uint64_t(rb + 1) * R, n);
+            const uint32_t r0 = rbs[rb], r1 = rbs[rb + 1];
             key.clear();
             for (uint32_t r = r0; r < r1; ++r)
                 for (uint32_t j = a->row[r]; j < a->row[r + 1]; ++j)
@@ -504,10 +526,10 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     const bool packed = rbits + cbits <= 32 && !(p.o.flags & AXSPMV_PLAN_RCT_SPLIT);
     void *dv = p.dalloc(vs * nnz);
     auto *dt = static_cast<uint32_t *>(p.dalloc(4 * tptr.size()));
+    auto *drb = static_cast<uint32_t *>(p.dalloc(4 * rbs.size()));
     auto *part = static_cast<double *>(p.dalloc(S > 1 ? 8ull * S * n : 8));
-    auto *arrived = static_cast<uint32_t *>(p.dalloc(4ull * nrb));
-    if (!dv || !dt || !part || !arrived) return AXSPMV_E_NOMEM;
-    if (hipMemsetAsync(arrived, 0, 4ull * nrb, s) != hipSuccess) return fail(AXSPMV_E_HIP, "RCT: counter reset");
+    if (!dv || !dt || !drb || !part) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(drb, rbs.data(), 4 * rbs.size(), s));
     if (p.o.vtype == AXSPMV_F32) {
         std::vector<float> v(val.begin(), val.begin() + nnz);
         AX_TRY(p.upload(dv, v.data(), 4 * nnz, s));
@@ -535,15 +557,17 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         AX_TRY(p.upload(dl, lrow.data(), 2 * nnz, s));
     }
     uint32_t block = p.o.block;
-    if (block == 0 || block > 1024 || block % 64) block = 1024;
-    p.rct = launch::RctArgs{dv, dc, dl, dk, dt, part, arrived, n, R, S, nrb, p.o.vtype, block, rbits, C};
+    // 512 threads (8 waves, one workgroup per CU with R = 16384) measured
+    // fastest on the 2^20-row power-law matrix; 1024 is 2-3 % slower
+    if (block == 0 || block > 1024 || block % 64) block = 512;
+    p.rct = launch::RctArgs{dv, dc, dl, dk, dt, drb, part, n, R, S, nrb, p.o.vtype, block, rbits, C};
     p.slots = nnz;
     // matrix stream + per-slice partial write/read when S > 1
     p.stream_bytes = (vs + (packed ? 4 : 6)) * nnz + 4 * tptr.size() + (S > 1 ? 16ull * S * n : 0);
     p.launches = S > 1 ? 2 : 1;
     p.nblocks = uint32_t(ntile);
     p.kind = Plan::RCT;
-    std::snprintf(p.name, sizeof p.name, "rct_r%u_s%u%s", R, S, packed ? "" : "_split");
+    std::snprintf(p.name, sizeof p.name, "rct_r%u_s%u%s", Rt, S, packed ? "" : "_split");
     return AXSPMV_OK;
 }
 

@@ -232,7 +232,9 @@ typedef struct axspmv_opts {
     uint32_t flags;   /* AXSPMV_PLAN_* */
     uint32_t ktile;   /* SpMM k-tile: right-hand sides per lane group, 8/16/32/64
                          (0 = smallest of those >= min(k, 64)) */
-    uint32_t tile_rows; /* RCT: rows per tile (0 = as many as the LDS holds) */
+    uint32_t tile_rows; /* RCT: at most this many rows per row block (0 =
+                           min(16384, LDS / 8)); blocks are cut to about
+                           equal nonzeros */
     uint32_t slices;    /* RCT: column slices 1, 2, 4 or 8 (0 = x slice <= 2 MB) */
 } axspmv_opts;
 
