@@ -104,7 +104,7 @@ class BatchInfo(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("mean_ms", f64), ("median_ms", f64), ("min_ms", f64), ("max_ms", f64),
-                ("iters", u32), ("mode", u32)]
+                ("iters", u32), ("mode", u32), ("main_ms", f64)]
 
 
 # enum axspmv_op

@@ -250,7 +250,7 @@ def opts(fmt: int, **kw) -> L.Opts:
     return o
 
 
-TIME_EVENTS, TIME_GRAPH, TIME_REGION = 0, 1, 2
+TIME_EVENTS, TIME_GRAPH, TIME_REGION, TIME_KERNEL = 0, 1, 2, 3
 
 
 @dataclass
@@ -261,6 +261,7 @@ class TimingResult:
     max_ms: float
     iters: int
     mode: int
+    main_ms: float = 0.0
 
 
 class Plan:
@@ -291,7 +292,7 @@ class Plan:
         t = L.Timing()
         L.call("axspmv_time_spmv", self.ptr, _dptr(dx), _dptr(dy), iters, warmup, mode, batch,
                stream, C.byref(t))
-        return TimingResult(t.mean_ms, t.median_ms, t.min_ms, t.max_ms, t.iters, t.mode)
+        return TimingResult(t.mean_ms, t.median_ms, t.min_ms, t.max_ms, t.iters, t.mode, t.main_ms)
 
     def adjoint(self, dybar, dxbar, stream=None) -> None:
         """x_bar = A^T y_bar (plan built with flags=PLAN_TRANSPOSE)."""
@@ -318,7 +319,7 @@ class Plan:
             setattr(o, name, v if name == "k" else _dptr(v))
         t = L.Timing()
         L.call("axspmv_time_op", self.ptr, op, C.byref(o), iters, warmup, mode, batch, stream, C.byref(t))
-        return TimingResult(t.mean_ms, t.median_ms, t.min_ms, t.max_ms, t.iters, t.mode)
+        return TimingResult(t.mean_ms, t.median_ms, t.min_ms, t.max_ms, t.iters, t.mode, t.main_ms)
 
     def close(self) -> None:
         if self.ptr:
@@ -426,7 +427,7 @@ class Batch:
     def time(self, dx, dy, iters=50, warmup=5, mode=TIME_REGION, batch=1, stream=None) -> TimingResult:
         t = L.Timing()
         L.call("axspmv_batch_time", self.ptr, _dptr(dx), _dptr(dy), iters, warmup, mode, batch, stream, C.byref(t))
-        return TimingResult(t.mean_ms, t.median_ms, t.min_ms, t.max_ms, t.iters, t.mode)
+        return TimingResult(t.mean_ms, t.median_ms, t.min_ms, t.max_ms, t.iters, t.mode, t.main_ms)
 
     def close(self) -> None:
         if self.ptr:

@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(256) rct_merge(const double *__restrict__ part
 namespace axspmv {
 namespace launch {
 
-hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s) {
+hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid) {
     if (!a.nrows) return hipSuccess;
     const uint32_t per = 8 / a.S;
     const uint32_t groups = (a.nrb + per - 1) / per;
@@ -161,6 +161,10 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s) {
             rct_kernel<double, true><<<grid, a.block, lds, s>>>(b, x);
         else
             rct_kernel<double, false><<<grid, a.block, lds, s>>>(b, x);
+    }
+    if (mid) {
+        const hipError_t e = hipEventRecord(mid, s);
+        if (e != hipSuccess) return e;
     }
     if (a.S > 1) rct_merge<<<(a.nrows + 255) / 256, 256, 0, s>>>(a.part, a.S, a.nrows, y);
     return hipGetLastError();

@@ -77,7 +77,8 @@ struct RctArgs {
     double *part;
     uint32_t nrows, R, S, nrb, vtype, block, rbits, C;
 };
-hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s);
+// mid (optional): recorded on s between the tile kernel and the merge
+hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid = nullptr);
 
 // Batched persistent SpMV over AXT UNC matrices.
 struct BatchMat {

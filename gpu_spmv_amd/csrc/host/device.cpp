@@ -98,6 +98,10 @@ void summarize(std::vector<double> &ms, double scale, axspmv_timing *out) {
 int axspmv::time_launches(const std::function<hipError_t(hipStream_t)> &fn, uint32_t iters, uint32_t warmup,
                           uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
     if (!out || iters == 0 || mode > AXSPMV_TIME_REGION) return fail(AXSPMV_E_INVALID, "timing: bad argument");
+    struct MainFromMean {
+        axspmv_timing *o;
+        ~MainFromMean() { o->main_ms = o->mean_ms; }
+    } fill_main{out};
     std::memset(out, 0, sizeof *out);
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipStream_t own = nullptr;
@@ -167,11 +171,75 @@ int axspmv::time_launches(const std::function<hipError_t(hipStream_t)> &fn, uint
     return AXSPMV_OK;
 }
 
+namespace {
+
+// AXSPMV_TIME_KERNEL: back-to-back SpMVs; event k[i] before SpMV i, m[i]
+// after its first kernel (if the path marks it), one more after the last.
+int time_kernel(axspmv_plan *p, const axspmv_operands &o, uint32_t iters, uint32_t warmup, hipStream_t s,
+                axspmv_timing *out) {
+    if (iters > 4096) return fail(AXSPMV_E_INVALID, "timing: AXSPMV_TIME_KERNEL takes at most 4096 iterations");
+    std::memset(out, 0, sizeof *out);
+    out->mode = AXSPMV_TIME_KERNEL;
+    out->iters = iters;
+    std::vector<hipEvent_t> k(iters + 1, nullptr), m(iters, nullptr);
+    struct Guard {
+        std::vector<hipEvent_t> &a, &b;
+        axspmv_plan *p;
+        ~Guard() {
+            p->mid = nullptr;
+            for (hipEvent_t e : a)
+                if (e) (void)hipEventDestroy(e);
+            for (hipEvent_t e : b)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } guard{k, m, p};
+    // Markers inside the region skip the system-scope fence (an L2 write-back
+    // and invalidate that would slow the next SpMV); the closing event keeps
+    // it, so its completion covers all the work.
+    hipError_t e = hipSuccess;
+    for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) {
+        e = hipEventCreateWithFlags(&k[i], hipEventDisableSystemFence);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&m[i], hipEventDisableSystemFence);
+    }
+    if (e == hipSuccess) e = hipEventCreate(&k[iters]);
+    if (e != hipSuccess) return hip_fail(e, "timing: event creation");
+    const bool marks = p->kind == Plan::RCT && p->rct.S > 1;
+    for (uint32_t i = 0; i < warmup && e == hipSuccess; ++i) e = plan_launch(p, o.x, o.y, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) {
+        e = hipEventRecord(k[i], s);
+        p->mid = marks ? m[i] : nullptr;
+        if (e == hipSuccess) e = plan_launch(p, o.x, o.y, s);
+    }
+    p->mid = nullptr;
+    if (e == hipSuccess) e = hipEventRecord(k[iters], s);
+    if (e == hipSuccess) e = hipEventSynchronize(k[iters]);
+    float tot = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&tot, k[0], k[iters]);
+    double main_sum = 0.0;
+    for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) {
+        float t = 0.f;
+        e = hipEventElapsedTime(&t, k[i], marks ? m[i] : k[i + 1]);
+        main_sum += t;
+    }
+    if (e != hipSuccess) return hip_fail(e, "timing (kernel)");
+    out->mean_ms = out->median_ms = out->min_ms = out->max_ms = double(tot) / iters;
+    out->main_ms = main_sum / iters;
+    return AXSPMV_OK;
+}
+
+}  // namespace
+
 extern "C" int axspmv_time_op(axspmv_plan *p, uint32_t op, const axspmv_operands *ops, uint32_t iters,
                               uint32_t warmup, uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
     if (!p || !ops) return fail(AXSPMV_E_INVALID, "time_op: NULL argument");
     AX_TRY(plan_op_check(p, op, *ops));
     const axspmv_operands o = *ops;
+    if (mode == AXSPMV_TIME_KERNEL) {
+        if (op != AXSPMV_OP_SPMV) return fail(AXSPMV_E_INVALID, "timing: AXSPMV_TIME_KERNEL times SpMV only");
+        if (!out || iters == 0) return fail(AXSPMV_E_INVALID, "timing: bad argument");
+        return time_kernel(p, o, iters, warmup, static_cast<hipStream_t>(stream), out);
+    }
     return time_launches([p, op, o](hipStream_t st) { return plan_op(p, op, o, st); }, iters, warmup, mode,
                          graph_batch, stream, out);
 }

@@ -461,9 +461,6 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     if (hipGetDeviceProperties(&prop, p.device) == hipSuccess && prop.sharedMemPerBlock > 0)
         lds_max = prop.sharedMemPerBlock;
     const uint32_t lds_rows = uint32_t(std::min<size_t>(65536, lds_max / 8));
-    const uint32_t Rt = p.o.tile_rows ? p.o.tile_rows : std::min<uint32_t>(16384, lds_rows);
-    if (Rt == 0 || Rt > lds_rows)
-        return fail(AXSPMV_E_UNSUPPORTED, "RCT: %u rows per tile do not fit %zu bytes of LDS", Rt, lds_max);
     uint32_t S = p.o.slices;
     if (S == 0) {
         S = 1;
@@ -471,6 +468,15 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     }
     if (S != 1 && S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_INVALID, "RCT: slices must be 1, 2, 4 or 8");
     const uint32_t n = a->nrows;
+    // Default tile height: as tall as the LDS allows (fewer row blocks, fewer
+    // passes over x), but short enough that the tiles cover the 256 CUs.
+    uint32_t Rt = p.o.tile_rows;
+    if (Rt == 0) {
+        const uint64_t fill = ceil_div(ceil_div(uint64_t(n) * S, 256), 64) * 64;
+        Rt = uint32_t(std::min<uint64_t>(std::min<uint32_t>(16384, lds_rows), std::max<uint64_t>(64, fill)));
+    }
+    if (Rt == 0 || Rt > lds_rows)
+        return fail(AXSPMV_E_UNSUPPORTED, "RCT: %u rows per tile do not fit %zu bytes of LDS", Rt, lds_max);
     const uint32_t nrb = uint32_t(ceil_div(n, Rt));
     // Blocks keep at most Rt rows (so the packed key keeps its bits); the
     // balancing uses whatever slack ceil(n / Rt) * Rt - n leaves.
@@ -679,7 +685,7 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
         case Plan::AXT_COM: return launch::axt_com(p->com, x, y, s);
         case Plan::K1: return launch::k1(p->k1, x, y, s);
         case Plan::AXC: return launch::axc(p->axc, x, y, s);
-        case Plan::RCT: return launch::rct(p->rct, x, y, s);
+        case Plan::RCT: return launch::rct(p->rct, x, y, s, p->mid);
         case Plan::AXT_UNC_H1_AX: {
             const hipError_t e = launch::x_refresh(p->d_axmap, p->slots, ilog2(p->thw), p->ncols, x, p->d_ax, s);
             if (e != hipSuccess) return e;

@@ -46,6 +46,8 @@ struct axspmv_plan {
     // of every value slot (0xffffffff for padding)
     double *d_ax = nullptr;
     uint32_t *d_axmap = nullptr;
+    // AXSPMV_TIME_KERNEL: event the path records after its first kernel (RCT)
+    hipEvent_t mid = nullptr;
     // A^T in the same format (AXSPMV_PLAN_TRANSPOSE)
     axspmv_plan *T = nullptr;
 

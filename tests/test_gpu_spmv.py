@@ -160,3 +160,12 @@ def test_timing_modes():
         t = p.time(dx, dy, iters=20, warmup=2, mode=mode, batch=10)
         assert t.mean_ms > 0 and t.min_ms <= t.median_ms <= t.max_ms and t.mode == mode
     assert rel_err(orc.ncsr(m, x), dy.download(np.float64, m.nrows)) <= TOL
+    # the main-kernel split: RCT with slices marks its tile kernel
+    t = p.time(dx, dy, iters=20, warmup=2, mode=ax.core.TIME_KERNEL)
+    assert t.mode == ax.core.TIME_KERNEL and t.main_ms == pytest.approx(t.mean_ms)
+    r = ax.Plan(m, ax.FMT_RCT, tile_rows=1024, slices=4)
+    t = r.time(dx, dy, iters=20, warmup=2, mode=ax.core.TIME_KERNEL)
+    assert 0 < t.main_ms < t.mean_ms
+    assert rel_err(orc.ncsr(m, x), dy.download(np.float64, m.nrows)) <= TOL
+    with pytest.raises(ax.AxspmvError):
+        r.time(dx, dy, iters=5000, warmup=0, mode=ax.core.TIME_KERNEL)
