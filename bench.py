@@ -57,6 +57,8 @@ OTHERS = [
     ("rct_r8192_s8", ax.FMT_RCT, dict(tile_rows=8192, slices=8)),
     ("rct_r16384_s2", ax.FMT_RCT, dict(tile_rows=16384, slices=2)),
     ("axt_unc_h1_thw4_ax", ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_AX_LAYOUT)),
+    ("rct_auto_f32val", ax.FMT_RCT, dict(vtype=ax.F32)),
+    ("axt_unc_h1_thw4_f32val", ax.FMT_AXT, dict(thw=4, vtype=ax.F32)),
 ]
 
 

@@ -14,11 +14,13 @@
 //    XCD w % 8), and tile w belongs to column slice (w % 8) % S: each XCD only
 //    ever reads its own slice of x, which stays resident in its L2.
 //
-// Each tile writes its rows' partial sums (zero for rows it does not touch) to
-// the partial vector of its slice; a second kernel adds the S partials of
-// every row in slice order.  The LDS adds (ds_add_f64) make the order of the
-// additions inside a row nondeterministic; the result stays within the FP64
-// tolerance of the left-to-right CPU sum.
+// With one slice a tile's sums are its rows' totals and are stored to y.
+// With S slices y is first cleared by a fill kernel, and every tile adds its
+// rows' sums to y with hardware FP64 atomics (global_atomic_add_f64, S adds
+// per row): no partial vectors in HBM and no merge pass.  The LDS adds
+// (ds_add_f64) and the slice adds make the order of the additions inside a
+// row nondeterministic; the result stays within the FP64 tolerance of the
+// left-to-right CPU sum.
 #include "launch.hpp"
 #include "wave64.hpp"
 
@@ -54,7 +56,7 @@ struct RctEntries {
 };
 
 template <class VT, bool PACKED>
-__global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__restrict__ x) {
+__global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__restrict__ x, double *__restrict__ y) {
     extern __shared__ double yl[];
     const uint32_t w = blockIdx.x;
     const uint32_t xcd = w % 8, i = w / 8;
@@ -122,18 +124,26 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
         }
     }
     __syncthreads();
-    double *out = a.part + size_t(s) * a.nrows + row0;
-    for (uint32_t k = tid; k < rows; k += nt) out[k] = yl[k];
+    double *out = y + row0;
+    if (a.S == 1) {
+        for (uint32_t k = tid; k < rows; k += nt) out[k] = yl[k];
+    } else {
+        for (uint32_t k = tid; k < rows; k += nt) unsafeAtomicAdd(out + k, yl[k]);
+    }
 }
 
-// y[r] = sum over slices s of part[s][r], in slice order.
-__global__ void __launch_bounds__(256) rct_merge(const double *__restrict__ part, uint32_t S, uint32_t n,
-                                                 double *__restrict__ y) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    double acc = part[r];
-    for (uint32_t s = 1; s < S; ++s) acc += part[size_t(s) * n + r];
-    y[r] = acc;
+// y[0 .. n) = 0 with 16-byte stores (y 16-byte aligned; VEC = false: 8-byte)
+template <bool VEC>
+__global__ void __launch_bounds__(256) rct_clear(double *__restrict__ y, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if constexpr (VEC) {
+        if (2 * i + 1 < n)
+            reinterpret_cast<double2 *>(y)[i] = double2{0.0, 0.0};
+        else if (2 * i < n)
+            y[2 * i] = 0.0;
+    } else {
+        if (i < n) y[i] = 0.0;
+    }
 }
 
 }  // namespace
@@ -147,26 +157,28 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
     const uint32_t groups = (a.nrb + per - 1) / per;
     const unsigned grid = groups * 8;
     const size_t lds = size_t(a.R) * sizeof(double);
-    // With S == 1 the tile's sums are the rows' totals: write y directly.
-    RctArgs b = a;
-    if (a.S == 1) b.part = y;
+    if (a.S > 1) {
+        if (reinterpret_cast<uintptr_t>(y) % 16 == 0)
+            rct_clear<true><<<(a.nrows / 2 + 256) / 256, 256, 0, s>>>(y, a.nrows);
+        else
+            rct_clear<false><<<(a.nrows + 255) / 256, 256, 0, s>>>(y, a.nrows);
+        if (mid) {
+            const hipError_t e = hipEventRecord(mid, s);
+            if (e != hipSuccess) return e;
+        }
+    }
     const bool packed = a.key != nullptr;
     if (a.vtype == 1) {
         if (packed)
-            rct_kernel<float, true><<<grid, a.block, lds, s>>>(b, x);
+            rct_kernel<float, true><<<grid, a.block, lds, s>>>(a, x, y);
         else
-            rct_kernel<float, false><<<grid, a.block, lds, s>>>(b, x);
+            rct_kernel<float, false><<<grid, a.block, lds, s>>>(a, x, y);
     } else {
         if (packed)
-            rct_kernel<double, true><<<grid, a.block, lds, s>>>(b, x);
+            rct_kernel<double, true><<<grid, a.block, lds, s>>>(a, x, y);
         else
-            rct_kernel<double, false><<<grid, a.block, lds, s>>>(b, x);
+            rct_kernel<double, false><<<grid, a.block, lds, s>>>(a, x, y);
     }
-    if (mid) {
-        const hipError_t e = hipEventRecord(mid, s);
-        if (e != hipSuccess) return e;
-    }
-    if (a.S > 1) rct_merge<<<(a.nrows + 255) / 256, 256, 0, s>>>(a.part, a.S, a.nrows, y);
     return hipGetLastError();
 }
 

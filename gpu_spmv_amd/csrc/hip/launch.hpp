@@ -65,8 +65,7 @@ hipError_t spmm(const SpmmArgs &a, uint32_t k, uint32_t kt, const double *X, dou
 
 // RCT tiles: row block rb = rows rb_start[rb] .. rb_start[rb+1]-1 (at most R
 // rows); tile t = rb * S + s holds entries tile_ptr[t] .. tile_ptr[t+1]-1
-// (column col, row rb_start[rb] + lrow), sorted by column; part holds S
-// vectors of nrows partial sums.
+// (column col, row rb_start[rb] + lrow), sorted by column.
 struct RctArgs {
     const void *val;
     const uint32_t *col;    // unpacked: column of each entry
@@ -74,10 +73,10 @@ struct RctArgs {
     const uint32_t *key;    // packed: (column - slice start) << rbits | row
     const uint32_t *tile_ptr;
     const uint32_t *rb_start;  // nrb + 1 row-block starts
-    double *part;
     uint32_t nrows, R, S, nrb, vtype, block, rbits, C;
 };
-// mid (optional): recorded on s between the tile kernel and the merge
+// S > 1: a fill kernel clears y, then the tile kernel adds into it; mid
+// (optional) is recorded on s between the two
 hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid = nullptr);
 
 // Batched persistent SpMV over AXT UNC matrices.

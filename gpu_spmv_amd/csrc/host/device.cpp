@@ -174,7 +174,8 @@ int axspmv::time_launches(const std::function<hipError_t(hipStream_t)> &fn, uint
 namespace {
 
 // AXSPMV_TIME_KERNEL: back-to-back SpMVs; event k[i] before SpMV i, m[i]
-// after its first kernel (if the path marks it), one more after the last.
+// where its main kernel starts (if the path marks it: RCT with slices marks
+// the end of its y clear), one more after the last.  main = m[i] .. k[i+1].
 int time_kernel(axspmv_plan *p, const axspmv_operands &o, uint32_t iters, uint32_t warmup, hipStream_t s,
                 axspmv_timing *out) {
     if (iters > 4096) return fail(AXSPMV_E_INVALID, "timing: AXSPMV_TIME_KERNEL takes at most 4096 iterations");
@@ -219,7 +220,7 @@ int time_kernel(axspmv_plan *p, const axspmv_operands &o, uint32_t iters, uint32
     double main_sum = 0.0;
     for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) {
         float t = 0.f;
-        e = hipEventElapsedTime(&t, k[i], marks ? m[i] : k[i + 1]);
+        e = hipEventElapsedTime(&t, marks ? m[i] : k[i], k[i + 1]);
         main_sum += t;
     }
     if (e != hipSuccess) return hip_fail(e, "timing (kernel)");

@@ -533,8 +533,7 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     void *dv = p.dalloc(vs * nnz);
     auto *dt = static_cast<uint32_t *>(p.dalloc(4 * tptr.size()));
     auto *drb = static_cast<uint32_t *>(p.dalloc(4 * rbs.size()));
-    auto *part = static_cast<double *>(p.dalloc(S > 1 ? 8ull * S * n : 8));
-    if (!dv || !dt || !drb || !part) return AXSPMV_E_NOMEM;
+    if (!dv || !dt || !drb) return AXSPMV_E_NOMEM;
     AX_TRY(p.upload(drb, rbs.data(), 4 * rbs.size(), s));
     if (p.o.vtype == AXSPMV_F32) {
         std::vector<float> v(val.begin(), val.begin() + nnz);
@@ -566,10 +565,10 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     // 512 threads (8 waves, one workgroup per CU with R = 16384) measured
     // fastest on the 2^20-row power-law matrix; 1024 is 2-3 % slower
     if (block == 0 || block > 1024 || block % 64) block = 512;
-    p.rct = launch::RctArgs{dv, dc, dl, dk, dt, drb, part, n, R, S, nrb, p.o.vtype, block, rbits, C};
+    p.rct = launch::RctArgs{dv, dc, dl, dk, dt, drb, n, R, S, nrb, p.o.vtype, block, rbits, C};
     p.slots = nnz;
-    // matrix stream + per-slice partial write/read when S > 1
-    p.stream_bytes = (vs + (packed ? 4 : 6)) * nnz + 4 * tptr.size() + (S > 1 ? 16ull * S * n : 0);
+    // matrix stream, and with S > 1 the clear of y and S atomic adds per row
+    p.stream_bytes = (vs + (packed ? 4 : 6)) * nnz + 4 * tptr.size() + (S > 1 ? 8ull * (S + 1) * n : 0);
     p.launches = S > 1 ? 2 : 1;
     p.nblocks = uint32_t(ntile);
     p.kind = Plan::RCT;

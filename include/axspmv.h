@@ -307,19 +307,18 @@ int axspmv_spmm(axspmv_plan *p, uint32_t k, const double *d_X, double *d_Y, void
 typedef struct axspmv_timing {
     double mean_ms, median_ms, min_ms, max_ms; /* per operation */
     uint32_t iters, mode;                      /* the AXSPMV_TIME_* used */
-    double main_ms; /* AXSPMV_TIME_KERNEL: mean time of the path's first
-                       (main) kernel alone; other modes: = mean_ms */
+    double main_ms; /* AXSPMV_TIME_KERNEL: mean time of the path's main
+                       kernel alone; other modes: = mean_ms */
 } axspmv_timing;
 #define AXSPMV_TIME_EVENTS 0u /* one hipEvent pair + sync per launch (reference) */
 #define AXSPMV_TIME_GRAPH 1u  /* replays of a hipGraph of `graph_batch` SpMVs    */
 #define AXSPMV_TIME_REGION 2u /* one event pair around `iters` back-to-back SpMVs */
 #define AXSPMV_TIME_KERNEL 3u /* as REGION (SpMV only, iters <= 4096), with an
-                                 event before every SpMV and one after its first
-                                 kernel: main_ms is that kernel's mean time.
-                                 Paths of one kernel, and those that do not
-                                 mark their first kernel (only RCT does, its
-                                 second kernel being the slice merge), report
-                                 main_ms = the SpMV's own time. */
+                                 event before every SpMV and one where its main
+                                 kernel starts: main_ms is that kernel's mean
+                                 time.  Only RCT with slices has a kernel
+                                 before its main one (the clear of y); other
+                                 paths report main_ms = the SpMV's time. */
 /* Time `iters` SpMVs after `warmup` untimed ones.  AXSPMV_TIME_EVENTS is the
  * reference's loop (cudaSpmv.cu:563-581): mean and median of per-launch
  * times.  AXSPMV_TIME_GRAPH reports per-SpMV times of graph replays (for

@@ -76,6 +76,7 @@ def matrices():
     yield "stencil20p", ax.stencil27(20, permute=True, seed=2)
     yield "banded_empty", ax.banded(4000, 2, 0.3, seed=6)
     yield "long_rows", long_rows()
+    yield "random_odd", ax.random_sparse(3001, 0, 40, seed=9)
 
 
 MATS = dict(matrices())
@@ -140,6 +141,21 @@ def test_powerlaw_config2_full_size():
                     (ax.FMT_AXC, dict(hbs=4)), (ax.FMT_RCT, dict())):
         y = ax.spmv_host(ax.Plan(m, fmt, **kw), x)
         assert rel_err(ref, y) <= TOL, kw
+
+
+@pytest.mark.parametrize("kw", [dict(slices=4, tile_rows=512), dict(slices=1, tile_rows=512)])
+def test_rct_unaligned_y(kw):
+    """y at an 8-byte offset (the clear of y falls back to 8-byte stores)."""
+    m = MATS["random_odd"]
+    x = ax.uniform(13, m.ncols)
+    p = ax.Plan(m, ax.FMT_RCT, **kw)
+    dx = ax.DeviceBuffer.from_array(x)
+    dy = ax.DeviceBuffer.from_array(np.full(m.nrows + 2, np.nan))
+    p.spmv(dx, dy.ptr + 8)
+    ax._lib.call("axspmv_device_sync")
+    y = dy.download(np.float64, m.nrows + 2)
+    assert np.isnan(y[0]) and np.isnan(y[-1])
+    assert rel_err(orc.ncsr(m, x), y[1:-1]) <= TOL
 
 
 def test_edge_shapes():
