@@ -166,6 +166,7 @@ SIGNATURES = {
     "axspmv_batch_graph": (C.c_int, [vp, vp, vp, u32, P(vp)]),
     "axspmv_graph_launch": (C.c_int, [vp, vp]),
     "axspmv_graph_destroy": (C.c_int, [vp]),
+    "axspmv_autotune_defaults": (C.c_int, [u32, P(Opts), u32, P(u32)]),
     "axspmv_autotune": (C.c_int, [P(Csr), u32, u32, P(Opts), u32, u32, vp, P(u32), P(f64)]),
     "axspmv_autotune_batch": (C.c_int, [P(P(Csr)), u32, P(Opts), u32, u32, vp, P(Opts), P(u32)]),
     "axspmv_dev_alloc": (C.c_int, [C.c_size_t, P(vp)]),

@@ -441,15 +441,26 @@ class Batch:
             pass
 
 
+def autotune_defaults(op: int = L.OP_SPMV) -> list:
+    """The library's built-in candidate option sets for `op`."""
+    cnt = C.c_uint32()
+    L.call("axspmv_autotune_defaults", op, None, 0, C.byref(cnt))
+    out = (L.Opts * cnt.value)()
+    L.call("axspmv_autotune_defaults", op, out, cnt.value, C.byref(cnt))
+    return list(out)
+
+
 def autotune(m: CsrMatrix, op: int = L.OP_SPMV, k: int = 0, candidates=None, iters: int = 20):
-    """Index of the fastest candidate (list of Opts, or None for the built-in
-    set) and each candidate's time in ms (-1 = rejected)."""
-    n = len(candidates) if candidates is not None else 0
-    cand = (L.Opts * n)(*candidates) if candidates is not None else None
-    times = np.full(max(n, 16), -1.0)
+    """Index of the fastest candidate (list of Opts; None = the built-in set,
+    see autotune_defaults) and each candidate's time in ms (-1 = rejected)."""
+    if candidates is None:
+        candidates = autotune_defaults(op)
+    n = len(candidates)
+    cand = (L.Opts * n)(*candidates)
+    times = np.full(n, -1.0)
     best = C.c_uint32()
     L.call("axspmv_autotune", m.ref, op, k, cand, n, iters, None, C.byref(best), _ptr(times, L.f64))
-    return best.value, times[:n] if n else times
+    return best.value, times
 
 
 def autotune_batch(mats, candidates=None, iters: int = 20):

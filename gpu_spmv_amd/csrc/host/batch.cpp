@@ -188,7 +188,9 @@ extern "C" int axspmv_batch_time(axspmv_batch *b, const double *d_x, double *d_y
 // ---------------------------------------------------------------- tuner
 namespace {
 
-std::vector<axspmv_opts> default_candidates(uint32_t op) {
+// Built-in candidates.  Batches take AXT UNC only (the persistent batch
+// kernel runs that layout); single plans also try K1 and RCT.
+std::vector<axspmv_opts> default_candidates(uint32_t op, bool for_batch = false) {
     std::vector<axspmv_opts> c;
     axspmv_opts o;
     axspmv_opts_default(AXSPMV_FMT_AXT, &o);
@@ -212,10 +214,27 @@ std::vector<axspmv_opts> default_candidates(uint32_t op) {
             c.push_back(o);
         }
     }
+    if (for_batch) return c;
+    axspmv_opts_default(AXSPMV_FMT_K1, &o);
+    c.push_back(o);
+    axspmv_opts_default(AXSPMV_FMT_RCT, &o);
+    for (uint32_t block : {0u, 256u, 1024u}) {
+        o.block = block;
+        c.push_back(o);
+    }
     return c;
 }
 
 }  // namespace
+
+extern "C" int axspmv_autotune_defaults(uint32_t op, axspmv_opts *out, uint32_t cap, uint32_t *count) {
+    if (!count || (op != AXSPMV_OP_SPMV && op != AXSPMV_OP_SPMM) || (cap && !out))
+        return fail(AXSPMV_E_INVALID, "autotune_defaults: bad argument");
+    const std::vector<axspmv_opts> cs = default_candidates(op);
+    *count = uint32_t(cs.size());
+    for (uint32_t i = 0; i < cap && i < cs.size(); ++i) out[i] = cs[i];
+    return AXSPMV_OK;
+}
 
 extern "C" int axspmv_autotune(const axspmv_csr *a, uint32_t op, uint32_t k, const axspmv_opts *cand, uint32_t ncand,
                                uint32_t iters, void *stream, uint32_t *best, double *times_ms) {
@@ -223,6 +242,9 @@ extern "C" int axspmv_autotune(const axspmv_csr *a, uint32_t op, uint32_t k, con
         return fail(AXSPMV_E_INVALID, "autotune: bad argument");
     std::vector<axspmv_opts> cs = cand ? std::vector<axspmv_opts>(cand, cand + ncand) : default_candidates(op);
     if (cs.empty()) return fail(AXSPMV_E_INVALID, "autotune: no candidates");
+    if (!cand && times_ms && ncand < cs.size())
+        return fail(AXSPMV_E_INVALID, "autotune: times_ms holds %u entries, the built-in set has %zu", ncand,
+                    cs.size());
     if (iters == 0) iters = 20;
     const uint64_t kk = op == AXSPMV_OP_SPMM ? k : 1;
     void *dx = nullptr, *dy = nullptr;
@@ -271,7 +293,8 @@ extern "C" int axspmv_autotune_batch(const axspmv_csr *const *mats, uint32_t cou
                                      uint32_t ncand, uint32_t iters, void *stream, axspmv_opts *out_opts,
                                      uint32_t *ngroups) {
     if (!mats || !out_opts || count == 0) return fail(AXSPMV_E_INVALID, "autotune_batch: bad argument");
-    std::vector<axspmv_opts> cs = cand ? std::vector<axspmv_opts>(cand, cand + ncand) : default_candidates(AXSPMV_OP_SPMV);
+    std::vector<axspmv_opts> cs =
+        cand ? std::vector<axspmv_opts>(cand, cand + ncand) : default_candidates(AXSPMV_OP_SPMV, true);
     std::map<std::pair<uint32_t, uint32_t>, uint32_t> chosen;  // shape -> candidate index
     for (uint32_t i = 0; i < count; ++i) {
         const axspmv_csr *a = mats[i];

@@ -430,15 +430,22 @@ int axspmv_graph_destroy(axspmv_graph *g);
  * back-to-back launches of `op`, AXSPMV_OP_SPMV or AXSPMV_OP_SPMM with k
  * right-hand sides) and returns the index of the fastest in *best.
  * times_ms (ncand entries, may be NULL) gets each candidate's time per
- * operation, or -1 for a candidate the library rejects.  cand = NULL uses a
- * built-in set (AXT UNC th = 1 with thw 2..16, th = 2..8 with thw 64, two
- * block budgets; for SpMM the four k-tiles). */
+ * operation, or -1 for a candidate the library rejects.  cand = NULL uses
+ * the built-in set of axspmv_autotune_defaults; ncand then gives the room in
+ * times_ms (which must hold the whole set).  No reference counterpart: the
+ * reference sweeps block sizes by relaunching (jobLauncher.sh:145-176). */
+/* The built-in candidates: for SpMV AXT UNC th = 1 with thw 2..16 and th =
+ * 2..8 with thw 64 at two block budgets, K1 (chunk 64) and RCT (default, 256
+ * and 1024 threads); for SpMM the four k-tiles.  Writes min(cap, count)
+ * options to out and the set's size to *count. */
+int axspmv_autotune_defaults(uint32_t op, axspmv_opts *out, uint32_t cap, uint32_t *count);
 int axspmv_autotune(const axspmv_csr *a, uint32_t op, uint32_t k, const axspmv_opts *cand,
                     uint32_t ncand, uint32_t iters, void *stream, uint32_t *best, double *times_ms);
 /* Per-matrix choice for a batch: matrices are grouped by shape (rows rounded
  * to a power of two, mean row length in steps of 4); the first matrix of
  * each group is tuned with axspmv_autotune (SpMV) and its winner is written
- * to out_opts[i] for every member.  *ngroups gets the number of groups. */
+ * to out_opts[i] for every member.  *ngroups gets the number of groups.
+ * cand = NULL uses the AXT UNC part of the built-in set (batches run AXT UNC). */
 int axspmv_autotune_batch(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *cand,
                           uint32_t ncand, uint32_t iters, void *stream, axspmv_opts *out_opts,
                           uint32_t *ngroups);
