@@ -38,6 +38,7 @@ BUILD_AX, BUILD_PLANES = 1, 2
 PLAN_TRANSPOSE = 1
 PLAN_RCT_SPLIT = 2
 PLAN_AX_LAYOUT = 4
+PLAN_STAGE_X = 8
 
 
 class Csr(C.Structure):
@@ -94,7 +95,8 @@ class Opts(C.Structure):
 class PlanInfo(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("nrows", u32), ("ncols", u32), ("nnz", u32),
                 ("format", u32), ("slots", u64), ("device_bytes", u64), ("stream_bytes", u64),
-                ("alg_bytes", u64), ("beta", f64), ("launches", u32), ("nblocks", u32)]
+                ("alg_bytes", u64), ("beta", f64), ("launches", u32), ("nblocks", u32),
+                ("staged_blocks", u32), ("pad_", u32)]
 
 
 class BatchInfo(C.Structure):

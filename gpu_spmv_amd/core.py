@@ -279,7 +279,8 @@ class Plan:
         L.call("axspmv_plan_get_info", self.ptr, C.byref(i))
         return dict(name=i.name.decode(), nrows=i.nrows, ncols=i.ncols, nnz=i.nnz, format=i.format,
                     slots=i.slots, device_bytes=i.device_bytes, stream_bytes=i.stream_bytes,
-                    alg_bytes=i.alg_bytes, beta=i.beta, launches=i.launches, nblocks=i.nblocks)
+                    alg_bytes=i.alg_bytes, beta=i.beta, launches=i.launches, nblocks=i.nblocks,
+                    staged_blocks=i.staged_blocks)
 
     def spmv(self, dx, dy, stream=None) -> None:
         L.call("axspmv_spmv", self.ptr, _dptr(dx), _dptr(dy), stream)

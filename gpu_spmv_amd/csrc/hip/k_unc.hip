@@ -24,17 +24,29 @@ namespace {
 using namespace axspmv::dev;
 using axspmv::launch::Block;
 
-template <class Src>
+// STAGE: blocks whose window fits STAGE_COLS read x from a per-wave LDS copy
+// (banded / stencil structure); the others, and every block when STAGE is
+// false, gather from HBM.
+template <class Src, bool STAGE>
 __global__ void __launch_bounds__(256) rowblock_kernel(Src src, const Block *__restrict__ blocks, uint32_t nblocks,
                                                        const uint32_t *__restrict__ empty, uint32_t nempty,
-                                                       const double *__restrict__ x, double *__restrict__ y) {
+                                                       const Window *__restrict__ win, const double *__restrict__ x,
+                                                       double *__restrict__ y) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t i = wave * 64 + lane; i < nempty; i += nwaves * 64) y[empty[i]] = 0.0;
     if (wave >= nblocks) return;
     const Block b = blocks[wave];
-    rowblock_process(src, b, x, y);
+    if constexpr (STAGE) {
+        __shared__ double xs[4][axspmv::launch::STAGE_COLS];
+        const Window w = win[wave];
+        if (w.width <= axspmv::launch::STAGE_COLS) {
+            rowblock_process(src, b, stage_window(xs[threadIdx.x >> 6], w, x), y);
+            return;
+        }
+    }
+    rowblock_process(src, b, XGlobal{x}, y);
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -116,7 +128,7 @@ __global__ void __launch_bounds__(256) fused_kernel(UncH1Src<VT, K> src, const B
     // one long row
     const uint32_t r = src.row(bl.beg);
     double acc = 0.0;
-    for (uint32_t u = bl.beg + lane; u < bl.end; u += 64) acc += src.sum(u, x);
+    for (uint32_t u = bl.beg + lane; u < bl.end; u += 64) acc += src.sum(u, XGlobal{x});
     const double tot = group_sum_f64<64>(acc);
     const double rv = tot - b[r];
     if (lane == 0) {
@@ -132,10 +144,13 @@ __global__ void __launch_bounds__(256) fused_kernel(UncH1Src<VT, K> src, const B
 
 template <class Src>
 hipError_t run(const Src &src, const axspmv::launch::SegArgs &a, const double *x, double *y, hipStream_t s) {
-    const uint32_t block = 256;
+    const uint32_t block = 256;  // 4 waves: the LDS windows are indexed by wave
     uint32_t waves = a.nblocks ? a.nblocks : 1;
     const unsigned grid = (waves + block / 64 - 1) / (block / 64);
-    rowblock_kernel<Src><<<grid, block, 0, s>>>(src, a.blocks, a.nblocks, a.empty, a.nempty, x, y);
+    if (a.win)
+        rowblock_kernel<Src, true><<<grid, block, 0, s>>>(src, a.blocks, a.nblocks, a.empty, a.nempty, a.win, x, y);
+    else
+        rowblock_kernel<Src, false><<<grid, block, 0, s>>>(src, a.blocks, a.nblocks, a.empty, a.nempty, nullptr, x, y);
     return hipGetLastError();
 }
 

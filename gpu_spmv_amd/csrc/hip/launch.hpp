@@ -13,6 +13,13 @@ struct Block {
     uint32_t beg, end;
 };
 
+// Columns a wave block touches, [cmin, cmin + width): with width <=
+// STAGE_COLS the wave stages that part of x in LDS before its gathers.
+struct Window {
+    uint32_t cmin, width;
+};
+constexpr uint32_t STAGE_COLS = 1024;
+
 // Row-aligned wave-block kernels (AXT UNC th==1 / th>1, CSR stream).
 struct SegArgs {
     const void *vplane;       // values (double or float by vtype)
@@ -25,6 +32,7 @@ struct SegArgs {
     uint32_t thw, th;         // AXT tile half width / height
     uint32_t vtype;           // 0 = f64 values, 1 = f32 values
     uint32_t block;           // threads per workgroup
+    const Window *win;        // per block (nullptr: x from HBM for every block)
 };
 hipError_t axt_unc_h1(const SegArgs &a, const double *x, double *y, hipStream_t s);
 // As axt_unc_h1 on the interleaved layout: a.vplane is the FP64 ax array
@@ -99,6 +107,7 @@ struct BatchArgs {
     uint32_t nitems, nwaves, vtype;
     const uint32_t *empty;  // global y indices of rows without entries
     uint32_t nempty;
+    const Window *win;      // per item (nullptr: x from HBM for every item)
 };
 hipError_t batch(const BatchArgs &a, const double *x, double *y, hipStream_t s);
 

@@ -14,6 +14,40 @@
 namespace axspmv {
 namespace dev {
 using axspmv::launch::Block;
+using axspmv::launch::Window;
+
+// x as the kernels read it: straight from HBM, or from a wave's LDS copy of
+// the columns [cmin, cmin + width) its block touches.  A column outside the
+// window belongs to a padding slot (value 0) and reads 0.
+struct XGlobal {
+    const double *__restrict__ p;
+    __device__ __forceinline__ double operator()(uint32_t c) const { return p[c]; }
+};
+struct XWindow {
+    const double *xw;
+    uint32_t cmin, width;
+    __device__ __forceinline__ double operator()(uint32_t c) const {
+        const uint32_t i = c - cmin;
+        return i < width ? xw[i] : 0.0;
+    }
+};
+
+// LDS stores of this wave become visible to its own later loads (and loads
+// finish before later stores), without a workgroup barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The wave copies x[cmin, cmin + width) into xw (width <= STAGE_COLS).
+__device__ __forceinline__ XWindow stage_window(double *xw, const Window w, const double *__restrict__ x) {
+    const uint32_t lane = threadIdx.x & 63;
+    wave_lds_sync();  // the previous item's reads are done
+    for (uint32_t i = lane; i < w.width; i += 64) xw[i] = x[w.cmin + i];
+    wave_lds_sync();
+    return XWindow{xw, w.cmin, w.width};
+}
 
 template <class VT, int K>
 struct VecLoad;
@@ -73,7 +107,8 @@ struct UncH1Src {
     const uint32_t *__restrict__ cp;
     const uint32_t *__restrict__ sec;
     uint32_t ushift;  // log2(thw / K)
-    __device__ __forceinline__ double sum(uint32_t u, const double *__restrict__ x) const {
+    template <class XA>
+    __device__ __forceinline__ double sum(uint32_t u, const XA &x) const {
         const size_t q = size_t(u) * K;
         VT v[K];
         uint32_t c[K];
@@ -81,7 +116,7 @@ struct UncH1Src {
         load_cols<K>(cp + q, c);
         double xv[K];
 #pragma unroll
-        for (int j = 0; j < K; ++j) xv[j] = x[c[j]];
+        for (int j = 0; j < K; ++j) xv[j] = x(c[j]);
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < K; ++j) acc = fma(double(v[j]), xv[j], acc);
@@ -119,7 +154,8 @@ struct AxH1Src {
     const uint32_t *__restrict__ sec;
     uint32_t ushift;   // log2(thw / K)
     uint32_t thw_log;
-    __device__ __forceinline__ double sum(uint32_t u, const double *__restrict__) const {
+    template <class XA>
+    __device__ __forceinline__ double sum(uint32_t u, const XA &) const {
         const size_t q = size_t(u) * K;
         const size_t p = ((q >> thw_log) << (thw_log + 1)) + (q & ((size_t(1) << thw_log) - 1));
         double v[K], xv[K];
@@ -141,7 +177,8 @@ struct UncHSrc {
     const uint32_t *__restrict__ cp;
     const uint32_t *__restrict__ sec;
     uint32_t thw_log, th;
-    __device__ __forceinline__ double sum(uint32_t g, const double *__restrict__ x) const {
+    template <class XA>
+    __device__ __forceinline__ double sum(uint32_t g, const XA &x) const {
         const uint32_t thw = 1u << thw_log;
         const uint32_t h = TH > 0 ? uint32_t(TH) : th;
         const size_t base = size_t(g >> thw_log) * (size_t(h) << thw_log) + (g & (thw - 1));
@@ -152,22 +189,21 @@ struct UncHSrc {
 #pragma unroll
             for (int f = 0; f < TH; ++f) {
                 v[f] = vp[base + size_t(f) * thw];
-                xv[f] = x[cp[base + size_t(f) * thw]];
+                xv[f] = x(cp[base + size_t(f) * thw]);
             }
 #pragma unroll
             for (int f = 0; f < TH; ++f) acc = fma(double(v[f]), xv[f], acc);
         } else {
 #pragma unroll 4
-            for (uint32_t f = 0; f < h; ++f) acc = fma(double(vp[base + size_t(f) * thw]), x[cp[base + size_t(f) * thw]], acc);
+            for (uint32_t f = 0; f < h; ++f) acc = fma(double(vp[base + size_t(f) * thw]), x(cp[base + size_t(f) * thw]), acc);
         }
         return acc;
     }
     __device__ __forceinline__ uint32_t row(uint32_t g) const { return sec[g]; }
 };
 
-template <class Src>
-__device__ __forceinline__ void rowblock_process(const Src &src, const Block b, const double *__restrict__ x,
-                                                 double *__restrict__ y) {
+template <class Src, class XA>
+__device__ __forceinline__ void rowblock_process(const Src &src, const Block b, const XA &x, double *__restrict__ y) {
     const uint32_t lane = threadIdx.x & 63;
     uint32_t crow = NOROW;  // row of the segment still open at the end of the previous 64 units
     double carry = 0.0;

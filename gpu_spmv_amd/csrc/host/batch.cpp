@@ -3,6 +3,7 @@
 #include <cmath>
 #include <map>
 #include <memory>
+#include <numeric>
 #include <tuple>
 
 #include "plan.hpp"
@@ -61,6 +62,7 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
     const size_t vs = vtype == AXSPMV_F32 ? 4 : 8;
     std::vector<launch::BatchMat> desc(count);
     std::vector<launch::BatchItem> items;
+    std::vector<launch::Window> wins;  // column window of every item
     std::vector<uint32_t> empty;
     std::vector<double> vcat;
     std::vector<float> vcat32;
@@ -114,8 +116,12 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
             if (rl == 0) empty.push_back(uint32_t(yoff) + r);
             units[r] = o.th == 1 ? uint32_t(ceil_div(rl, o.thw) * (o.thw / K)) : uint32_t(ceil_div(rl, o.th));
         }
-        for (const launch::Block &bl : row_blocks(units, o.budget ? o.budget : 256))
-            items.push_back({i, bl.beg, bl.end, 0});
+        const std::vector<launch::Block> blocks = row_blocks(units, o.budget ? o.budget : 256);
+        const std::vector<launch::Window> w = block_windows(a, units, blocks);
+        for (size_t k = 0; k < blocks.size(); ++k) {
+            items.push_back({i, blocks[k].beg, blocks[k].end, 0});
+            wins.push_back(w[k]);
+        }
         b->info.nrows += a->nrows;
         b->info.ncols += a->ncols;
         b->info.nnz += a->nnz;
@@ -124,10 +130,23 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
         xoff += a->ncols;
         yoff += a->nrows;
     }
-    // longest blocks first across the whole batch
-    std::stable_sort(items.begin(), items.end(), [](const launch::BatchItem &p, const launch::BatchItem &q) {
-        return p.end - p.beg > q.end - q.beg;
+    // longest blocks first across the whole batch (windows follow their items)
+    std::vector<uint32_t> order(items.size());
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t p, uint32_t q) {
+        return items[p].end - items[p].beg > items[q].end - items[q].beg;
     });
+    {
+        std::vector<launch::BatchItem> it2(items.size());
+        std::vector<launch::Window> w2(wins.size());
+        for (size_t k = 0; k < order.size(); ++k) {
+            it2[k] = items[order[k]];
+            w2[k] = wins[order[k]];
+        }
+        items.swap(it2);
+        wins.swap(w2);
+    }
+    const bool stage = (opts[0].flags & AXSPMV_PLAN_STAGE_X) && worth_staging(wins);
     const uint64_t nsl = ccat.size();
     void *dv = b->dalloc(vs * nsl);
     auto *dc = static_cast<uint32_t *>(b->dalloc(4 * nsl));
@@ -135,7 +154,9 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
     auto *dm = static_cast<launch::BatchMat *>(b->dalloc(sizeof(launch::BatchMat) * desc.size()));
     auto *di = static_cast<launch::BatchItem *>(b->dalloc(sizeof(launch::BatchItem) * items.size()));
     auto *de = static_cast<uint32_t *>(b->dalloc(4 * empty.size()));
-    if (!dv || !dc || !ds || !dm || !di || !de) return AXSPMV_E_NOMEM;
+    auto *dw = stage ? static_cast<launch::Window *>(b->dalloc(sizeof(launch::Window) * wins.size())) : nullptr;
+    if (!dv || !dc || !ds || !dm || !di || !de || (stage && !dw)) return AXSPMV_E_NOMEM;
+    if (stage) AX_TRY(h2d(dw, wins.data(), sizeof(launch::Window) * wins.size(), s));
     AX_TRY(h2d(dv, vtype == AXSPMV_F32 ? static_cast<const void *>(vcat32.data()) : vcat.data(), vs * nsl, s));
     AX_TRY(h2d(dc, ccat.data(), 4 * nsl, s));
     AX_TRY(h2d(ds, scat.data(), 4 * scat.size(), s));
@@ -153,6 +174,7 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
     g.vtype = vtype;
     g.empty = de;
     g.nempty = uint32_t(empty.size());
+    g.win = dw;
     b->info.count = count;
     b->info.nitems = g.nitems;
     b->info.nwaves = g.nwaves;

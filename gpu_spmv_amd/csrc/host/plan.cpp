@@ -86,6 +86,32 @@ std::vector<launch::Block> row_blocks(const std::vector<uint32_t> &units_per_row
     return big;
 }
 
+std::vector<launch::Window> block_windows(const axspmv_csr *a, const std::vector<uint32_t> &units,
+                                          const std::vector<launch::Block> &blocks) {
+    const uint32_t n = a->nrows;
+    std::vector<uint64_t> cum(size_t(n) + 1, 0);
+    for (uint32_t r = 0; r < n; ++r) cum[r + 1] = cum[r] + units[r];
+    std::vector<launch::Window> w(blocks.size(), launch::Window{0, 0});
+    for (size_t i = 0; i < blocks.size(); ++i) {
+        const launch::Block &b = blocks[i];
+        uint32_t r = uint32_t(std::upper_bound(cum.begin(), cum.end(), uint64_t(b.beg)) - cum.begin()) - 1;
+        uint32_t cmin = UINT32_MAX, cmax = 0;
+        for (; r < n && cum[r] < b.end; ++r) {
+            if (a->row[r + 1] == a->row[r]) continue;
+            cmin = std::min(cmin, a->col[a->row[r]]);
+            cmax = std::max(cmax, a->col[a->row[r + 1] - 1]);
+        }
+        if (cmin <= cmax) w[i] = launch::Window{cmin, cmax - cmin + 1};
+    }
+    return w;
+}
+
+bool worth_staging(const std::vector<launch::Window> &w) {
+    size_t fit = 0;
+    for (const launch::Window &x : w) fit += x.width <= launch::STAGE_COLS;
+    return !w.empty() && 2 * fit >= w.size();
+}
+
 }  // namespace axspmv
 
 namespace {
@@ -307,7 +333,20 @@ int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
     }
     // Budget in units: 4 passes of 64 lanes per wave block by default.
     const uint32_t budget = p.o.budget ? p.o.budget : 256;
-    AX_TRY(upload_blocks(p, row_blocks(units, budget), empty, s));
+    const std::vector<launch::Block> blocks = row_blocks(units, budget);
+    AX_TRY(upload_blocks(p, blocks, empty, s));
+    // On request, banded / stencil structure: blocks whose columns fit a
+    // per-wave LDS window read x from there (not the AX layout, which reads
+    // no x).
+    if (!axl && (p.o.flags & AXSPMV_PLAN_STAGE_X)) {
+        const std::vector<launch::Window> win = block_windows(a, units, blocks);
+        if (worth_staging(win)) {
+            for (const launch::Window &w : win) p.staged_blocks += w.width <= launch::STAGE_COLS;
+            p.d_win = static_cast<launch::Window *>(p.dalloc(sizeof(launch::Window) * win.size()));
+            if (!p.d_win) return AXSPMV_E_NOMEM;
+            AX_TRY(p.upload(p.d_win, win.data(), sizeof(launch::Window) * win.size(), s));
+        }
+    }
     p.thw = thw;
     p.th = th;
     if (axl) {
@@ -589,6 +628,7 @@ launch::SegArgs seg_args(const Plan &p, const uint32_t *rowsrc) {
     g.th = p.th;
     g.vtype = p.o.vtype;
     g.block = p.o.block;
+    g.win = p.d_win;
     return g;
 }
 
@@ -670,6 +710,7 @@ extern "C" int axspmv_plan_get_info(const axspmv_plan *p, axspmv_plan_info *info
     info->beta = p->beta;
     info->launches = p->launches;
     info->nblocks = p->nblocks;
+    info->staged_blocks = p->staged_blocks;
     return AXSPMV_OK;
 }
 
@@ -690,6 +731,7 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
             if (e != hipSuccess) return e;
             launch::SegArgs g = seg_args(*p, p->d_sec);
             g.vplane = p->d_ax;
+            g.win = nullptr;
             return launch::axt_unc_h1_ax(g, y, s);
         }
         default: return hipErrorInvalidValue;
@@ -711,6 +753,7 @@ launch::SegArgs fused_args(const Plan &p) {
     launch::SegArgs g = seg_args(p, p.d_sec);
     g.blocks = p.d_fblocks;
     g.nblocks = p.nfblocks;
+    g.win = nullptr;  // windows belong to the SpMV blocks
     return g;
 }
 

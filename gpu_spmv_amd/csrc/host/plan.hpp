@@ -19,7 +19,7 @@ struct axspmv_plan {
     char name[64] = "";
     uint64_t slots = 0, device_bytes = 0, stream_bytes = 0, alg_bytes = 0;
     double beta = 0.0;
-    uint32_t launches = 1, nblocks = 0, nempty = 0;
+    uint32_t launches = 1, nblocks = 0, nempty = 0, staged_blocks = 0;
     uint32_t thw = 0, th = 0;
 
     // CSR arrays
@@ -28,6 +28,7 @@ struct axspmv_plan {
     // planes + metadata of the tiled formats
     void *d_vplane = nullptr;
     uint32_t *d_cplane = nullptr, *d_sec = nullptr, *d_empty = nullptr;
+    axspmv::launch::Window *d_win = nullptr;  // per wave block, when x staging pays
     axspmv::launch::Block *d_blocks = nullptr;
     // K1 work items / AXC long rows
     axspmv::launch::K1Args k1{};
@@ -71,4 +72,10 @@ int time_launches(const std::function<hipError_t(hipStream_t)> &fn, uint32_t ite
 // Captures `repeats` calls of fn into an executable graph.
 int capture_graph(const std::function<hipError_t(hipStream_t)> &fn, uint32_t repeats, axspmv_graph **out);
 std::vector<launch::Block> row_blocks(const std::vector<uint32_t> &units_per_row, uint32_t budget);
+// Column window of every block of row_blocks(units_per_row, ...): the blocks
+// hold whole consecutive rows, and columns are sorted within a row.
+std::vector<launch::Window> block_windows(const axspmv_csr *a, const std::vector<uint32_t> &units_per_row,
+                                          const std::vector<launch::Block> &blocks);
+// True when at least half of the windows fit STAGE_COLS.
+bool worth_staging(const std::vector<launch::Window> &w);
 }  // namespace axspmv

@@ -242,6 +242,14 @@ typedef struct axspmv_opts {
 #define AXSPMV_PLAN_RCT_SPLIT 2u /* RCT: keep column (4 B) and tile row (2 B) of
                                     each entry apart even when both fit one
                                     32-bit key */
+#define AXSPMV_PLAN_STAGE_X 8u   /* AXT UNC (plans and batches): when at least
+                                    half of the wave blocks touch <= 1024
+                                    consecutive columns (banded / stencil
+                                    structure), each such block first copies
+                                    its part of x to LDS and gathers from
+                                    there.  Off by default: on gfx950 the
+                                    vector L1 already serves these gathers and
+                                    the staging step measured slower. */
 #define AXSPMV_PLAN_AX_LAYOUT 4u /* AXT UNC th == 1, FP64 values: keep the
                                     reference's interleaved ax array on the
                                     device (tiles of thw values then thw
@@ -274,6 +282,8 @@ typedef struct axspmv_plan_info {
     double beta;              /* nnz / slots                              */
     uint32_t launches;        /* kernel launches per SpMV                 */
     uint32_t nblocks;         /* wave blocks (segmented kernels)          */
+    uint32_t staged_blocks;   /* wave blocks that read x from LDS         */
+    uint32_t pad_;
 } axspmv_plan_info;
 int axspmv_plan_get_info(const axspmv_plan *p, axspmv_plan_info *info);
 

@@ -63,6 +63,24 @@ def test_batch_mixed_per_matrix_options():
     check_batch(b, MATS, ax.uniform(78, int(b.xoff[-1])))
 
 
+def test_batch_lds_staging_is_exact():
+    """Items that stage x in LDS give the same y, bit for bit, as without."""
+    for kw in (dict(thw=4), dict(thw=64, th=4)):
+        staged = ax.Batch(MATS, flags=ax.PLAN_STAGE_X, **kw)
+        plain = ax.Batch(MATS, **kw)
+        x = ax.uniform(80, int(staged.xoff[-1]))
+        dx = ax.DeviceBuffer.from_array(x)
+        ny = int(staged.yoff[-1])
+        ys = []
+        for b in (staged, plain):
+            dy = ax.DeviceBuffer(8 * ny)
+            b.spmv(dx, dy)
+            ax._lib.call("axspmv_device_sync")
+            ys.append(dy.download(np.float64, ny))
+        assert np.array_equal(ys[0], ys[1]), kw
+        check_batch(staged, MATS, x)
+
+
 def test_autotune_and_tuned_batch():
     defaults = ax.autotune_defaults()
     assert {o.format for o in defaults} == {ax.FMT_AXT, ax.FMT_K1, ax.FMT_RCT}

@@ -158,6 +158,25 @@ def test_rct_unaligned_y(kw):
     assert rel_err(orc.ncsr(m, x), y[1:-1]) <= TOL
 
 
+@pytest.mark.parametrize("mname", ["laplace256", "band300", "banded_empty", "lap16", "long_rows", "powerlaw64k"])
+@pytest.mark.parametrize("kw", [dict(thw=4), dict(thw=2), dict(thw=64, th=4), dict(thw=32, th=6)])
+def test_lds_staging_is_exact(mname, kw):
+    """x from the per-wave LDS window gives the same y, bit for bit, as x from
+    HBM (same summation order); banded matrices stage, random ones do not."""
+    m = MATS[mname]
+    x = ax.uniform(14, m.ncols)
+    staged = ax.Plan(m, ax.FMT_AXT, flags=ax.PLAN_STAGE_X, **kw)
+    plain = ax.Plan(m, ax.FMT_AXT, **kw)
+    assert plain.info()["staged_blocks"] == 0
+    if mname in ("laplace256", "band300", "banded_empty", "lap16"):
+        assert staged.info()["staged_blocks"] > 0
+    if mname == "powerlaw64k":
+        assert staged.info()["staged_blocks"] == 0
+    y = ax.spmv_host(staged, x)
+    assert np.array_equal(y, ax.spmv_host(plain, x))
+    assert rel_err(orc.ncsr(m, x), y) <= TOL
+
+
 def test_edge_shapes():
     one = ax.CsrMatrix(np.array([0, 1], np.uint32), np.array([0], np.uint32), np.array([2.5]))
     assert ax.spmv_host(ax.Plan(one, ax.FMT_AXT, thw=4), np.array([3.0]))[0] == 7.5
