@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 
 import gpu_spmv_amd as ax  # noqa: E402
 from gpu_spmv_amd import _lib as L  # noqa: E402
-from gpu_spmv_amd.core import TIME_GRAPH, TIME_KERNEL, TIME_REGION  # noqa: E402
+from gpu_spmv_amd.core import TIME_COLD, TIME_GRAPH, TIME_KERNEL, TIME_REGION  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 FLAGSHIP = ("rct_auto", ax.FMT_RCT, dict())
@@ -397,6 +397,11 @@ def main():
     path_s = t.mean_ms * 1e-3
     achieved = info["alg_bytes"] / kern_s / 1e9
     gflops = 2.0 * m.nnz / (wall / args.steps) / 1e9
+    # the same SpMV with the Infinity Cache and L2 flushed before each launch
+    tc = plan.time(dx, dy, iters=20, warmup=0, mode=TIME_COLD)
+    cold = dict(spmv_us_mean=round(tc.mean_ms * 1e3, 2), spmv_us_median=round(tc.median_ms * 1e3, 2),
+                alg_gbs=round(info["alg_bytes"] / (tc.median_ms * 1e-3) / 1e9, 1))
+    log(f"cold: {cold}")
     del plan
 
     formats = [flag]
@@ -446,6 +451,7 @@ def main():
                      "alg_bytes_per_launch": info["alg_bytes"], "launches_per_spmv": info["launches"],
                      "spmv_us": round(path_s * 1e6, 2),
                      "spmv_frac": round(info["alg_bytes"] / path_s / 1e9 / PEAK_HBM_GBS, 4),
+                     "cold_caches": cold,
                      "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "formats": formats,

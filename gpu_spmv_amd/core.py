@@ -250,7 +250,7 @@ def opts(fmt: int, **kw) -> L.Opts:
     return o
 
 
-TIME_EVENTS, TIME_GRAPH, TIME_REGION, TIME_KERNEL = 0, 1, 2, 3
+TIME_EVENTS, TIME_GRAPH, TIME_REGION, TIME_KERNEL, TIME_COLD = 0, 1, 2, 3, 4
 
 
 @dataclass

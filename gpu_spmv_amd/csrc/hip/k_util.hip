@@ -38,6 +38,18 @@ __global__ void __launch_bounds__(256) x_refresh_kernel(const uint32_t *__restri
     }
 }
 
+// Reads n2 double2 (a cache flush that leaves no dirty lines); the store
+// happens only for a sum no real data produces, so the loads stay.
+__global__ void __launch_bounds__(256) touch_kernel(const double2 *__restrict__ p, size_t n2, double *__restrict__ sink) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    double acc = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n2; i += stride) {
+        const double2 v = p[i];
+        acc += v.x + v.y;
+    }
+    if (acc == -1.0e300) sink[0] = acc;
+}
+
 unsigned grid_for(size_t work, unsigned block) {
     size_t g = (work + block - 1) / block;
     const size_t cap = 256 * 64;  // 256 CUs x 64 workgroups: enough to fill the chip
@@ -66,6 +78,11 @@ hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s) {
 hipError_t x_refresh(const uint32_t *map, uint64_t n, uint32_t hw_log, uint32_t nx, const double *x, double *ax,
                      hipStream_t s) {
     if (n) x_refresh_kernel<<<grid_for(n, 256), 256, 0, s>>>(map, n, hw_log, nx, x, ax);
+    return hipGetLastError();
+}
+
+hipError_t touch(const double *p, size_t n, double *sink, hipStream_t s) {
+    if (n / 2) touch_kernel<<<grid_for(n / 2, 256), 256, 0, s>>>(reinterpret_cast<const double2 *>(p), n / 2, sink);
     return hipGetLastError();
 }
 

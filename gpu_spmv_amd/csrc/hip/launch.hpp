@@ -188,6 +188,8 @@ hipError_t build_fill(const DevCsr &a, int kind, const uint64_t *start, const Bu
 
 hipError_t copy_f64(const double *src, double *dst, size_t n, hipStream_t s);
 hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s);
+// reads p[0 .. n) (n even, p 16-byte aligned): evicts caches without dirtying them
+hipError_t touch(const double *p, size_t n, double *sink, hipStream_t s);
 // y[rows[i]] = 0 for i < n
 hipError_t fill_rows(const uint32_t *rows, uint32_t n, double *y, hipStream_t s);
 

@@ -323,6 +323,9 @@ typedef struct axspmv_timing {
 #define AXSPMV_TIME_EVENTS 0u /* one hipEvent pair + sync per launch (reference) */
 #define AXSPMV_TIME_GRAPH 1u  /* replays of a hipGraph of `graph_batch` SpMVs    */
 #define AXSPMV_TIME_REGION 2u /* one event pair around `iters` back-to-back SpMVs */
+#define AXSPMV_TIME_COLD 4u   /* per-launch event pairs as EVENTS, each launch
+                                 preceded by a 512 MB read that evicts the
+                                 Infinity Cache and L2: times from HBM */
 #define AXSPMV_TIME_KERNEL 3u /* as REGION (SpMV only, iters <= 4096), with an
                                  event before every SpMV and one where its main
                                  kernel starts: main_ms is that kernel's mean

@@ -191,7 +191,7 @@ def test_timing_modes():
     x = ax.uniform(0, m.ncols)
     p = ax.Plan(m, ax.FMT_AXT, thw=4)
     dx, dy = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer(8 * m.nrows)
-    for mode in (ax.core.TIME_EVENTS, ax.core.TIME_GRAPH, ax.core.TIME_REGION):
+    for mode in (ax.core.TIME_EVENTS, ax.core.TIME_GRAPH, ax.core.TIME_REGION, ax.core.TIME_COLD):
         t = p.time(dx, dy, iters=20, warmup=2, mode=mode, batch=10)
         assert t.mean_ms > 0 and t.min_ms <= t.median_ms <= t.max_ms and t.mode == mode
     assert rel_err(orc.ncsr(m, x), dy.download(np.float64, m.nrows)) <= TOL
