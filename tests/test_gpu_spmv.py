@@ -177,13 +177,31 @@ def test_lds_staging_is_exact(mname, kw):
     assert rel_err(orc.ncsr(m, x), y) <= TOL
 
 
-def test_edge_shapes():
+EDGE_PLANS = [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_AXT, dict(thw=64, th=4)), (ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+              (ax.FMT_K1, dict(chunk=64)), (ax.FMT_AXC, dict(hbs=4)), (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1)),
+              (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)), (ax.FMT_RCT, dict()),
+              (ax.FMT_RCT, dict(slices=4, tile_rows=64)), (ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_AX_LAYOUT)),
+              (ax.FMT_AXT, dict(thw=4, vtype=ax.F32)), (ax.FMT_RCT, dict(vtype=ax.F32, slices=2))]
+
+
+@pytest.mark.parametrize("fmt,kw", EDGE_PLANS, ids=[f"{f}-{k}" for f, k in EDGE_PLANS])
+def test_edge_shapes(fmt, kw):
+    """One nonzero; no nonzeros at all (y = 0 over a dirty buffer); more
+    columns than rows."""
     one = ax.CsrMatrix(np.array([0, 1], np.uint32), np.array([0], np.uint32), np.array([2.5]))
-    assert ax.spmv_host(ax.Plan(one, ax.FMT_AXT, thw=4), np.array([3.0]))[0] == 7.5
+    assert ax.spmv_host(ax.Plan(one, fmt, **kw), np.array([3.0]))[0] == 7.5
     empty = ax.CsrMatrix(np.zeros(7, np.uint32), np.zeros(0, np.uint32), np.zeros(0), ncols=6)
-    for fmt, kw in ((ax.FMT_AXT, dict(thw=4)), (ax.FMT_AXT, dict(thw=64, th=4)), (ax.FMT_CSR, dict(kernel=ax.KER_NAIVE))):
-        y = ax.spmv_host(ax.Plan(empty, fmt, **kw), np.ones(6))
-        assert np.array_equal(y, np.zeros(6))
+    p = ax.Plan(empty, fmt, **kw)
+    dx = ax.DeviceBuffer.from_array(np.ones(6))
+    dy = ax.DeviceBuffer.from_array(np.full(6, np.nan))
+    p.spmv(dx, dy)
+    ax._lib.call("axspmv_device_sync")
+    assert np.array_equal(dy.download(np.float64, 6), np.zeros(6))
+    wide = ax.CsrMatrix(np.array([0, 2, 2, 5], np.uint32), np.array([1, 900, 0, 7, 999], np.uint32),
+                        np.array([1.0, -2.0, 0.5, 4.0, 3.0]), ncols=1000)
+    x = ax.uniform(15, 1000)
+    y = ax.spmv_host(ax.Plan(wide, fmt, **kw), x)
+    assert rel_err(orc.ncsr(wide, x), y) <= (TOL if kw.get("vtype") != ax.F32 else 1e-6)
 
 
 def test_timing_modes():
