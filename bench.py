@@ -141,7 +141,9 @@ def extras_same_matrix(m, x, dx, yref, plain_us):
                           r_rel_err=rel_err(r_ref, r) if r_ref is not None else None,
                           xbar_rel_err=rel_err(orc.ncsr_t(m, r_ref, nc), xbar) if r_ref is not None else None)
     log(f"fwd_adj: {out['fwd_adj']}")
-    del bufs
+    del bufs, p
+    out["fwd_adj_rct"] = fwd_adj_rct(m, dx, db, yref, b)
+    log(f"fwd_adj_rct: {out['fwd_adj_rct']}")
     # SpMM, FP64 and FP32 values
     sp = []
     for k in (8, 32, 64):
@@ -292,6 +294,9 @@ def suite_full(res):
     plain = pl.time(d3, e3, iters=20, warmup=0, mode=TIME_REGION).mean_ms * 1e3
     del pl
     res["stencil128p_fwd_adj"] = fwd_adj_only(m3, x3, d3, y5, plain)
+    b5 = ax.uniform(4, m3.nrows, stream=ax.VEC_STREAM + 1)
+    res["stencil128p_fwd_adj_rct"] = fwd_adj_rct(m3, d3, ax.DeviceBuffer.from_array(b5), y5, b5)
+    log(f"stencil fwd_adj_rct: {res['stencil128p_fwd_adj_rct']}")
     del m3, d3, e3
     # config 4B: 256 uneven banded matrices in one persistent launch
     rng = np.random.default_rng(5)
@@ -319,6 +324,26 @@ def suite_full(res):
                                       max_rel_err_sampled=max(errs), tuner_groups=ngroups)
         log(f"batch256 {label}: {res['batch256'][label]}")
         del dxb, dyb
+
+
+def fwd_adj_rct(m, dx, db, yref, b):
+    """Forward + adjoint on RCT plans: SpMV, residual, SpMV of the transposed
+    RCT plan, value gradient in entry order (four passes), against plain RCT."""
+    from oracle import oracle as orc
+    n, nc, nnz = m.nrows, m.ncols, m.nnz
+    p = ax.Plan(m, ax.FMT_RCT, flags=ax.PLAN_TRANSPOSE)
+    dy = ax.DeviceBuffer(8 * n)
+    p.time(dx, dy, iters=3, warmup=0, mode=TIME_REGION)
+    plain = p.time(dx, dy, iters=20, warmup=0, mode=TIME_REGION).mean_ms * 1e3
+    bufs = dict(y=dy, r=ax.DeviceBuffer(8 * n), xbar=ax.DeviceBuffer(8 * nc), vbar=ax.DeviceBuffer(8 * nnz))
+    t = p.time_op(L.OP_FWD_ADJ, iters=20, warmup=2, x=dx, b=db, **bufs)
+    out = dict(plan=p.info()["name"], plain_spmv_us=round(plain, 2), fwd_adj_us=round(t.mean_ms * 1e3, 2),
+               fwd_adj_over_plain=round(t.mean_ms * 1e3 / plain, 3))
+    if yref is not None:
+        r_ref = yref - b
+        out["r_rel_err"] = rel_err(r_ref, bufs["r"].download(np.float64, n))
+        out["xbar_rel_err"] = rel_err(orc.ncsr_t(m, r_ref, nc), bufs["xbar"].download(np.float64, nc))
+    return out
 
 
 def fwd_adj_only(m, x, dx, yref, plain_us):

@@ -154,6 +154,7 @@ SIGNATURES = {
     "axspmv_plan_get_info": (C.c_int, [vp, P(PlanInfo)]),
     "axspmv_spmv": (C.c_int, [vp, vp, vp, vp]),
     "axspmv_adjoint": (C.c_int, [vp, vp, vp, vp]),
+    "axspmv_plan_vbar_positions": (C.c_int, [vp, P(u32), u64]),
     "axspmv_vbar": (C.c_int, [vp, vp, vp, vp, vp]),
     "axspmv_fwd_adj": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]),
     "axspmv_spmm": (C.c_int, [vp, u32, vp, vp, vp]),

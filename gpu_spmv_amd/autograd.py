@@ -45,7 +45,7 @@ class SpMVFunction(torch.autograd.Function):
         dvb = DeviceBuffer(8 * max(plan.nnz, 1))
         plan.adjoint(dg, dxb)
         plan.vbar(dg, ctx.dx, dvb)
-        return (torch.from_numpy(dvb.download(np.float64, plan.nnz)),
+        return (torch.from_numpy(plan.vbar_csr(dvb.download(np.float64, plan.nnz))),
                 torch.from_numpy(dxb.download(np.float64, plan.ncols)), None, None, None)
 
 
@@ -61,7 +61,7 @@ class LeastSquaresFunction(torch.autograd.Function):
         plan.fwd_adj(dx, db, dy, dr, dxb, dvb, fused=fused)
         r = dr.download(np.float64, n)
         ctx.xbar = torch.from_numpy(dxb.download(np.float64, nc))
-        ctx.vbar = torch.from_numpy(dvb.download(np.float64, nnz))
+        ctx.vbar = torch.from_numpy(plan.vbar_csr(dvb.download(np.float64, nnz)))
         ctx.r = torch.from_numpy(r)
         return torch.tensor(0.5 * float(np.dot(r, r)), dtype=torch.float64)
 

@@ -312,6 +312,20 @@ class Plan:
         """Y = A X, X (ncols x k) and Y (nrows x k) row-major."""
         L.call("axspmv_spmm", self.ptr, k, _dptr(dX), _dptr(dY), stream)
 
+    def vbar_positions(self) -> np.ndarray:
+        """CSR position of every value-gradient entry (identity except RCT)."""
+        pos = np.empty(max(self.nnz, 1), np.uint32)
+        L.call("axspmv_plan_vbar_positions", self.ptr, _ptr(pos, L.u32), self.nnz)
+        return pos[:self.nnz]
+
+    def vbar_csr(self, v: np.ndarray) -> np.ndarray:
+        """A value gradient in CSR order (undoes RCT's entry order)."""
+        if self.opts.format != L.FMT_RCT:
+            return v
+        out = np.empty_like(v)
+        out[self.vbar_positions()] = v
+        return out
+
     def time_op(self, op: int, iters: int = 100, warmup: int = 10, mode: int = TIME_REGION, batch: int = 1,
                 stream=None, **operands) -> TimingResult:
         """Time one operation (L.OP_*); operands by name (x, b, ybar, X, y, r, xbar, vbar, Y, k)."""

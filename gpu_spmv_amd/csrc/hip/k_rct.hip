@@ -44,6 +44,10 @@ struct RctEntries {
     uint32_t rbits, cbase;
     __device__ __forceinline__ void load(uint32_t e, uint32_t &c, uint32_t &r, double &v) const {
         v = double(val[e]);
+        at(e, c, r);
+    }
+    // column and tile row only
+    __device__ __forceinline__ void at(uint32_t e, uint32_t &c, uint32_t &r) const {
         if constexpr (PACKED) {
             const uint32_t k = key[e];
             c = cbase + (k >> rbits);
@@ -55,14 +59,25 @@ struct RctEntries {
     }
 };
 
+// Tile w of the grid: XCD w % 8 takes slice (w % 8) % S; false for the
+// workgroups past the last row block.
+struct TileOf {
+    uint32_t s, rb;
+    __device__ __forceinline__ bool from(const RctArgs &a, uint32_t w) {
+        const uint32_t xcd = w % 8, i = w / 8;
+        const uint32_t per = 8 / a.S;  // row-block groups per slice
+        s = xcd % a.S;
+        rb = i * per + xcd / a.S;
+        return rb < a.nrb;
+    }
+};
+
 template <class VT, bool PACKED>
 __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__restrict__ x, double *__restrict__ y) {
     extern __shared__ double yl[];
-    const uint32_t w = blockIdx.x;
-    const uint32_t xcd = w % 8, i = w / 8;
-    const uint32_t per = 8 / a.S;  // row-block groups per slice
-    const uint32_t s = xcd % a.S, rb = i * per + xcd / a.S;
-    if (rb >= a.nrb) return;  // the whole workgroup leaves together
+    TileOf tile;
+    if (!tile.from(a, blockIdx.x)) return;  // the whole workgroup leaves together
+    const uint32_t s = tile.s, rb = tile.rb;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t row0 = a.rb_start[rb], rows = a.rb_start[rb + 1] - row0;  // rows <= a.R
     for (uint32_t k = tid; k < rows; k += nt) yl[k] = 0.0;
@@ -146,10 +161,56 @@ __global__ void __launch_bounds__(256) rct_clear(double *__restrict__ y, uint32_
     }
 }
 
+// Value gradient in entry order: vbar[e] = ybar[row(e)] * x[col(e)].  The
+// tile's ybar rows sit in LDS; x is gathered in the tile's column order.
+template <bool PACKED>
+__global__ void __launch_bounds__(1024) rct_vbar_kernel(RctArgs a, const double *__restrict__ ybar,
+                                                        const double *__restrict__ x, double *__restrict__ vbar) {
+    extern __shared__ double yb[];
+    TileOf tile;
+    if (!tile.from(a, blockIdx.x)) return;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t row0 = a.rb_start[tile.rb], rows = a.rb_start[tile.rb + 1] - row0;
+    for (uint32_t k = tid; k < rows; k += nt) yb[k] = ybar[row0 + k];
+    __syncthreads();
+    const uint32_t t = tile.rb * a.S + tile.s;
+    const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
+    const RctEntries<double, PACKED> ent{nullptr, a.key, a.col, a.lrow, a.rbits, tile.s * a.C};
+    constexpr int U = 4;
+    uint32_t e = beg + tid;
+    for (; e + (U - 1) * nt < end; e += U * nt) {
+        uint32_t c[U], r[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) ent.at(e + j * nt, c[j], r[j]);
+        double xv[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) xv[j] = x[c[j]];
+#pragma unroll
+        for (int j = 0; j < U; ++j) vbar[e + j * nt] = yb[r[j]] * xv[j];
+    }
+    for (; e < end; e += nt) {
+        uint32_t c, r;
+        ent.at(e, c, r);
+        vbar[e] = yb[r] * x[c];
+    }
+}
+
 }  // namespace
 
 namespace axspmv {
 namespace launch {
+
+hipError_t rct_vbar(const RctArgs &a, const double *ybar, const double *x, double *vbar, hipStream_t s) {
+    if (!a.nrows) return hipSuccess;
+    const uint32_t per = 8 / a.S;
+    const unsigned grid = (a.nrb + per - 1) / per * 8;
+    const size_t lds = size_t(a.R) * sizeof(double);
+    if (a.key)
+        rct_vbar_kernel<true><<<grid, a.block, lds, s>>>(a, ybar, x, vbar);
+    else
+        rct_vbar_kernel<false><<<grid, a.block, lds, s>>>(a, ybar, x, vbar);
+    return hipGetLastError();
+}
 
 hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid) {
     if (!a.nrows) return hipSuccess;

@@ -86,6 +86,8 @@ struct RctArgs {
 // S > 1: a fill kernel clears y, then the tile kernel adds into it; mid
 // (optional) is recorded on s between the two
 hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid = nullptr);
+// vbar[e] = ybar[row(e)] * x[col(e)] for every entry e, in entry (tile) order
+hipError_t rct_vbar(const RctArgs &a, const double *ybar, const double *x, double *vbar, hipStream_t s);
 
 // Batched persistent SpMV over AXT UNC matrices.
 struct BatchMat {

@@ -534,6 +534,9 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     std::vector<uint32_t> col(nnz ? nnz : 1);
     std::vector<uint16_t> lrow(nnz ? nnz : 1);
     std::vector<double> val(nnz ? nnz : 1);
+    // the value gradient comes out in entry order: keep each entry's CSR position
+    const bool keep_pos = p.o.flags & AXSPMV_PLAN_TRANSPOSE;
+    std::vector<uint32_t> pos(keep_pos && nnz ? nnz : 1);
 #pragma omp parallel
     {
         std::vector<std::pair<uint64_t, uint32_t>> key;  // (col << 16 | lrow, CSR position)
@@ -551,6 +554,7 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
                 col[q] = uint32_t(k.first >> 16);
                 lrow[q] = uint16_t(k.first & 0xffff);
                 val[q] = a->val[k.second];
+                if (keep_pos) pos[q] = k.second;
                 ++q;
             }
         }
@@ -573,6 +577,11 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     auto *dt = static_cast<uint32_t *>(p.dalloc(4 * tptr.size()));
     auto *drb = static_cast<uint32_t *>(p.dalloc(4 * rbs.size()));
     if (!dv || !dt || !drb) return AXSPMV_E_NOMEM;
+    if (keep_pos) {
+        p.d_rct_pos = static_cast<uint32_t *>(p.dalloc(4 * nnz));
+        if (!p.d_rct_pos) return AXSPMV_E_NOMEM;
+        AX_TRY(p.upload(p.d_rct_pos, pos.data(), 4 * nnz, s));
+    }
     AX_TRY(p.upload(drb, rbs.data(), 4 * rbs.size(), s));
     if (p.o.vtype == AXSPMV_F32) {
         std::vector<float> v(val.begin(), val.begin() + nnz);
@@ -758,6 +767,7 @@ launch::SegArgs fused_args(const Plan &p) {
 }
 
 hipError_t launch_vbar(Plan *p, const double *ybar, const double *x, double *vbar, hipStream_t s) {
+    if (p->kind == Plan::RCT) return launch::rct_vbar(p->rct, ybar, x, vbar, s);
     if (p->kind == Plan::AXT_UNC_H1)
         return launch::axt_unc_h1_vbar(seg_args(*p, p->d_sec), p->nunits, x, ybar, vbar,
                                        launch::VbarArgs{p->d_rowptr, p->d_first}, s);
@@ -767,7 +777,7 @@ hipError_t launch_vbar(Plan *p, const double *ybar, const double *x, double *vba
 hipError_t launch_fwd_adj(Plan *p, const double *x, const double *b, double *y, double *r, double *xbar,
                           double *vbar, bool fused, hipStream_t s) {
     hipError_t e;
-    if (fused) {
+    if (fused && p->kind != Plan::RCT) {  // RCT: r needs every slice's sums first
         if (p->kind == Plan::AXT_UNC_H1)
             e = launch::axt_unc_h1_fused(fused_args(*p), x, b, y, r, vbar, launch::VbarArgs{p->d_rowptr, p->d_first}, s);
         else
@@ -789,7 +799,9 @@ uint32_t pick_ktile(const Plan *p, uint32_t k) {
     return kt;
 }
 
-bool adjoint_kind(const Plan *p) { return p->kind == Plan::AXT_UNC_H1 || p->kind == Plan::CSR_NAIVE; }
+bool adjoint_kind(const Plan *p) {
+    return p->kind == Plan::AXT_UNC_H1 || p->kind == Plan::CSR_NAIVE || p->kind == Plan::RCT;
+}
 
 }  // namespace
 
@@ -827,7 +839,7 @@ int plan_op_check(const Plan *p, uint32_t op, const axspmv_operands &o) {
     const bool needT = op == AXSPMV_OP_ADJOINT || op == AXSPMV_OP_FWD_ADJ || op == AXSPMV_OP_FWD_ADJ_SPLIT;
     if (needT && !p->T) return fail(AXSPMV_E_INVALID, "the adjoint needs a plan created with AXSPMV_PLAN_TRANSPOSE");
     if ((op == AXSPMV_OP_VBAR || op == AXSPMV_OP_FWD_ADJ || op == AXSPMV_OP_FWD_ADJ_SPLIT) && !adjoint_kind(p))
-        return fail(AXSPMV_E_UNSUPPORTED, "value gradient: plan %s is not CSR naive or AXT UNC th=1", p->name);
+        return fail(AXSPMV_E_UNSUPPORTED, "value gradient: plan %s is not CSR naive, AXT UNC th=1 or RCT", p->name);
     if (op == AXSPMV_OP_SPMM) {
         if (!p->rowwise) return fail(AXSPMV_E_UNSUPPORTED, "SpMM: plan %s is not CSR, AXC or AXT UNC th=1", p->name);
         if (o.k == 0) return fail(AXSPMV_E_INVALID, "SpMM: k must be > 0");
@@ -888,4 +900,18 @@ extern "C" int axspmv_spmm(axspmv_plan *p, uint32_t k, const double *d_X, double
     AX_TRY(plan_op_check(p, AXSPMV_OP_SPMM, o));
     const hipError_t e = plan_op(p, AXSPMV_OP_SPMM, o, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "spmm launch");
+}
+
+extern "C" int axspmv_plan_vbar_positions(const axspmv_plan *p, uint32_t *h_pos, uint64_t count) {
+    if (!p || (count && !h_pos)) return fail(AXSPMV_E_INVALID, "vbar_positions: NULL argument");
+    if (count != p->nnz) return fail(AXSPMV_E_INVALID, "vbar_positions: count %llu != nnz %u", (unsigned long long)count, p->nnz);
+    if (p->kind == Plan::RCT) {
+        if (!p->d_rct_pos)
+            return fail(AXSPMV_E_UNSUPPORTED, "vbar_positions: RCT plan created without AXSPMV_PLAN_TRANSPOSE");
+        const hipError_t e = hipMemcpy(h_pos, p->d_rct_pos, 4ull * count, hipMemcpyDeviceToHost);
+        return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "vbar_positions copy");
+    }
+    if (!adjoint_kind(p)) return fail(AXSPMV_E_UNSUPPORTED, "vbar_positions: plan %s has no value gradient", p->name);
+    for (uint64_t k = 0; k < count; ++k) h_pos[k] = uint32_t(k);
+    return AXSPMV_OK;
 }

@@ -35,6 +35,7 @@ struct axspmv_plan {
     axspmv::launch::AxcArgs axc{};
     axspmv::launch::ComArgs com{};
     axspmv::launch::RctArgs rct{};
+    uint32_t *d_rct_pos = nullptr;  // CSR position of every RCT entry (plans with AXSPMV_PLAN_TRANSPOSE)
     // Row-contiguous layouts (CSR, AXT UNC th == 1, AXC): CSR row pointer and
     // first plane slot of each row, for SpMM and the value gradient.
     uint32_t *d_rowptr = nullptr, *d_first = nullptr;

@@ -21,6 +21,9 @@ ADJ_KERNELS = [
     ("unc_h1_thw8", ax.FMT_AXT, dict(thw=8, th=1)),
     ("unc_h1_thw2", ax.FMT_AXT, dict(thw=2, th=1)),
     ("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+    ("rct_auto", ax.FMT_RCT, dict()),
+    ("rct_r512_s4", ax.FMT_RCT, dict(tile_rows=512, slices=4)),
+    ("rct_r1024_s2_split", ax.FMT_RCT, dict(tile_rows=1024, slices=2, flags=ax.PLAN_RCT_SPLIT)),
 ]
 
 
@@ -49,14 +52,18 @@ def test_adjoint_vbar_fused(mname, kname, fmt, kw):
     x = ax.uniform(21, nc)
     b = ax.uniform(22, n, stream=ax.VEC_STREAM + 1)
     w = ax.uniform(23, n)
-    p = ax.Plan(m, fmt, flags=ax.PLAN_TRANSPOSE, **kw)
+    kw = dict(kw)
+    p = ax.Plan(m, fmt, flags=ax.PLAN_TRANSPOSE | kw.pop("flags", 0), **kw)
     dx, db, dw = (ax.DeviceBuffer.from_array(v) for v in (x, b, w))
     dxb, dvb = ax.DeviceBuffer(8 * nc), ax.DeviceBuffer(8 * max(m.nnz, 1))
     p.adjoint(dw, dxb)
     p.vbar(dw, dx, dvb)
     xb = dxb.download(np.float64, nc)
     assert rel_err(orc.ncsr_t(m, w, nc), xb) <= TOL
-    assert np.array_equal(dvb.download(np.float64, m.nnz), orc.vbar(m, w, x))
+    # v_bar in CSR order (RCT writes entry order; its positions map it back)
+    pos = p.vbar_positions()
+    assert np.array_equal(np.sort(pos), np.arange(m.nnz, dtype=np.uint32))
+    assert np.array_equal(p.vbar_csr(dvb.download(np.float64, m.nnz)), orc.vbar(m, w, x))
     y_ref = orc.ncsr(m, x)
     assert abs(np.dot(y_ref, w) - np.dot(x, xb)) <= 1e-11 * (np.abs(y_ref).sum() * np.abs(w).max() + 1)
     # fused and split forward + adjoint
@@ -65,6 +72,7 @@ def test_adjoint_vbar_fused(mname, kname, fmt, kw):
         outs = [ax.DeviceBuffer.from_array(np.full(s, np.nan)) for s in (n, n, nc, max(m.nnz, 1))]
         p.fwd_adj(dx, db, *outs, fused=fused)
         y, r, xbar, vbar = (o.download(np.float64, s) for o, s in zip(outs, (n, n, nc, m.nnz)))
+        vbar = p.vbar_csr(vbar)
         assert rel_err(y_ref, y) <= TOL
         assert rel_err(r_ref, r) <= TOL
         assert rel_err(orc.ncsr_t(m, r_ref, nc), xbar) <= 1e-11
@@ -109,7 +117,9 @@ def test_gradients_finite_differences_and_dense():
 
 
 @pytest.mark.timeout(900)  # the first import of torch on a fresh machine is slow
-def test_autograd_wrappers():
+@pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_RCT, dict(tile_rows=128, slices=2))],
+                         ids=["axt", "rct"])
+def test_autograd_wrappers(fmt, kw):
     torch = pytest.importorskip("torch")
     from gpu_spmv_amd import autograd as ag
     m = ax.random_sparse(500, 2, 9, seed=3)
@@ -117,7 +127,7 @@ def test_autograd_wrappers():
     x = torch.tensor(ax.uniform(41, m.ncols), requires_grad=True)
     b = torch.tensor(ax.uniform(42, m.nrows))
     # spmv: gradients against a dense torch reference
-    y = ag.spmv(vals, x, m, thw=4)
+    y = ag.spmv(vals, x, m, fmt, **kw)
     g = torch.tensor(ax.uniform(43, m.nrows))
     (y * g).sum().backward()
     A = torch.tensor(m.to_dense())
@@ -130,7 +140,7 @@ def test_autograd_wrappers():
     for fused in (True, False):
         vals.grad = None
         x.grad = None
-        loss = ag.least_squares(vals, x, b, m, fused=fused, thw=4)
+        loss = ag.least_squares(vals, x, b, m, fmt, fused=fused, **kw)
         loss.backward()
         r = A @ x.detach() - b
         assert abs(loss.item() - 0.5 * float(r @ r)) <= 1e-12 * (1 + loss.item())
@@ -179,3 +189,10 @@ def test_unsupported_combinations_raise():
         pk.spmm(8, d, d)
     with pytest.raises(ax.AxspmvError):
         pk.vbar(d, d, d)
+    with pytest.raises(ax.AxspmvError):
+        pk.vbar_positions()
+    pr = ax.Plan(m, ax.FMT_RCT)  # no transpose: no entry positions kept
+    with pytest.raises(ax.AxspmvError):
+        pr.vbar_positions()
+    assert np.array_equal(ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_TRANSPOSE).vbar_positions(),
+                          np.arange(m.nnz, dtype=np.uint32))
