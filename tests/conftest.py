@@ -14,6 +14,14 @@ FIXTURES = ["toy12", "rand200", "lap16", "band300", "plaw512"]
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and runs HIP kernels")
+    # A fresh checkout has no built libraries (they are not versioned): build
+    # them once, here, before any test loads them.  A tree that has them
+    # (built beforehand, as on the GPU machine) is used as it is.
+    libs = [os.path.join(ROOT, "gpu_spmv_amd", "libaxspmv.so"), os.path.join(ROOT, "oracle", "liboracle.so")]
+    if not all(os.path.exists(p) for p in libs):
+        import subprocess
+        subprocess.run(["make", "-j8", "-C", os.path.join(ROOT, "gpu_spmv_amd", "csrc")], check=True)
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
 
 
 class FixtureMatrix:
