@@ -42,6 +42,7 @@ FLAGSHIP_AXT = ("axt_unc_h1_thw4", ax.FMT_AXT, dict(thw=4, th=1))
 OTHERS = [
     FLAGSHIP_AXT,
     ("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+    ("csr_vector", ax.FMT_CSR, dict(kernel=ax.KER_STREAM)),
     ("axt_unc_h1_thw8", ax.FMT_AXT, dict(thw=8, th=1)),
     ("axt_unc_h1_thw16", ax.FMT_AXT, dict(thw=16, th=1)),
     ("axt_unc_h4_thw64", ax.FMT_AXT, dict(thw=64, th=4)),
@@ -267,6 +268,7 @@ def suite_full(res):
     d1, e1 = ax.DeviceBuffer.from_array(x1), ax.DeviceBuffer(8 * m1.nrows)
     c1 = []
     for name, fmt, kw in (("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+                          ("csr_vector", ax.FMT_CSR, dict(kernel=ax.KER_STREAM)),
                           ("axt_unc_h1_thw4", ax.FMT_AXT, dict(thw=4)), ("axt_unc_h4_thw64", ax.FMT_AXT, dict(thw=64, th=4)),
                           ("axc_hbs4", ax.FMT_AXC, dict(hbs=4)), ("k1_c64", ax.FMT_K1, dict(chunk=64)),
                           ("axt_com_h4_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)),
@@ -281,7 +283,7 @@ def suite_full(res):
     y3 = orc.ncsr(m3, x3, nthreads=omp_threads())
     d3, e3 = ax.DeviceBuffer.from_array(x3), ax.DeviceBuffer(8 * m3.nrows)
     c3 = []
-    for name, fmt, kw in [FLAGSHIP, FLAGSHIP_AXT] + OTHERS[1:13]:
+    for name, fmt, kw in [FLAGSHIP, FLAGSHIP_AXT] + OTHERS[1:14]:
         _, _, r = measure(m3, name, fmt, kw, d3, e3, 20, 3, y3)
         c3.append(r)
         log(f"stencil128p: {r}")

@@ -1,5 +1,8 @@
-// CSR, one lane per row: the reference's gcsr (cudaSpmv.cu:531-543) on
-// wave64.  y[r] = sum of val*x[col] over the row, left to right from 0.
+// CSR kernels.  csr_naive: one lane per row, the reference's gcsr
+// (cudaSpmv.cu:531-543) on wave64, y[r] = sum of val*x[col] left to right.
+// csr_vector: a group of G lanes per row (G divides 64), strided over the
+// row, then a DPP group sum; rows longer than long_len entries are listed on
+// the host and done one wave per row by csr_long.
 #include "launch.hpp"
 #include "wave64.hpp"
 
@@ -16,6 +19,58 @@ __global__ void __launch_bounds__(256) csr_naive_kernel(uint32_t nrows, const VT
     const uint32_t e = row[r + 1];
     for (uint32_t j = row[r]; j < e; ++j) acc += double(val[j]) * x[col[j]];
     y[r] = acc;
+}
+
+template <class VT, int G>
+__global__ void __launch_bounds__(256) csr_vector_kernel(uint32_t nrows, const VT *__restrict__ val,
+                                                         const uint32_t *__restrict__ col,
+                                                         const uint32_t *__restrict__ row, uint32_t long_len,
+                                                         const double *__restrict__ x, double *__restrict__ y) {
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r = gl / G, g = gl % G;
+    if (r >= nrows) return;  // whole groups leave together (G divides 64)
+    const uint32_t b = row[r], e = row[r + 1];
+    if (e - b > long_len) return;  // csr_long's row
+    double acc = 0.0;
+    for (uint32_t j = b + g; j < e; j += G) acc = fma(double(val[j]), x[col[j]], acc);
+    acc = axspmv::dev::group_sum_f64<G>(acc);
+    if (g == 0) y[r] = acc;
+}
+
+template <class VT>
+__global__ void __launch_bounds__(256) csr_long_kernel(const uint32_t *__restrict__ rows, uint32_t nlong,
+                                                       const VT *__restrict__ val, const uint32_t *__restrict__ col,
+                                                       const uint32_t *__restrict__ row, const double *__restrict__ x,
+                                                       double *__restrict__ y) {
+    const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (w >= nlong) return;
+    const uint32_t r = rows[w];
+    double acc = 0.0;
+    for (uint32_t j = row[r] + lane; j < row[r + 1]; j += 64) acc = fma(double(val[j]), x[col[j]], acc);
+    acc = axspmv::dev::group_sum_f64<64>(acc);
+    if (lane == 0) y[r] = acc;
+}
+
+template <class VT>
+hipError_t csr_vector_vt(uint32_t nrows, const VT *val, const uint32_t *col, const uint32_t *row, uint32_t lanes,
+                         uint32_t long_len, const uint32_t *longrows, uint32_t nlong, const double *x, double *y,
+                         hipStream_t s) {
+    const unsigned grid = unsigned((uint64_t(nrows) * lanes + 255) / 256);
+    switch (lanes) {
+#define AX_G_CASE(G) \
+    case G: csr_vector_kernel<VT, G><<<grid, 256, 0, s>>>(nrows, val, col, row, long_len, x, y); break;
+        AX_G_CASE(1)
+        AX_G_CASE(2)
+        AX_G_CASE(4)
+        AX_G_CASE(8)
+        AX_G_CASE(16)
+        AX_G_CASE(32)
+        AX_G_CASE(64)
+#undef AX_G_CASE
+        default: return hipErrorInvalidValue;
+    }
+    if (nlong) csr_long_kernel<VT><<<(nlong + 3) / 4, 256, 0, s>>>(longrows, nlong, val, col, row, x, y);
+    return hipGetLastError();
 }
 
 // v_bar[k] = y_bar[row(k)] * x[col(k)], one lane per row.
@@ -77,6 +132,16 @@ hipError_t csr_fused(uint32_t nrows, const void *val, uint32_t vtype, const uint
 hipError_t residual(uint32_t n, const double *y, const double *b, double *r, hipStream_t s) {
     if (n) residual_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, y, b, r);
     return hipGetLastError();
+}
+
+hipError_t csr_vector(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,
+                      uint32_t lanes, uint32_t long_len, const uint32_t *longrows, uint32_t nlong, const double *x,
+                      double *y, hipStream_t s) {
+    if (!nrows) return hipSuccess;
+    if (vtype == 1)
+        return csr_vector_vt(nrows, static_cast<const float *>(val), col, row, lanes, long_len, longrows, nlong, x, y,
+                             s);
+    return csr_vector_vt(nrows, static_cast<const double *>(val), col, row, lanes, long_len, longrows, nlong, x, y, s);
 }
 
 hipError_t csr_naive(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,

@@ -115,6 +115,11 @@ hipError_t batch(const BatchArgs &a, const double *x, double *y, hipStream_t s);
 
 hipError_t csr_naive(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,
                      const double *x, double *y, uint32_t block, hipStream_t s);
+// lanes (1..64, a power of two) per row; rows of more than long_len entries
+// are listed in longrows and done one wave per row
+hipError_t csr_vector(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,
+                      uint32_t lanes, uint32_t long_len, const uint32_t *longrows, uint32_t nlong, const double *x,
+                      double *y, hipStream_t s);
 
 // K1: a work item is steps [j0, j1) of one chunk; slot == 0xffffffff writes
 // y directly, otherwise partial sums go to part[slot*chunk + r] and the

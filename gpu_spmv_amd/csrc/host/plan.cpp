@@ -162,7 +162,22 @@ int plan_csr(Plan &p, const axspmv_csr *a, hipStream_t s) {
     AX_TRY(p.upload(p.d_row, a->row, 4ull * (a->nrows + 1), s));
     p.slots = a->nnz;
     p.stream_bytes = (vs + 4) * a->nnz + 4ull * (a->nrows + 1);
-    if (p.o.kernel == AXSPMV_KER_STREAM) return fail(AXSPMV_E_UNSUPPORTED, "CSR stream kernel not built");
+    if (p.o.kernel == AXSPMV_KER_STREAM) {
+        // lanes per row: opts.hbs, or the power of two nearest the mean row
+        // length within [2, 64]
+        uint32_t g = p.o.hbs;
+        if (g == 0) {
+            const double mean = a->nrows ? double(a->nnz) / a->nrows : 1.0;
+            g = 2;
+            while (g < 64 && double(g) * 1.5 < mean) g *= 2;
+        }
+        if (!is_pow2(g) || g > 64) return fail(AXSPMV_E_UNSUPPORTED, "CSR vector: lanes per row %u not a power of two <= 64", g);
+        p.lanes = g;
+        p.kind = Plan::CSR_VECTOR;
+        std::snprintf(p.name, sizeof p.name, "csr_vector_g%u", g);
+        p.launches = 2;
+        return setup_rowwise(p, a, p.d_row, p.d_val, p.d_col, s);
+    }
     p.kind = Plan::CSR_NAIVE;
     std::snprintf(p.name, sizeof p.name, "csr_naive");
     return setup_rowwise(p, a, p.d_row, p.d_val, p.d_col, s);
@@ -729,6 +744,9 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
     switch (p->kind) {
         case Plan::CSR_NAIVE:
             return launch::csr_naive(p->nrows, p->d_val, p->o.vtype, p->d_col, p->d_row, x, y, p->o.block, s);
+        case Plan::CSR_VECTOR:
+            return launch::csr_vector(p->nrows, p->d_val, p->o.vtype, p->d_col, p->d_row, p->lanes, p->spmm.long_slots,
+                                      p->spmm.longrows, p->spmm.nlong, x, y, s);
         case Plan::AXT_UNC_H1: return launch::axt_unc_h1(seg_args(*p, p->d_sec), x, y, s);
         case Plan::AXT_UNC_H: return launch::axt_unc_h(seg_args(*p, p->d_sec), x, y, s);
         case Plan::AXT_COM: return launch::axt_com(p->com, x, y, s);

@@ -11,7 +11,7 @@
 #include "common.hpp"
 
 struct axspmv_plan {
-    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC, RCT, AXT_UNC_H1_AX };
+    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC, RCT, AXT_UNC_H1_AX, CSR_VECTOR };
     Kind kind = NONE;
     int device = 0;
     axspmv_opts o{};
@@ -21,6 +21,7 @@ struct axspmv_plan {
     double beta = 0.0;
     uint32_t launches = 1, nblocks = 0, nempty = 0, staged_blocks = 0;
     uint32_t thw = 0, th = 0;
+    uint32_t lanes = 0;  // CSR vector: lanes per row
 
     // CSR arrays
     void *d_val = nullptr;

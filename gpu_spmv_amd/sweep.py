@@ -39,6 +39,7 @@ BLOCKS = (32, 64, 128, 256, 512, 1024)
 # run once per block size, the others once (their kernels fix the workgroup).
 CANDIDATES = [
     ("csr_naive", L.FMT_CSR, dict(kernel=L.KER_NAIVE), True),
+    ("csr_vector", L.FMT_CSR, dict(kernel=L.KER_STREAM), False),
     ("k1_c64", L.FMT_K1, dict(chunk=64), False),
     ("axc_hbs4", L.FMT_AXC, dict(hbs=4), False),
     ("axc_hbs32", L.FMT_AXC, dict(hbs=32), False),

@@ -212,7 +212,9 @@ enum axspmv_format {
 enum axspmv_kernel {
     AXSPMV_KER_DEFAULT = 0, /* the format's own kernel                  */
     AXSPMV_KER_NAIVE = 1,   /* CSR: one lane per row (gcsr)              */
-    AXSPMV_KER_STREAM = 2   /* CSR: row-aligned wave blocks, segmented   */
+    AXSPMV_KER_STREAM = 2   /* CSR: a group of hbs lanes per row (CSR
+                               vector; hbs = 0 picks it from the mean row
+                               length), rows over 256 entries one wave each */
 };
 enum axspmv_vtype { AXSPMV_F64 = 0, AXSPMV_F32 = 1 };
 

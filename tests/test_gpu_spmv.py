@@ -19,6 +19,10 @@ TOL = 1e-12
 
 KERNELS = [
     ("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+    ("csr_vector_auto", ax.FMT_CSR, dict(kernel=ax.KER_STREAM)),
+    ("csr_vector_g1", ax.FMT_CSR, dict(kernel=ax.KER_STREAM, hbs=1)),
+    ("csr_vector_g8", ax.FMT_CSR, dict(kernel=ax.KER_STREAM, hbs=8)),
+    ("csr_vector_g64", ax.FMT_CSR, dict(kernel=ax.KER_STREAM, hbs=64)),
     ("unc_h1_thw4", ax.FMT_AXT, dict(thw=4, th=1)),
     ("unc_h1_thw8", ax.FMT_AXT, dict(thw=8, th=1)),
     ("unc_h1_thw16", ax.FMT_AXT, dict(thw=16, th=1)),
@@ -114,7 +118,7 @@ def test_spmv_f64(mname, kname, fmt, kw):
         assert np.array_equal(y, y2)
 
 
-F32_KERNELS = [k for k in KERNELS if k[0] in ("csr_naive", "unc_h1_thw4", "unc_h4_thw64", "k1_c64", "axc_h8",
+F32_KERNELS = [k for k in KERNELS if k[0] in ("csr_naive", "csr_vector_auto", "unc_h1_thw4", "unc_h4_thw64", "k1_c64", "axc_h8",
                                                 "com_h1_thw64_bs256", "com_h8_thw64", "rct_r1024_s4",
                                                 "rct_r2048_s4_split")]
 
@@ -178,6 +182,7 @@ def test_lds_staging_is_exact(mname, kw):
 
 
 EDGE_PLANS = [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_AXT, dict(thw=64, th=4)), (ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+              (ax.FMT_CSR, dict(kernel=ax.KER_STREAM)), (ax.FMT_CSR, dict(kernel=ax.KER_STREAM, hbs=64)),
               (ax.FMT_K1, dict(chunk=64)), (ax.FMT_AXC, dict(hbs=4)), (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1)),
               (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)), (ax.FMT_RCT, dict()),
               (ax.FMT_RCT, dict(slices=4, tile_rows=64)), (ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_AX_LAYOUT)),
