@@ -452,6 +452,17 @@ int plan_k1(Plan &p, const axspmv_csr *a, hipStream_t s) {
 int plan_axc(Plan &p, const axspmv_csr *a, hipStream_t s) {
     const uint32_t G = p.o.hbs;
     if (!is_pow2(G) || G > 64) return fail(AXSPMV_E_UNSUPPORTED, "AXC kernel needs hbs a power of two <= 64, got %u", G);
+    if (p.o.flags & AXSPMV_PLAN_AX_LAYOUT) {
+        // AXC's ax (bricks of hbs values then hbs x values from brp[r],
+        // get_axAXC cudaSpmv.cu:952-979) is the AXT UNC th = 1 layout with
+        // thw = hbs, slot for slot: the same plan serves it.
+        p.o.thw = G;
+        p.o.th = 1;
+        p.o.mode = AXSPMV_AXT_UNC;
+        AX_TRY(plan_axt(p, a, s));
+        std::snprintf(p.name, sizeof p.name, "axc_hbs%u_ax", G);
+        return AXSPMV_OK;
+    }
     axspmv_axc *m = nullptr;
     AX_TRY(axspmv_build_axc(a, nullptr, G, AXSPMV_BUILD_PLANES, &m, nullptr));
     std::unique_ptr<axspmv_axc, void (*)(axspmv_axc *)> guard(m, axspmv_axc_free);

@@ -252,11 +252,13 @@ typedef struct axspmv_opts {
                                     there.  Off by default: on gfx950 the
                                     vector L1 already serves these gathers and
                                     the staging step measured slower. */
-#define AXSPMV_PLAN_AX_LAYOUT 4u /* AXT UNC th == 1, FP64 values: keep the
-                                    reference's interleaved ax array on the
+#define AXSPMV_PLAN_AX_LAYOUT 4u /* AXT UNC th == 1 or AXC, FP64 values: keep
+                                    the reference's interleaved ax array on the
                                     device (tiles of thw values then thw
                                     x-values, getArraysAxSecAXT_UNC_H1,
-                                    cudaSpmv.cu:1239-1271).  Every SpMV first
+                                    cudaSpmv.cu:1239-1271; AXC's bricks,
+                                    get_axAXC 952-979, are that layout with
+                                    thw = hbs).  Every SpMV first
                                     refreshes the x-halves from x
                                     (axspmv_x_refresh), then streams ax like
                                     gaxtuh1hw*w (cudaSpmv.cu:1570-1850). */

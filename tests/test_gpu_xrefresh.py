@@ -102,7 +102,8 @@ def test_refresh_rejects_bad_width():
         ax.x_refresh(d, 4, 3, 1, d, d)
 
 
-AX_PLANS = [dict(thw=4), dict(thw=8), dict(thw=16), dict(thw=64), dict(thw=2), dict(thw=4, budget=64)]
+AX_PLANS = [dict(thw=4), dict(thw=8), dict(thw=16), dict(thw=64), dict(thw=2), dict(thw=4, budget=64),
+            dict(fmt=ax.FMT_AXC, hbs=4), dict(fmt=ax.FMT_AXC, hbs=32)]
 
 
 @pytest.mark.parametrize("mname", list(MATS) + ["long_rows"])
@@ -113,7 +114,8 @@ def test_ax_layout_spmv(mname, kw):
         m = long_rows()
     else:
         m = MATS[mname]
-    p = ax.Plan(m, ax.FMT_AXT, flags=ax.PLAN_AX_LAYOUT, **kw)
+    kw = dict(kw)
+    p = ax.Plan(m, kw.pop("fmt", ax.FMT_AXT), flags=ax.PLAN_AX_LAYOUT, **kw)
     info = p.info()
     assert info["name"].endswith("_ax") and info["launches"] == 2
     dy = ax.DeviceBuffer(8 * max(m.nrows, 1))
