@@ -153,11 +153,15 @@ def extras_same_matrix(m, x, dx, yref, plain_us):
         dY = ax.DeviceBuffer(8 * n * k)
         rows = np.arange(0, n, max(1, n // 2000))
         for vt in (ax.F64, ax.F32):
-            best, times = ax.autotune(m, op=L.OP_SPMM, k=k,
-                                      candidates=[ax.opts(ax.FMT_AXT, thw=4, vtype=vt, ktile=kt)
-                                                  for kt in (8, 16, 32, 64)], iters=5)
-            kt = (8, 16, 32, 64)[best]
-            pm = ax.Plan(m, ax.FMT_AXT, thw=4, vtype=vt, ktile=kt)
+            # the library's SpMM candidates (AXT k-tiles, RCT tiles), in this value type
+            cands = ax.autotune_defaults(L.OP_SPMM)
+            for o in cands:
+                o.vtype = vt
+            best, times = ax.autotune(m, op=L.OP_SPMM, k=k, candidates=cands, iters=5)
+            c = cands[best]
+            pm = ax.Plan(m, c.format, **{f: getattr(c, f) for f in ("vtype", "ktile", "thw", "th", "tile_rows",
+                                                                     "slices")})
+            kt = c.ktile
             t = pm.time_op(L.OP_SPMM, iters=10, warmup=2, k=k, X=dX, Y=dY)
             Y = dY.download(np.float64, n * k).reshape(n, k)
             # accuracy on sampled rows against the FP64 oracle
@@ -165,7 +169,8 @@ def extras_same_matrix(m, x, dx, yref, plain_us):
             err = rel_err(Yref, Y[rows]) if Yref is not None else None
             alg = (12 if vt == ax.F64 else 8) * nnz + 4 * (n + 1) + 16 * n * k
             us = t.mean_ms * 1e3
-            sp.append(dict(k=k, vtype="f64" if vt == ax.F64 else "f32val", ktile=kt, us=round(us, 1),
+            sp.append(dict(k=k, vtype="f64" if vt == ax.F64 else "f32val", kernel=pm.info()["name"], ktile=kt,
+                           us=round(us, 1),
                            gflops=round(2.0 * nnz * k / (us * 1e-6) / 1e9, 1),
                            alg_gbs=round(alg / (us * 1e-6) / 1e9, 1),
                            per_rhs_vs_plain=round(us / k / plain_us, 3), rel_err_vs_f64_oracle=err))

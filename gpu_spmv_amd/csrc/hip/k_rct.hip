@@ -149,8 +149,8 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
 
 // y[0 .. n) = 0 with 16-byte stores (y 16-byte aligned; VEC = false: 8-byte)
 template <bool VEC>
-__global__ void __launch_bounds__(256) rct_clear(double *__restrict__ y, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) rct_clear(double *__restrict__ y, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if constexpr (VEC) {
         if (2 * i + 1 < n)
             reinterpret_cast<double2 *>(y)[i] = double2{0.0, 0.0};
@@ -195,10 +195,96 @@ __global__ void __launch_bounds__(1024) rct_vbar_kernel(RctArgs a, const double 
     }
 }
 
+// SpMM: Y[:, k0 .. k0 + KT) (+)= A X[:, k0 .. k0 + KT) over one tile.  KT
+// lanes serve one entry (one coalesced KT x 8-byte gather of the entry's X
+// row); the tile's rows x KT sums stay in LDS (ds_add_f64) and go to Y at
+// the end: stored with one slice, added with FP64 atomics (Y cleared first)
+// with several.
+template <class VT, bool PACKED, int KT>
+__global__ void __launch_bounds__(1024) rct_spmm_kernel(RctArgs a, uint32_t k, uint32_t k0,
+                                                        const double *__restrict__ X, double *__restrict__ Y) {
+    extern __shared__ double acc[];
+    TileOf tile;
+    if (!tile.from(a, blockIdx.x)) return;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t row0 = a.rb_start[tile.rb], rows = a.rb_start[tile.rb + 1] - row0;
+    for (uint32_t q = tid; q < rows * KT; q += nt) acc[q] = 0.0;
+    __syncthreads();
+    const uint32_t t = tile.rb * a.S + tile.s;
+    const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
+    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.rbits, tile.s * a.C};
+    const uint32_t j = tid % KT, nsub = nt / KT;  // nt is a multiple of 64
+    const bool live = k0 + j < k;                 // the last k-tile may be partial
+    const double *__restrict__ xj = X + k0 + j;
+    constexpr int U = 4;
+    uint32_t e = beg + tid / KT;
+    for (; e + (U - 1) * nsub < end; e += U * nsub) {
+        uint32_t c[U], r[U];
+        double v[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) ent.load(e + u * nsub, c[u], r[u], v[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = live ? xj[size_t(c[u]) * k] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) atomicAdd(&acc[r[u] * KT + j], v[u] * xv[u]);
+    }
+    for (; e < end; e += nsub) {
+        uint32_t c, r;
+        double v;
+        ent.load(e, c, r, v);
+        const double xv = live ? xj[size_t(c) * k] : 0.0;
+        atomicAdd(&acc[r * KT + j], v * xv);
+    }
+    __syncthreads();
+    for (uint32_t q = tid; q < rows * KT; q += nt) {
+        const uint32_t rr = q / KT, jj = q % KT;
+        if (k0 + jj >= k) continue;
+        double *out = Y + size_t(row0 + rr) * k + k0 + jj;
+        if (a.S == 1)
+            *out = acc[q];
+        else
+            unsafeAtomicAdd(out, acc[q]);
+    }
+}
+
+template <class VT, bool PACKED>
+hipError_t rct_spmm_vt(const RctArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s) {
+    const size_t lds = size_t(a.R) * kt * sizeof(double);
+    const uint32_t per = 8 / a.S;
+    const unsigned grid = (a.nrb + per - 1) / per * 8;
+    for (uint32_t k0 = 0; k0 < k; k0 += kt) {
+        switch (kt) {
+            case 8: rct_spmm_kernel<VT, PACKED, 8><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
+            case 16: rct_spmm_kernel<VT, PACKED, 16><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
+            case 32: rct_spmm_kernel<VT, PACKED, 32><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
+            case 64: rct_spmm_kernel<VT, PACKED, 64><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
+            default: return hipErrorInvalidValue;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 }  // namespace
 
 namespace axspmv {
 namespace launch {
+
+hipError_t rct_spmm(const RctArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s) {
+    if (!a.nrows || !k) return hipSuccess;
+    if (a.S > 1) {
+        const size_t n = size_t(a.nrows) * k;
+        if (reinterpret_cast<uintptr_t>(Y) % 16 == 0)
+            rct_clear<true><<<unsigned((n / 2 + 256) / 256), 256, 0, s>>>(Y, n);
+        else
+            rct_clear<false><<<unsigned((n + 255) / 256), 256, 0, s>>>(Y, n);
+    }
+    const bool packed = a.key != nullptr;
+    if (a.vtype == 1)
+        return packed ? rct_spmm_vt<float, true>(a, k, kt, X, Y, s) : rct_spmm_vt<float, false>(a, k, kt, X, Y, s);
+    return packed ? rct_spmm_vt<double, true>(a, k, kt, X, Y, s) : rct_spmm_vt<double, false>(a, k, kt, X, Y, s);
+}
 
 hipError_t rct_vbar(const RctArgs &a, const double *ybar, const double *x, double *vbar, hipStream_t s) {
     if (!a.nrows) return hipSuccess;

@@ -86,6 +86,9 @@ struct RctArgs {
 // S > 1: a fill kernel clears y, then the tile kernel adds into it; mid
 // (optional) is recorded on s between the two
 hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid = nullptr);
+// Y = A X (k interleaved right-hand sides) on a one-slice plan, kt (8, 16,
+// 32 or 64) right-hand sides per launch; needs R * kt * 8 bytes of LDS
+hipError_t rct_spmm(const RctArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s);
 // vbar[e] = ybar[row(e)] * x[col(e)] for every entry e, in entry (tile) order
 hipError_t rct_vbar(const RctArgs &a, const double *ybar, const double *x, double *vbar, hipStream_t s);
 

@@ -211,7 +211,9 @@ extern "C" int axspmv_batch_time(axspmv_batch *b, const double *d_x, double *d_y
 namespace {
 
 // Built-in candidates.  Batches take AXT UNC only (the persistent batch
-// kernel runs that layout); single plans also try K1 and RCT.
+// kernel runs that layout); single plans also try K1 and RCT.  SpMM: AXT
+// k-tiles, and RCT tiles whose rows x k-tile sums fill the LDS (one column
+// slice and the default slicing).
 std::vector<axspmv_opts> default_candidates(uint32_t op, bool for_batch = false) {
     std::vector<axspmv_opts> c;
     axspmv_opts o;
@@ -220,6 +222,20 @@ std::vector<axspmv_opts> default_candidates(uint32_t op, bool for_batch = false)
         for (uint32_t kt : {8u, 16u, 32u, 64u}) {
             o.ktile = kt;
             c.push_back(o);
+        }
+        axspmv_opts_default(AXSPMV_FMT_RCT, &o);
+        for (uint32_t kt : {8u, 16u, 32u}) {
+            o.ktile = kt;
+            o.tile_rows = 8192 / kt;  // 64 KB of sums (ktile 8: also 2048 rows)
+            for (uint32_t sl : {1u, 0u}) {
+                o.slices = sl;
+                c.push_back(o);
+                if (kt == 8) {
+                    axspmv_opts w = o;
+                    w.tile_rows = 2048;
+                    c.push_back(w);
+                }
+            }
         }
         return c;
     }

@@ -518,6 +518,23 @@ std::vector<uint32_t> rct_row_blocks(const axspmv_csr *a, uint32_t nrb, uint32_t
     return rbs;
 }
 
+// True when the rows, cut into blocks of `rows`, read narrow column bands:
+// at most 1/8 of the nonzeros lie in blocks whose column span (max - min + 1)
+// exceeds `cols`.  Such a matrix (banded, a stencil in natural order) keeps
+// each block's x within one L2 already, and column slices would only leave
+// most tiles empty.
+bool narrow_bands(const axspmv_csr *a, uint32_t rows, uint64_t cols) {
+    uint64_t wide = 0;
+    for (uint64_t r0 = 0; r0 < a->nrows; r0 += rows) {
+        const uint32_t r1 = uint32_t(std::min<uint64_t>(a->nrows, r0 + rows));
+        const uint32_t b = a->row[r0], e = a->row[r1];
+        if (b == e) continue;
+        const auto mm = std::minmax_element(a->col + b, a->col + e);
+        if (uint64_t(*mm.second) - *mm.first + 1 > cols) wide += e - b;
+    }
+    return wide * 8 <= a->nnz;
+}
+
 // RCT: row blocks of at most tile_rows rows and about equal nonzeros, column
 // slices, entries of a tile sorted by (column, row).
 int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
@@ -526,20 +543,25 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     if (hipGetDeviceProperties(&prop, p.device) == hipSuccess && prop.sharedMemPerBlock > 0)
         lds_max = prop.sharedMemPerBlock;
     const uint32_t lds_rows = uint32_t(std::min<size_t>(65536, lds_max / 8));
-    uint32_t S = p.o.slices;
-    if (S == 0) {
-        S = 1;
-        while (S < 8 && uint64_t(a->ncols) * 8 / S > (2ull << 20)) S *= 2;
-    }
-    if (S != 1 && S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_INVALID, "RCT: slices must be 1, 2, 4 or 8");
+    p.lds_max = lds_max;
     const uint32_t n = a->nrows;
     // Default tile height: as tall as the LDS allows (fewer row blocks, fewer
     // passes over x), but short enough that the tiles cover the 256 CUs.
-    uint32_t Rt = p.o.tile_rows;
-    if (Rt == 0) {
-        const uint64_t fill = ceil_div(ceil_div(uint64_t(n) * S, 256), 64) * 64;
-        Rt = uint32_t(std::min<uint64_t>(std::min<uint32_t>(16384, lds_rows), std::max<uint64_t>(64, fill)));
+    auto auto_rows = [&](uint32_t slices) {
+        const uint64_t fill = ceil_div(ceil_div(uint64_t(n) * slices, 256), 64) * 64;
+        return uint32_t(std::min<uint64_t>(std::min<uint32_t>(16384, lds_rows), std::max<uint64_t>(64, fill)));
+    };
+    // Default slicing: x slices of at most 2 MB (one XCD's L2 share), unless
+    // the row blocks read narrow column bands anyway.
+    constexpr uint64_t slice_cols = (2ull << 20) / 8;
+    uint32_t S = p.o.slices;
+    if (S == 0) {
+        S = 1;
+        if (a->ncols > slice_cols && !narrow_bands(a, p.o.tile_rows ? p.o.tile_rows : auto_rows(1), slice_cols))
+            while (S < 8 && ceil_div(a->ncols, S) > slice_cols) S *= 2;
     }
+    if (S != 1 && S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_INVALID, "RCT: slices must be 1, 2, 4 or 8");
+    uint32_t Rt = p.o.tile_rows ? p.o.tile_rows : auto_rows(S);
     if (Rt == 0 || Rt > lds_rows)
         return fail(AXSPMV_E_UNSUPPORTED, "RCT: %u rows per tile do not fit %zu bytes of LDS", Rt, lds_max);
     const uint32_t nrb = uint32_t(ceil_div(n, Rt));
@@ -682,7 +704,7 @@ extern "C" int axspmv_opts_default(uint32_t format, axspmv_opts *o) {
     o->chunk = 64;
     o->reorder = 1;
     o->vtype = AXSPMV_F64;
-    o->block = 256;
+    o->block = format == AXSPMV_FMT_RCT ? 0 : 256;  // RCT: 512 for SpMV, 1024 for SpMM
     o->budget = 0;
     if (format > AXSPMV_FMT_RCT) return fail(AXSPMV_E_INVALID, "opts_default: unknown format %u", format);
     return AXSPMV_OK;
@@ -821,10 +843,14 @@ hipError_t launch_fwd_adj(Plan *p, const double *x, const double *b, double *y, 
     return e;
 }
 
+// RCT tiles hold R x ktile sums in LDS: the default k-tile is also cut to
+// what fits.
 uint32_t pick_ktile(const Plan *p, uint32_t k) {
     if (p->o.ktile) return p->o.ktile;
     uint32_t kt = 8;
     while (kt < 64 && kt < k) kt *= 2;
+    if (p->kind == Plan::RCT)
+        while (kt > 8 && uint64_t(p->rct.R) * kt * 8 > p->lds_max) kt /= 2;
     return kt;
 }
 
@@ -856,7 +882,15 @@ hipError_t plan_op(Plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s
         case AXSPMV_OP_VBAR: return launch_vbar(p, o.ybar, o.x, o.vbar, s);
         case AXSPMV_OP_FWD_ADJ: return launch_fwd_adj(p, o.x, o.b, o.y, o.r, o.xbar, o.vbar, true, s);
         case AXSPMV_OP_FWD_ADJ_SPLIT: return launch_fwd_adj(p, o.x, o.b, o.y, o.r, o.xbar, o.vbar, false, s);
-        case AXSPMV_OP_SPMM: return launch::spmm(p->spmm, o.k, pick_ktile(p, o.k), o.X, o.Y, s);
+        case AXSPMV_OP_SPMM:
+            if (p->kind == Plan::RCT) {
+                // 1024-lane workgroups unless the plan sets the block: a tile
+                // has ktile times the SpMV's LDS adds to hide
+                launch::RctArgs a = p->rct;
+                if (!p->o.block) a.block = 1024;
+                return launch::rct_spmm(a, o.k, pick_ktile(p, o.k), o.X, o.Y, s);
+            }
+            return launch::spmm(p->spmm, o.k, pick_ktile(p, o.k), o.X, o.Y, s);
         case AXSPMV_OP_XREFRESH:
             return launch::x_refresh(p->d_axmap, p->slots, ilog2(p->thw), p->ncols, o.x, p->d_ax, s);
         default: return hipErrorInvalidValue;
@@ -870,10 +904,17 @@ int plan_op_check(const Plan *p, uint32_t op, const axspmv_operands &o) {
     if ((op == AXSPMV_OP_VBAR || op == AXSPMV_OP_FWD_ADJ || op == AXSPMV_OP_FWD_ADJ_SPLIT) && !adjoint_kind(p))
         return fail(AXSPMV_E_UNSUPPORTED, "value gradient: plan %s is not CSR naive, AXT UNC th=1 or RCT", p->name);
     if (op == AXSPMV_OP_SPMM) {
-        if (!p->rowwise) return fail(AXSPMV_E_UNSUPPORTED, "SpMM: plan %s is not CSR, AXC or AXT UNC th=1", p->name);
+        if (!p->rowwise && p->kind != Plan::RCT)
+            return fail(AXSPMV_E_UNSUPPORTED, "SpMM: plan %s is not CSR, AXC, AXT UNC th=1 or RCT", p->name);
         if (o.k == 0) return fail(AXSPMV_E_INVALID, "SpMM: k must be > 0");
         const uint32_t kt = pick_ktile(p, o.k);
         if (kt != 8 && kt != 16 && kt != 32 && kt != 64) return fail(AXSPMV_E_INVALID, "SpMM: ktile %u not in 8/16/32/64", kt);
+        if (p->kind == Plan::RCT && uint64_t(p->rct.R) * kt * 8 > p->lds_max)
+            return fail(AXSPMV_E_UNSUPPORTED,
+                        "SpMM: RCT tiles of %u rows x ktile %u need %llu bytes of LDS, %zu available (plan with "
+                        "tile_rows <= %zu)",
+                        p->rct.R, kt, (unsigned long long)(uint64_t(p->rct.R) * kt * 8), p->lds_max,
+                        p->lds_max / (8 * size_t(kt)));
         if ((!o.X && p->ncols) || (!o.Y && p->nrows)) return fail(AXSPMV_E_INVALID, "SpMM: NULL operand");
     }
     if (op == AXSPMV_OP_XREFRESH) {

@@ -36,6 +36,7 @@ struct axspmv_plan {
     axspmv::launch::AxcArgs axc{};
     axspmv::launch::ComArgs com{};
     axspmv::launch::RctArgs rct{};
+    size_t lds_max = 0;             // LDS bytes per workgroup (RCT SpMM k-tile limit)
     uint32_t *d_rct_pos = nullptr;  // CSR position of every RCT entry (plans with AXSPMV_PLAN_TRANSPOSE)
     // Row-contiguous layouts (CSR, AXT UNC th == 1, AXC): CSR row pointer and
     // first plane slot of each row, for SpMM and the value gradient.

@@ -229,7 +229,9 @@ typedef struct axspmv_opts {
     uint32_t chunk;   /* K1 rows per chunk (0 = 64) */
     uint32_t reorder; /* K1: 1 = sort rows by length (default), 0 = natural */
     uint32_t vtype;   /* enum axspmv_vtype: value storage */
-    uint32_t block;   /* threads per workgroup (0 = kernel default) */
+    uint32_t block;   /* threads per workgroup (0 = kernel default; the
+                         RCT default options keep 0: 512 for SpMV, 1024
+                         for SpMM) */
     uint32_t budget;  /* units per wave block for segmented kernels (0 = auto) */
     uint32_t flags;   /* AXSPMV_PLAN_* */
     uint32_t ktile;   /* SpMM k-tile: right-hand sides per lane group, 8/16/32/64
@@ -237,7 +239,10 @@ typedef struct axspmv_opts {
     uint32_t tile_rows; /* RCT: at most this many rows per row block (0 =
                            min(16384, LDS / 8)); blocks are cut to about
                            equal nonzeros */
-    uint32_t slices;    /* RCT: column slices 1, 2, 4 or 8 (0 = x slice <= 2 MB) */
+    uint32_t slices;    /* RCT: column slices 1, 2, 4 or 8 (0 = one slice if
+                           the row blocks read column bands of <= 2 MB of x
+                           -- banded matrices -- else the fewest slices with
+                           an x slice <= 2 MB) */
 } axspmv_opts;
 
 #define AXSPMV_PLAN_TRANSPOSE 1u /* also build A^T (needed by the adjoint) */
@@ -321,7 +326,12 @@ int axspmv_fwd_adj(axspmv_plan *p, const double *d_x, const double *d_b, double 
 /* ---------------------------------------------------------------- SpMM */
 /* Y = A X for k right-hand sides, X (ncols x k) and Y (nrows x k) row-major
  * ("interleaved": X[c*k + j]).  Every value/index load feeds ktile FMAs.
- * Plans: CSR, AXC and AXT UNC th == 1.  No reference counterpart. */
+ * Plans: CSR, AXC, AXT UNC th == 1 and RCT (RCT: a tile keeps its rows x
+ * ktile sums in LDS, so tile_rows * ktile * 8 bytes must fit one workgroup's
+ * LDS -- tile_rows <= 2560 at ktile 8 on 160 KB; the default ktile is cut
+ * to fit; with several column slices Y is cleared, then atomically added
+ * to; 1024-lane workgroups unless the plan sets block).  No reference
+ * counterpart. */
 int axspmv_spmm(axspmv_plan *p, uint32_t k, const double *d_X, double *d_Y, void *stream);
 
 /* ---------------------------------------------------------- timing */
@@ -460,8 +470,10 @@ int axspmv_graph_destroy(axspmv_graph *g);
  * reference sweeps block sizes by relaunching (jobLauncher.sh:145-176). */
 /* The built-in candidates: for SpMV AXT UNC th = 1 with thw 2..16 and th =
  * 2..8 with thw 64 at two block budgets, K1 (chunk 64) and RCT (default, 256
- * and 1024 threads); for SpMM the four k-tiles.  Writes min(cap, count)
- * options to out and the set's size to *count. */
+ * and 1024 threads); for SpMM the four AXT k-tiles and eight RCT plans
+ * (ktile 8 with 1024 and 2048 rows, 16 with 512, 32 with 256; one column
+ * slice and the default slicing).  Writes min(cap, count) options to out and
+ * the set's size to *count. */
 int axspmv_autotune_defaults(uint32_t op, axspmv_opts *out, uint32_t cap, uint32_t *count);
 int axspmv_autotune(const axspmv_csr *a, uint32_t op, uint32_t k, const axspmv_opts *cand,
                     uint32_t ncand, uint32_t iters, void *stream, uint32_t *best, double *times_ms);

@@ -154,7 +154,9 @@ SPMM_CASES = [(1, 0), (3, 0), (8, 8), (8, 0), (32, 32), (32, 8), (64, 64), (64, 
 @pytest.mark.parametrize("mname", ["random4096", "long_rows", "banded_empty"])
 @pytest.mark.parametrize("k,kt", SPMM_CASES)
 @pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
-                                    (ax.FMT_AXC, dict(hbs=8))], ids=["axt", "csr", "axc"])
+                                    (ax.FMT_AXC, dict(hbs=8)), (ax.FMT_RCT, dict(tile_rows=256)),
+                                    (ax.FMT_RCT, dict(tile_rows=200, slices=4))],
+                         ids=["axt", "csr", "axc", "rct_r256", "rct_r200_s4"])
 def test_spmm(mname, k, kt, fmt, kw):
     m = MATS[mname]
     X = ax.uniform(50 + k, m.ncols * k).reshape(m.ncols, k)
@@ -167,15 +169,49 @@ def test_spmm(mname, k, kt, fmt, kw):
     assert rel_err(ref, Y) <= TOL
 
 
-def test_spmm_f32_values():
+@pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_RCT, dict(tile_rows=512, slices=2))],
+                         ids=["axt", "rct"])
+def test_spmm_f32_values(fmt, kw):
     m = MATS["random4096"]
     k = 32
     X = ax.uniform(60, m.ncols * k).reshape(m.ncols, k)
     m32 = ax.CsrMatrix(m.row, m.col, m.val.astype(np.float32).astype(np.float64), ncols=m.ncols)
-    p = ax.Plan(m, ax.FMT_AXT, thw=4, vtype=ax.F32)
+    p = ax.Plan(m, fmt, vtype=ax.F32, **kw)
     dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * m.nrows * k)
     p.spmm(k, dX, dY)
     assert rel_err(orc.spmm(m32, X), dY.download(np.float64, m.nrows * k).reshape(m.nrows, k)) <= TOL
+
+
+@pytest.mark.parametrize("k", [8, 5])
+def test_rct_spmm_unaligned_y(k):
+    """Y at an 8-byte offset with column slices (the clear of Y falls back to
+    8-byte stores); the words either side stay untouched."""
+    m = MATS["long_rows"]
+    X = ax.uniform(61, m.ncols * k).reshape(m.ncols, k)
+    p = ax.Plan(m, ax.FMT_RCT, tile_rows=1024, slices=8)
+    n = m.nrows * k
+    dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer.from_array(np.full(n + 2, np.nan))
+    p.spmm(k, dX, dY.ptr + 8)
+    ax._lib.call("axspmv_device_sync")
+    Y = dY.download(np.float64, n + 2)
+    assert np.isnan(Y[0]) and np.isnan(Y[-1])
+    assert rel_err(orc.spmm(m, X), Y[1:-1].reshape(m.nrows, k)) <= TOL
+
+
+def test_rct_spmm_lds_limit():
+    """RCT SpMM keeps tile_rows x ktile sums in LDS: the default k-tile is cut
+    to fit, an explicit one that does not fit is an error."""
+    m = MATS["random4096"]
+    k = 64
+    X = ax.uniform(62, m.ncols * k).reshape(m.ncols, k)
+    dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * m.nrows * k)
+    p = ax.Plan(m, ax.FMT_RCT, tile_rows=2048)  # ktile 64 -> 8 (2048 x 8 x 8 B = 128 KB)
+    p.spmm(k, dX, dY)
+    assert rel_err(orc.spmm(m, X), dY.download(np.float64, m.nrows * k).reshape(m.nrows, k)) <= TOL
+    with pytest.raises(ax.AxspmvError, match="LDS"):
+        ax.Plan(m, ax.FMT_RCT, tile_rows=2048, ktile=16).spmm(k, dX, dY)
+    with pytest.raises(ax.AxspmvError, match="LDS"):
+        ax.Plan(m, ax.FMT_RCT, tile_rows=4096).spmm(k, dX, dY)
 
 
 def test_unsupported_combinations_raise():

@@ -103,6 +103,18 @@ def test_autotune_and_tuned_batch():
     kt_best, kt_times = ax.autotune(m, op=ax._lib.OP_SPMM, k=32,
                                     candidates=[ax.opts(ax.FMT_AXT, thw=4, ktile=k) for k in (8, 16, 32, 64)])
     assert 0 <= kt_best < 4 and (kt_times >= 0).all()
+    # the built-in SpMM set (AXT k-tiles, RCT tiles sized to the LDS): every
+    # candidate runs, and the winner computes the product
+    sd = ax.autotune_defaults(ax._lib.OP_SPMM)
+    assert {o.format for o in sd} == {ax.FMT_AXT, ax.FMT_RCT}
+    s_best, s_times = ax.autotune(m, op=ax._lib.OP_SPMM, k=24)
+    assert s_times.size == len(sd) and (s_times > 0).all() and s_times[s_best] == s_times.min()
+    c = sd[s_best]
+    p = ax.Plan(m, c.format, ktile=c.ktile, thw=c.thw, tile_rows=c.tile_rows, slices=c.slices)
+    X = ax.uniform(81, m.ncols * 24).reshape(m.ncols, 24)
+    dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * m.nrows * 24)
+    p.spmm(24, dX, dY)
+    assert rel_err(orc.spmm(m, X), dY.download(np.float64, m.nrows * 24).reshape(m.nrows, 24)) <= 1e-12
 
 
 def test_batch_graph_replays():

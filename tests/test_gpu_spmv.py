@@ -147,6 +147,25 @@ def test_powerlaw_config2_full_size():
         assert rel_err(ref, y) <= TOL, kw
 
 
+@pytest.mark.parametrize("gen,slices", [
+    (lambda: ax.laplace2d(1024), 1),                          # 2^20 columns, 5-point band
+    (lambda: ax.banded(1 << 19, 64, 0.5, seed=3), 1),         # 2^19 columns, band of 129
+    (lambda: ax.stencil27(96, permute=False), 1),             # natural order: bands of about 2 planes
+    (lambda: ax.stencil27(96, permute=True, seed=2), 4),      # 884,736 random columns
+    (lambda: ax.powerlaw(1 << 19, 2.0, 2048, 20.0, seed=1), 2),
+    (lambda: ax.random_sparse(200000, 1, 30, seed=4), 1),     # x of 1.6 MB fits one slice
+], ids=["laplace1024", "banded", "stencil96", "stencil96p", "powerlaw512k", "random200k"])
+def test_rct_auto_slices(gen, slices):
+    """Default slicing: one slice when the row blocks read narrow column
+    bands or x fits 2 MB, else the fewest slices with an x slice <= 2 MB; the
+    product is the same either way."""
+    m = gen()
+    p = ax.Plan(m, ax.FMT_RCT)
+    assert p.info()["name"].split("_")[2] == f"s{slices}", p.info()["name"]
+    x = ax.uniform(15, m.ncols)
+    assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
+
+
 @pytest.mark.parametrize("kw", [dict(slices=4, tile_rows=512), dict(slices=1, tile_rows=512)])
 def test_rct_unaligned_y(kw):
     """y at an 8-byte offset (the clear of y falls back to 8-byte stores)."""
