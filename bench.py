@@ -310,9 +310,16 @@ def suite_full(res):
     mats = []
     for i in range(256):
         mats.append(ax.banded(int(rng.integers(1000, 200001)), int(rng.integers(1, 65)), 0.3, seed=5000 + i))
+    t0 = time.perf_counter()
     tuned, ngroups = ax.autotune_batch(mats, iters=5)
+    tune_s = round(time.perf_counter() - t0, 1)
     res["batch256"] = {}
-    for label, bt in (("uniform_thw4", ax.Batch(mats, thw=4)), ("autotuned", ax.Batch(mats, opts_list=tuned))):
+    for label, make in (("uniform_thw4", lambda: ax.Batch(mats, thw=4)),
+                        ("autotuned", lambda: ax.Batch(mats, opts_list=tuned)),
+                        ("rct_block_diagonal", lambda: ax.Batch(mats, fmt=ax.FMT_RCT))):
+        t0 = time.perf_counter()
+        bt = make()
+        build_s = round(time.perf_counter() - t0, 2)
         info = bt.info()
         xb = ax.uniform(6, int(bt.xoff[-1]))
         dxb, dyb = ax.DeviceBuffer.from_array(xb), ax.DeviceBuffer(8 * int(bt.yoff[-1]))
@@ -328,9 +335,12 @@ def suite_full(res):
                                       us=round(sec * 1e6, 1), graph_us=round(tg.mean_ms * 1e3, 1),
                                       alg_gbs=round(info["alg_bytes"] / sec / 1e9, 1),
                                       frac=round(info["alg_bytes"] / sec / 1e9 / PEAK_HBM_GBS, 4),
-                                      max_rel_err_sampled=max(errs), tuner_groups=ngroups)
+                                      max_rel_err_sampled=max(errs), build_s=build_s)
+        if label == "autotuned":
+            res["batch256"][label].update(tuner_groups=ngroups, tuner_s=tune_s,
+                                          thw_th_used=sorted({(o.thw, o.th) for o in tuned}))
         log(f"batch256 {label}: {res['batch256'][label]}")
-        del dxb, dyb
+        del dxb, dyb, bt
 
 
 def fwd_adj_rct(m, dx, db, yref, b):

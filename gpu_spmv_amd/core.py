@@ -412,15 +412,17 @@ def spmv_host(plan: Plan, x: np.ndarray) -> np.ndarray:
 
 # ----------------------------------------------------------------- batches
 class Batch:
-    """Many AXT UNC matrices, SpMV in one persistent launch (axspmv_batch).
-    x / y are the concatenations of the matrices' vectors in batch order."""
+    """Many matrices, SpMV in one launch (axspmv_batch): FMT_AXT, one
+    persistent launch over every matrix's AXT UNC layout (per-matrix options
+    allowed); FMT_RCT, the RCT plan of the block-diagonal matrix.  x / y are
+    the concatenations of the matrices' vectors in batch order."""
 
-    def __init__(self, mats, opts_list=None, **kw):
+    def __init__(self, mats, opts_list=None, fmt: int = L.FMT_AXT, **kw):
         self.mats = list(mats)
         n = len(self.mats)
         arr = (C.POINTER(L.Csr) * n)(*[C.pointer(m._s) for m in self.mats])
         if opts_list is None:
-            o = (L.Opts * 1)(opts(L.FMT_AXT, **kw))
+            o = (L.Opts * 1)(opts(fmt, **kw))
             per = 0
         else:
             o = (L.Opts * n)(*opts_list)

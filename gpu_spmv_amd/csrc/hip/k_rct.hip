@@ -33,8 +33,8 @@ namespace {
 using axspmv::launch::RctArgs;
 
 // Entry e of a tile: value val[e]; column and tile row either packed in one
-// word key[e] = (column - slice start) << rbits | row (PACKED) or in col[] /
-// lrow[].
+// word key[e] = (column - tile base) << rbits | row (PACKED; the base is the
+// tile's smallest column) or in col[] / lrow[].
 template <class VT, bool PACKED>
 struct RctEntries {
     const VT *__restrict__ val;
@@ -84,7 +84,8 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     __syncthreads();
     const uint32_t t = rb * a.S + s;  // tile index in storage order
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.rbits, s * a.C};
+    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.rbits,
+                                     PACKED ? a.cbase[t] : 0u};
     // Rounds of U entries per lane, in two register sets A and B: a round
     // issues its x gathers, then the loads of the next round into the other
     // set, then waits for the gathers only and does its LDS adds.  Loads are
@@ -175,7 +176,7 @@ __global__ void __launch_bounds__(1024) rct_vbar_kernel(RctArgs a, const double 
     __syncthreads();
     const uint32_t t = tile.rb * a.S + tile.s;
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const RctEntries<double, PACKED> ent{nullptr, a.key, a.col, a.lrow, a.rbits, tile.s * a.C};
+    const RctEntries<double, PACKED> ent{nullptr, a.key, a.col, a.lrow, a.rbits, PACKED ? a.cbase[t] : 0u};
     constexpr int U = 4;
     uint32_t e = beg + tid;
     for (; e + (U - 1) * nt < end; e += U * nt) {
@@ -212,7 +213,8 @@ __global__ void __launch_bounds__(1024) rct_spmm_kernel(RctArgs a, uint32_t k, u
     __syncthreads();
     const uint32_t t = tile.rb * a.S + tile.s;
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.rbits, tile.s * a.C};
+    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.rbits,
+                                     PACKED ? a.cbase[t] : 0u};
     const uint32_t j = tid % KT, nsub = nt / KT;  // nt is a multiple of 64
     const bool live = k0 + j < k;                 // the last k-tile may be partial
     const double *__restrict__ xj = X + k0 + j;

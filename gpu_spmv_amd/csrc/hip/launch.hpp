@@ -78,10 +78,11 @@ struct RctArgs {
     const void *val;
     const uint32_t *col;    // unpacked: column of each entry
     const uint16_t *lrow;   // unpacked: row within the tile
-    const uint32_t *key;    // packed: (column - slice start) << rbits | row
+    const uint32_t *key;    // packed: (column - tile base) << rbits | row
+    const uint32_t *cbase;  // packed: base column of every tile (its smallest)
     const uint32_t *tile_ptr;
     const uint32_t *rb_start;  // nrb + 1 row-block starts
-    uint32_t nrows, R, S, nrb, vtype, block, rbits, C;
+    uint32_t nrows, R, S, nrb, vtype, block, rbits;
 };
 // S > 1: a fill kernel clears y, then the tile kernel adds into it; mid
 // (optional) is recorded on s between the two

@@ -1,6 +1,7 @@
 // Batched plans (many matrices, one persistent launch) and the autotuner.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -14,6 +15,7 @@ struct axspmv_batch {
     std::vector<void *> bufs;
     launch::BatchArgs args{};
     axspmv_batch_info info{};
+    axspmv_plan *rct = nullptr;  // RCT batches: the plan of the block-diagonal matrix
     void *dalloc(size_t bytes) {
         void *p = nullptr;
         if (bytes == 0) bytes = 256;
@@ -27,6 +29,11 @@ struct axspmv_batch {
     }
     ~axspmv_batch() {
         for (void *p : bufs) (void)hipFree(p);
+        if (rct) axspmv_plan_destroy(rct);
+    }
+    // the batch's SpMV on stream st
+    hipError_t launch_on(const double *x, double *y, hipStream_t st) const {
+        return rct ? plan_launch(rct, x, y, st) : launch::batch(args, x, y, st);
     }
 };
 
@@ -50,12 +57,79 @@ int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
     return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "batch upload");
 }
 
+// Algorithmic bytes of one SpMV of a: values and column indices, row
+// pointer, x and y once.
+uint64_t alg_bytes(const axspmv_csr *a, size_t vs) {
+    return (vs + 4) * uint64_t(a->nnz) + 4ull * (a->nrows + 1) + 8ull * a->ncols + 8ull * a->nrows;
+}
+
+// RCT batch: the block-diagonal matrix diag(A_0, ..., A_{count-1}) -- x and
+// y concatenated in batch order, as for every batch -- as one RCT plan.  Its
+// row blocks balance nonzeros across matrix boundaries, and banded members
+// keep it at one column slice with packed keys.
+int batch_create_rct(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *opts, int per_matrix,
+                     void *stream, axspmv_batch **out) {
+    for (uint32_t i = 1; per_matrix && i < count; ++i)
+        if (std::memcmp(&opts[i], &opts[0], sizeof(axspmv_opts)) != 0)
+            return fail(AXSPMV_E_UNSUPPORTED, "batch_create: an RCT batch takes one option set (matrix %u differs)", i);
+    uint64_t nrows = 0, ncols = 0, nnz = 0, alg = 0;
+    const size_t vs = opts[0].vtype == AXSPMV_F32 ? 4 : 8;
+    for (uint32_t i = 0; i < count; ++i) {
+        const axspmv_csr *a = mats[i];
+        if (!a || !a->row || a->row[a->nrows] != a->nnz) return fail(AXSPMV_E_INVALID, "batch_create: matrix %u inconsistent", i);
+        nrows += a->nrows;
+        ncols += a->ncols;
+        nnz += a->nnz;
+        alg += alg_bytes(a, vs);
+    }
+    if (nrows >= (1ull << 32) || ncols >= (1ull << 32) || nnz >= (1ull << 32))
+        return fail(AXSPMV_E_OVERFLOW, "batch_create: the block-diagonal matrix exceeds 32-bit sizes");
+    std::vector<uint32_t> row(nrows + 1), col(nnz ? nnz : 1);
+    std::vector<double> val(nnz ? nnz : 1);
+    uint64_t r0 = 0, c0 = 0, e0 = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const axspmv_csr *a = mats[i];
+        for (uint32_t r = 0; r < a->nrows; ++r) row[r0 + r] = uint32_t(e0 + a->row[r]);
+        for (uint32_t j = 0; j < a->nnz; ++j) col[e0 + j] = uint32_t(c0 + a->col[j]);
+        if (a->nnz) std::memcpy(val.data() + e0, a->val, 8ull * a->nnz);
+        r0 += a->nrows;
+        c0 += a->ncols;
+        e0 += a->nnz;
+    }
+    row[nrows] = uint32_t(nnz);
+    axspmv_csr bd{};
+    bd.nrows = uint32_t(nrows);
+    bd.ncols = uint32_t(ncols);
+    bd.nnz = uint32_t(nnz);
+    bd.row = row.data();
+    bd.col = col.data();
+    bd.val = val.data();
+    axspmv_opts o = opts[0];
+    o.flags &= ~AXSPMV_PLAN_TRANSPOSE;
+    auto b = std::make_unique<axspmv_batch>();
+    AX_TRY(axspmv_plan_create(&bd, &o, stream, &b->rct));
+    axspmv_plan_info pi;
+    AX_TRY(axspmv_plan_get_info(b->rct, &pi));
+    b->info.nrows = nrows;
+    b->info.ncols = ncols;
+    b->info.nnz = nnz;
+    b->info.slots = pi.slots;
+    b->info.alg_bytes = alg;
+    b->info.device_bytes = pi.device_bytes;
+    b->info.count = count;
+    b->info.nitems = pi.nblocks;  // tiles
+    b->info.nwaves = 0;
+    *out = b.release();
+    return AXSPMV_OK;
+}
+
 }  // namespace
 
 extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *opts,
                                    int per_matrix, void *stream, axspmv_batch **out) {
     if (!mats || !opts || !out || count == 0) return fail(AXSPMV_E_INVALID, "batch_create: bad argument");
     *out = nullptr;
+    if (opts[0].format == AXSPMV_FMT_RCT) return batch_create_rct(mats, count, opts, per_matrix, stream, out);
     hipStream_t s = static_cast<hipStream_t>(stream);
     auto b = std::make_unique<axspmv_batch>();
     const uint32_t vtype = opts[0].vtype;
@@ -68,6 +142,13 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
     std::vector<float> vcat32;
     std::vector<uint32_t> ccat, scat;
     uint64_t xoff = 0, yoff = 0;
+    // A matrix listed more than once (the tuner repeats a group's members)
+    // is built once per tile shape; builds of the others are freed as soon
+    // as they are concatenated.
+    using AxtPtr = std::unique_ptr<axspmv_axt, void (*)(axspmv_axt *)>;
+    std::map<const axspmv_csr *, uint32_t> seen;
+    for (uint32_t i = 0; i < count; ++i) seen[mats[i]]++;
+    std::map<std::tuple<const axspmv_csr *, uint32_t, uint32_t>, AxtPtr> built;
     for (uint32_t i = 0; i < count; ++i) {
         const axspmv_csr *a = mats[i];
         axspmv_opts o = opts[per_matrix ? i : 0];
@@ -78,9 +159,19 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
         if (o.th == 0) o.th = 1;
         if (o.thw == 0) o.thw = o.th == 1 ? 4 : 64;
         if (!is_pow2(o.thw) || o.thw > 64) return fail(AXSPMV_E_UNSUPPORTED, "batch_create: thw must be a power of two <= 64");
-        axspmv_axt *m = nullptr;
-        AX_TRY(axspmv_build_axt(a, nullptr, 256, o.thw, o.th, AXSPMV_AXT_UNC, AXSPMV_BUILD_PLANES, &m, nullptr));
-        std::unique_ptr<axspmv_axt, void (*)(axspmv_axt *)> guard(m, axspmv_axt_free);
+        const auto key = std::make_tuple(a, o.thw, o.th);
+        auto hit = built.find(key);
+        AxtPtr own(nullptr, axspmv_axt_free);
+        const axspmv_axt *m = nullptr;
+        if (hit != built.end()) {
+            m = hit->second.get();
+        } else {
+            axspmv_axt *fresh = nullptr;
+            AX_TRY(axspmv_build_axt(a, nullptr, 256, o.thw, o.th, AXSPMV_AXT_UNC, AXSPMV_BUILD_PLANES, &fresh, nullptr));
+            own.reset(fresh);
+            m = fresh;
+            if (seen[a] > 1) built.emplace(key, std::move(own));
+        }
         // keep every matrix's planes 64-slot aligned (vector loads)
         const uint64_t voff = vtype == AXSPMV_F32 ? vcat32.size() : vcat.size();
         const uint64_t nslots = m->lenAX / 2;
@@ -126,7 +217,7 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
         b->info.ncols += a->ncols;
         b->info.nnz += a->nnz;
         b->info.slots += nslots;
-        b->info.alg_bytes += (vs + 4) * uint64_t(a->nnz) + 4ull * (a->nrows + 1) + 8ull * a->ncols + 8ull * a->nrows;
+        b->info.alg_bytes += alg_bytes(a, vs);
         xoff += a->ncols;
         yoff += a->nrows;
     }
@@ -195,16 +286,15 @@ extern "C" int axspmv_batch_get_info(const axspmv_batch *b, axspmv_batch_info *i
 
 extern "C" int axspmv_batch_spmv(axspmv_batch *b, const double *d_x, double *d_y, void *stream) {
     if (!b || !d_x || !d_y) return fail(AXSPMV_E_INVALID, "batch_spmv: NULL argument");
-    const hipError_t e = launch::batch(b->args, d_x, d_y, static_cast<hipStream_t>(stream));
+    const hipError_t e = b->launch_on(d_x, d_y, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "batch launch");
 }
 
 extern "C" int axspmv_batch_time(axspmv_batch *b, const double *d_x, double *d_y, uint32_t iters, uint32_t warmup,
                                  uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
     if (!b || !d_x || !d_y) return fail(AXSPMV_E_INVALID, "batch_time: NULL argument");
-    const launch::BatchArgs g = b->args;
-    return time_launches([g, d_x, d_y](hipStream_t st) { return launch::batch(g, d_x, d_y, st); }, iters, warmup,
-                         mode, graph_batch, stream, out);
+    return time_launches([b, d_x, d_y](hipStream_t st) { return b->launch_on(d_x, d_y, st); }, iters, warmup, mode,
+                         graph_batch, stream, out);
 }
 
 // ---------------------------------------------------------------- tuner
@@ -327,34 +417,87 @@ extern "C" int axspmv_autotune(const axspmv_csr *a, uint32_t op, uint32_t k, con
     return AXSPMV_OK;
 }
 
+namespace {
+
+// Nonzeros of the batch a group is tuned on: enough to fill the persistent
+// grid several times over, so each candidate is timed at the throughput its
+// matrices see inside a large batch, not at a lone small launch's latency.
+constexpr uint64_t TUNE_BATCH_NNZ = 4ull << 20;
+constexpr uint32_t TUNE_BATCH_MAX = 4096;
+
+// Index of the fastest candidate on a batch of the group's members (repeated
+// until it holds TUNE_BATCH_NNZ nonzeros).
+int tune_group(const axspmv_csr *const *mats, const std::vector<uint32_t> &members,
+               const std::vector<axspmv_opts> &cs, uint32_t iters, void *stream, uint32_t *best) {
+    std::vector<const axspmv_csr *> sub;
+    uint64_t nnz = 0, nrows = 0, ncols = 0;
+    for (size_t i = 0; sub.size() < TUNE_BATCH_MAX && nnz < TUNE_BATCH_NNZ; ++i) {
+        if (i == members.size() && nnz == 0) break;  // empty matrices only: once is enough
+        const axspmv_csr *a = mats[members[i % members.size()]];
+        sub.push_back(a);
+        nnz += a->nnz;
+        nrows += a->nrows;
+        ncols += a->ncols;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void *dx = nullptr, *dy = nullptr;
+    if (hipMalloc(&dx, 8 * std::max<uint64_t>(1, ncols)) != hipSuccess ||
+        hipMalloc(&dy, 8 * std::max<uint64_t>(1, nrows)) != hipSuccess) {
+        if (dx) (void)hipFree(dx);
+        return fail(AXSPMV_E_NOMEM, "autotune_batch: vector allocation failed");
+    }
+    int rc = launch::fill_f64(static_cast<double *>(dx), std::max<uint64_t>(1, ncols), 1.0, s) == hipSuccess
+                 ? AXSPMV_OK
+                 : fail(AXSPMV_E_HIP, "autotune_batch: fill launch failed");
+    double best_t = INFINITY;
+    *best = 0;
+    for (uint32_t c = 0; rc == AXSPMV_OK && c < cs.size(); ++c) {
+        axspmv_batch *b = nullptr;
+        if (axspmv_batch_create(sub.data(), uint32_t(sub.size()), &cs[c], 0, stream, &b) != AXSPMV_OK) continue;
+        axspmv_timing tm;
+        if (axspmv_batch_time(b, static_cast<double *>(dx), static_cast<double *>(dy), iters, 2, AXSPMV_TIME_REGION,
+                              1, stream, &tm) == AXSPMV_OK &&
+            tm.mean_ms < best_t) {
+            best_t = tm.mean_ms;
+            *best = c;
+        }
+        axspmv_batch_destroy(b);
+    }
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    if (rc == AXSPMV_OK && !std::isfinite(best_t)) rc = fail(AXSPMV_E_UNSUPPORTED, "autotune_batch: no candidate ran");
+    return rc;
+}
+
+}  // namespace
+
 extern "C" int axspmv_autotune_batch(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *cand,
                                      uint32_t ncand, uint32_t iters, void *stream, axspmv_opts *out_opts,
                                      uint32_t *ngroups) {
     if (!mats || !out_opts || count == 0) return fail(AXSPMV_E_INVALID, "autotune_batch: bad argument");
     std::vector<axspmv_opts> cs =
         cand ? std::vector<axspmv_opts>(cand, cand + ncand) : default_candidates(AXSPMV_OP_SPMV, true);
-    std::map<std::pair<uint32_t, uint32_t>, uint32_t> chosen;  // shape -> candidate index
+    if (cs.empty()) return fail(AXSPMV_E_INVALID, "autotune_batch: no candidates");
+    if (iters == 0) iters = 20;
+    std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> groups;  // shape -> members
     for (uint32_t i = 0; i < count; ++i) {
         const axspmv_csr *a = mats[i];
         if (!a) return fail(AXSPMV_E_INVALID, "autotune_batch: matrix %u is NULL", i);
         const uint32_t lg = a->nrows ? ilog2(a->nrows) : 0;
         const double mean = a->nrows ? double(a->nnz) / a->nrows : 0.0;
-        const auto key = std::make_pair(lg, uint32_t(mean / 4.0));
-        auto it = chosen.find(key);
-        if (it == chosen.end()) {
-            uint32_t best = 0;
-            AX_TRY(axspmv_autotune(a, AXSPMV_OP_SPMV, 0, cs.data(), uint32_t(cs.size()), iters, stream, &best, nullptr));
-            it = chosen.emplace(key, best).first;
-        }
-        out_opts[i] = cs[it->second];
+        groups[std::make_pair(lg, uint32_t(mean / 4.0))].push_back(i);
     }
-    if (ngroups) *ngroups = uint32_t(chosen.size());
+    for (const auto &g : groups) {
+        uint32_t best = 0;
+        AX_TRY(tune_group(mats, g.second, cs, iters, stream, &best));
+        for (uint32_t i : g.second) out_opts[i] = cs[best];
+    }
+    if (ngroups) *ngroups = uint32_t(groups.size());
     return AXSPMV_OK;
 }
 
 extern "C" int axspmv_batch_graph(axspmv_batch *b, const double *d_x, double *d_y, uint32_t repeats,
                                   axspmv_graph **out) {
     if (!b || !d_x || !d_y) return fail(AXSPMV_E_INVALID, "batch_graph: NULL argument");
-    const launch::BatchArgs g = b->args;
-    return capture_graph([g, d_x, d_y](hipStream_t st) { return launch::batch(g, d_x, d_y, st); }, repeats, out);
+    return capture_graph([b, d_x, d_y](hipStream_t st) { return b->launch_on(d_x, d_y, st); }, repeats, out);
 }

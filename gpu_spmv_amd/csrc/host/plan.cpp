@@ -555,13 +555,22 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     // the row blocks read narrow column bands anyway.
     constexpr uint64_t slice_cols = (2ull << 20) / 8;
     uint32_t S = p.o.slices;
+    bool banded = false;
     if (S == 0) {
         S = 1;
-        if (a->ncols > slice_cols && !narrow_bands(a, p.o.tile_rows ? p.o.tile_rows : auto_rows(1), slice_cols))
-            while (S < 8 && ceil_div(a->ncols, S) > slice_cols) S *= 2;
+        if (a->ncols > slice_cols) {
+            banded = narrow_bands(a, p.o.tile_rows ? p.o.tile_rows : auto_rows(1), slice_cols);
+            if (!banded)
+                while (S < 8 && ceil_div(a->ncols, S) > slice_cols) S *= 2;
+        }
     }
     if (S != 1 && S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_INVALID, "RCT: slices must be 1, 2, 4 or 8");
-    uint32_t Rt = p.o.tile_rows ? p.o.tile_rows : auto_rows(S);
+    // Narrow bands reuse x within a band whatever the tile height: 2048-row
+    // tiles (16 KB of LDS) let several workgroups share a CU, which pays on
+    // matrices mixing short and long rows (256 banded matrices as one
+    // block-diagonal plan: 1259 us against 1390 us at 16384 rows) and costs
+    // at most 2.5 % on uniform ones.
+    uint32_t Rt = p.o.tile_rows ? p.o.tile_rows : banded ? std::min<uint32_t>(2048, auto_rows(1)) : auto_rows(S);
     if (Rt == 0 || Rt > lds_rows)
         return fail(AXSPMV_E_UNSUPPORTED, "RCT: %u rows per tile do not fit %zu bytes of LDS", Rt, lds_max);
     const uint32_t nrb = uint32_t(ceil_div(n, Rt));
@@ -612,14 +621,22 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     std::vector<uint32_t> tptr(ntile + 1);
     for (uint64_t t = 0; t <= ntile; ++t) tptr[t] = uint32_t(cnt[t]);
     const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
-    // Pack (column within the slice, row within the tile) into one word when
-    // both fit in 32 bits.
+    // Pack (column - tile base, row within the tile) into one word when both
+    // fit in 32 bits.  A tile's entries are sorted by column, so its base (its
+    // smallest column) is its first entry's and its span ends at its last.
     auto bits = [](uint64_t v) {  // bits to hold values 0 .. v-1
         uint32_t b = 1;
         while ((1ull << b) < v) ++b;
         return b;
     };
-    const uint32_t rbits = bits(R), cbits = bits(C);
+    std::vector<uint32_t> cbase(ntile, 0);
+    uint64_t span = 1;
+    for (uint64_t t = 0; t < ntile; ++t)
+        if (cnt[t + 1] > cnt[t]) {
+            cbase[t] = col[cnt[t]];
+            span = std::max<uint64_t>(span, uint64_t(col[cnt[t + 1] - 1]) - cbase[t] + 1);
+        }
+    const uint32_t rbits = bits(R), cbits = bits(span);
     const bool packed = rbits + cbits <= 32 && !(p.o.flags & AXSPMV_PLAN_RCT_SPLIT);
     void *dv = p.dalloc(vs * nnz);
     auto *dt = static_cast<uint32_t *>(p.dalloc(4 * tptr.size()));
@@ -638,18 +655,18 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         AX_TRY(p.upload(dv, val.data(), 8 * nnz, s));
     }
     AX_TRY(p.upload(dt, tptr.data(), 4 * tptr.size(), s));
-    uint32_t *dc = nullptr, *dk = nullptr;
+    uint32_t *dc = nullptr, *dk = nullptr, *dcb = nullptr;
     uint16_t *dl = nullptr;
     if (packed) {
         std::vector<uint32_t> key(nnz ? nnz : 1);
 #pragma omp parallel for schedule(dynamic, 1)
-        for (uint64_t t = 0; t < ntile; ++t) {
-            const uint32_t cb = uint32_t(t % S) * C;
-            for (uint64_t q = cnt[t]; q < cnt[t + 1]; ++q) key[q] = ((col[q] - cb) << rbits) | lrow[q];
-        }
+        for (uint64_t t = 0; t < ntile; ++t)
+            for (uint64_t q = cnt[t]; q < cnt[t + 1]; ++q) key[q] = ((col[q] - cbase[t]) << rbits) | lrow[q];
         dk = static_cast<uint32_t *>(p.dalloc(4 * nnz));
-        if (!dk) return AXSPMV_E_NOMEM;
+        dcb = static_cast<uint32_t *>(p.dalloc(4 * ntile));
+        if (!dk || !dcb) return AXSPMV_E_NOMEM;
         AX_TRY(p.upload(dk, key.data(), 4 * nnz, s));
+        AX_TRY(p.upload(dcb, cbase.data(), 4 * ntile, s));
     } else {
         dc = static_cast<uint32_t *>(p.dalloc(4 * nnz));
         dl = static_cast<uint16_t *>(p.dalloc(2 * nnz));
@@ -661,7 +678,7 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     // 512 threads (8 waves, one workgroup per CU with R = 16384) measured
     // fastest on the 2^20-row power-law matrix; 1024 is 2-3 % slower
     if (block == 0 || block > 1024 || block % 64) block = 512;
-    p.rct = launch::RctArgs{dv, dc, dl, dk, dt, drb, n, R, S, nrb, p.o.vtype, block, rbits, C};
+    p.rct = launch::RctArgs{dv, dc, dl, dk, dcb, dt, drb, n, R, S, nrb, p.o.vtype, block, rbits};
     p.slots = nnz;
     // matrix stream, and with S > 1 the clear of y and S atomic adds per row
     p.stream_bytes = (vs + (packed ? 4 : 6)) * nnz + 4 * tptr.size() + (S > 1 ? 8ull * (S + 1) * n : 0);

@@ -425,11 +425,15 @@ int axspmv_dev_build_axc(const axspmv_dcsr *a, const double *d_x, uint32_t hbs, 
 void axspmv_dfmt_free(axspmv_dfmt *f);
 
 /* ------------------------------------------------------------ batches */
-/* Many matrices of different sizes, SpMV in one persistent launch.  Each
- * matrix uses the AXT UNC layout of its own options (opts[i] if per_matrix,
- * else opts[0] for all; format must be AXSPMV_FMT_AXT, mode UNC).  x and y
- * are the concatenations of the matrices' x (ncols each) and y (nrows each)
- * in batch order.  No reference counterpart. */
+/* Many matrices of different sizes, SpMV in one launch.  x and y are the
+ * concatenations of the matrices' x (ncols each) and y (nrows each) in batch
+ * order.  opts[0].format AXSPMV_FMT_AXT (mode UNC): one persistent launch,
+ * each matrix in the AXT UNC layout of its own options (opts[i] if
+ * per_matrix, else opts[0] for all).  AXSPMV_FMT_RCT: the block-diagonal
+ * matrix of all members as one RCT plan (one option set: per_matrix options
+ * must all be equal; nitems in the info = its tiles); sizes of the whole
+ * batch must fit 32 bits.  A graph of an RCT batch captures the plan's
+ * kernels.  No reference counterpart. */
 typedef struct axspmv_batch axspmv_batch;
 int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *opts,
                         int per_matrix, void *stream, axspmv_batch **out);
@@ -478,9 +482,11 @@ int axspmv_autotune_defaults(uint32_t op, axspmv_opts *out, uint32_t cap, uint32
 int axspmv_autotune(const axspmv_csr *a, uint32_t op, uint32_t k, const axspmv_opts *cand,
                     uint32_t ncand, uint32_t iters, void *stream, uint32_t *best, double *times_ms);
 /* Per-matrix choice for a batch: matrices are grouped by shape (rows rounded
- * to a power of two, mean row length in steps of 4); the first matrix of
- * each group is tuned with axspmv_autotune (SpMV) and its winner is written
- * to out_opts[i] for every member.  *ngroups gets the number of groups.
+ * to a power of two, mean row length in steps of 4); each group is tuned on a
+ * batch of its own members (repeated until it holds about 4M nonzeros, so a
+ * candidate is timed at the throughput it reaches inside a large batch) and
+ * the winner is written to out_opts[i] for every member.  *ngroups gets the
+ * number of groups.
  * cand = NULL uses the AXT UNC part of the built-in set (batches run AXT UNC). */
 int axspmv_autotune_batch(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *cand,
                           uint32_t ncand, uint32_t iters, void *stream, axspmv_opts *out_opts,

@@ -55,6 +55,65 @@ def test_batch_uniform_options(kw):
     assert info["count"] == len(MATS) and info["nnz"] == sum(m.nnz for m in MATS)
 
 
+@pytest.mark.parametrize("kw", [dict(), dict(tile_rows=1024), dict(tile_rows=256, slices=2), dict(vtype=ax.F32)])
+def test_rct_batch(kw):
+    """RCT batch: one RCT plan of the block-diagonal matrix; row blocks cross
+    matrix boundaries; empty matrices and empty rows give zero rows."""
+    e = ax.CsrMatrix(np.zeros(51, np.uint32), np.zeros(0, np.uint32), np.zeros(0), ncols=70)
+    mats = MATS[:20] + [e] + MATS[20:30] + [ax.banded(3000, 3, 0.3, seed=6)]
+    b = ax.Batch(mats, fmt=ax.FMT_RCT, **kw)
+    if kw.get("vtype") == ax.F32:
+        ref = [ax.CsrMatrix(m.row, m.col, m.val.astype(np.float32).astype(np.float64), ncols=m.ncols) for m in mats]
+    else:
+        ref = mats
+    dx, dy, y = check_batch(b, ref, ax.uniform(85, int(b.xoff[-1])))
+    info = b.info()
+    assert info["count"] == len(mats) and info["nnz"] == sum(m.nnz for m in mats) and info["nitems"] >= 1
+    # graph replays, eager work in between (the LDS sums are unordered: equal
+    # to the eager result within the tolerance, not bit for bit)
+    g = ax.Graph.of_batch(b, dx, dy, repeats=3)
+    for rep in range(3):
+        dy.upload(np.full(y.size, np.nan))
+        g.launch()
+        ax._lib.call("axspmv_device_sync")
+        assert rel_err(y, dy.download(np.float64, y.size)) <= TOL, rep
+
+
+def test_rct_batch_options():
+    """An RCT batch takes one option set: equal per-matrix options are fine,
+    different ones are an error."""
+    mats = MATS[:4]
+    same = [ax.opts(ax.FMT_RCT, tile_rows=512)] * 4
+    b = ax.Batch(mats, opts_list=same)
+    check_batch(b, mats, ax.uniform(86, int(b.xoff[-1])))
+    with pytest.raises(ax.AxspmvError, match="one option set"):
+        ax.Batch(mats, opts_list=same[:3] + [ax.opts(ax.FMT_RCT, tile_rows=1024)])
+
+
+def test_batch_repeated_matrices():
+    """One matrix listed several times (built once per tile shape, as the
+    tuner's group batches do), also with different options per listing."""
+    mats = [MATS[0], MATS[1], MATS[0], MATS[0], MATS[2], MATS[1]]
+    b = ax.Batch(mats, thw=4)
+    check_batch(b, mats, ax.uniform(82, int(b.xoff[-1])))
+    ol = [ax.opts(ax.FMT_AXT, thw=t, th=h) for t, h in ((4, 1), (2, 1), (2, 1), (64, 4), (4, 1), (4, 1))]
+    b = ax.Batch(mats, opts_list=ol)
+    check_batch(b, mats, ax.uniform(83, int(b.xoff[-1])))
+
+
+def test_autotune_batch_groups():
+    """Each shape group is tuned on a batch of its repeated members; a group
+    of empty matrices is tuned once and still gets options."""
+    e = ax.CsrMatrix(np.zeros(101, np.uint32), np.zeros(0, np.uint32), np.zeros(0), ncols=100)
+    mats = [MATS[3], e, MATS[4], e, MATS[3]]
+    cands = [ax.opts(ax.FMT_AXT, thw=t) for t in (2, 4)] + [ax.opts(ax.FMT_AXT, thw=64, th=4)]
+    ol, ng = ax.autotune_batch(mats, candidates=cands, iters=3)
+    assert len(ol) == len(mats) and ng >= 2
+    assert (ol[0].thw, ol[0].th) == (ol[4].thw, ol[4].th) and (ol[1].thw, ol[1].th) == (ol[3].thw, ol[3].th)
+    b = ax.Batch(mats, opts_list=ol)
+    check_batch(b, mats, ax.uniform(84, int(b.xoff[-1])))
+
+
 def test_batch_mixed_per_matrix_options():
     choices = [dict(thw=4), dict(thw=2), dict(thw=16), dict(thw=64, th=2), dict(thw=64, th=4), dict(thw=32, th=8),
                dict(thw=1)]

@@ -166,6 +166,26 @@ def test_rct_auto_slices(gen, slices):
     assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
 
 
+@pytest.mark.parametrize("gen,split", [
+    (lambda: ax.banded(300000, 16, 0.5, seed=9), False),    # span of a tile ~ 16k + 33 columns: 14 + 15 bits
+    (lambda: ax.random_sparse(300000, 1, 12, seed=10), True),  # span ~ 300k columns: 14 + 19 bits
+], ids=["banded", "random"])
+def test_rct_key_packing(gen, split):
+    """The packed key holds the column relative to the tile's smallest column:
+    a banded matrix with 2^14-row tiles over 300,000 columns packs, a random
+    one keeps separate column and row arrays; both give the same product."""
+    m = gen()
+    p = ax.Plan(m, ax.FMT_RCT, tile_rows=16384, slices=1)
+    assert p.info()["name"].endswith("_split") == split, p.info()["name"]
+    x = ax.uniform(16, m.ncols)
+    assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
+    X = ax.uniform(17, m.ncols * 8).reshape(m.ncols, 8)
+    q = ax.Plan(m, ax.FMT_RCT, tile_rows=2048, slices=1)
+    dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * m.nrows * 8)
+    q.spmm(8, dX, dY)
+    assert rel_err(orc.spmm(m, X), dY.download(np.float64, m.nrows * 8).reshape(m.nrows, 8)) <= TOL
+
+
 @pytest.mark.parametrize("kw", [dict(slices=4, tile_rows=512), dict(slices=1, tile_rows=512)])
 def test_rct_unaligned_y(kw):
     """y at an 8-byte offset (the clear of y falls back to 8-byte stores)."""
