@@ -7,9 +7,10 @@ columns, values N(0,1), x ~ U[-1,1], seed 1 -- FP64.
 
 Prints one JSON line: GFLOP/s of the flagship kernel (2*nnz per SpMV), its
 roofline against 8 TB/s of HBM (algorithmic bytes per SpMV / kernel time
-from HIP events), the CPU baseline (the oracle's CSR product on this host's
-cores), every format on the same matrix, and the adjoint / SpMM / FP32-value
-forms against plain SpMV.
+from HIP events), the CPU baseline (the reference's own CSR product, ncsr,
+built from its sources into oracle/_ref, else the oracle's restatement of
+it, on this host's cores), every format on the same matrix, and the
+adjoint / SpMM / FP32-value forms against plain SpMV.
 
   python bench.py [--steps K] [--warmup W] [--formats flagship|all]
   python bench.py --suite full     also the Laplacian, the permuted 27-point
@@ -76,8 +77,46 @@ def omp_threads():
     return int(os.environ.get("OMP_NUM_THREADS", 0)) or (os.cpu_count() or 1)
 
 
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ref_dump_c64")
+
+
+def cpu_baseline_reference(m, x, seconds: float):
+    """The reference's own ncsr (its host code compiled in place into
+    oracle/_ref), OpenMP over the host threads, repeated for `seconds`; None
+    when that binary is not built or the matrix is not square (it reads x of
+    nrows entries)."""
+    import subprocess
+    import tempfile
+    if not os.path.exists(REF_BIN) or m.nrows != m.ncols:
+        return None
+    threads = omp_threads()
+    with tempfile.TemporaryDirectory() as tmp:
+        mat, xf, out = (os.path.join(tmp, f) for f in ("m.bin", "x.bin", "o"))
+        m.write(mat)
+        np.ascontiguousarray(x[:m.nrows], dtype=np.float64).tofile(xf)
+        try:
+            subprocess.run([REF_BIN, mat, xf, out, "time", str(seconds), str(threads)], check=True,
+                           stdout=subprocess.DEVNULL, timeout=seconds * 6 + 120)
+        except (subprocess.SubprocessError, OSError) as e:
+            log(f"cpu baseline: reference ncsr did not run ({e}); timing the oracle port")
+            return None
+        meta = dict(line.split(None, 1) for line in open(out + ".meta").read().splitlines() if line.strip())
+        y = np.fromfile(out + ".y", np.float64)
+    calls, el = int(meta["calls"]), float(meta["seconds"])
+    per = el / calls
+    alg = 12.0 * m.nnz + 4.0 * (m.nrows + 1) + 8.0 * m.ncols + 8.0 * m.nrows
+    return y, dict(value=round(2.0 * m.nnz / per / 1e9, 3), unit="GFLOP/s", cores=threads, kind="reference",
+                   sample=f"{calls} calls of the reference's ncsr (cudaSpmv.cu:451-466, OpenMP over rows, "
+                          f"built from its sources into oracle/_ref) on the full matrix, {el:.1f} s",
+                   ms_per_spmv=round(per * 1e3, 3), alg_gbs=round(alg / per / 1e9, 1))
+
+
 def cpu_baseline(m, x, seconds: float):
-    """The oracle's CSR product (the reference's ncsr) on the host threads."""
+    """The reference's ncsr where oracle/_ref holds it, else the oracle's
+    restatement of it, on the host threads."""
+    ref = cpu_baseline_reference(m, x, seconds)
+    if ref is not None:
+        return ref
     from oracle import oracle as orc
     threads = omp_threads()
     y = orc.ncsr(m, x, nthreads=threads)

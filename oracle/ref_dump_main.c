@@ -6,10 +6,12 @@
  * (includes/macros, matrixReading, printMatrixStats, ncsr and the K1, AXC and
  * AXT builders) followed by this file, with g++, in one translation unit.
  * Nothing of the reference is copied into the repository; the binary lands
- * in oracle/_ref/ (git-ignored) and only tests/golden/make_fixtures.py runs
- * it, to write the fixture arrays the tests compare against.
+ * in oracle/_ref/ (git-ignored).  tests/golden/make_fixtures.py runs it to
+ * write the fixture arrays the tests compare against, and the benchmark's
+ * CPU baseline runs its `time` mode: the reference's own ncsr, repeated.
  *
  *   ref_dump M.bin X.bin OUT ncsr
+ *   ref_dump M.bin X.bin OUT time SECONDS THREADS
  *   ref_dump M.bin X.bin OUT k1
  *   ref_dump M.bin X.bin OUT axc HBS
  *   ref_dump M.bin X.bin OUT axt BS THW TH UNC|COM
@@ -45,7 +47,26 @@ int main(int argc, char **argv) {
     FILE *meta = fopen(metap, "w");
     if (!meta) { perror(metap); return 2; }
     fprintf(meta, "nrows %u\nnnz %u\n", csr.nrows, csr.nnz);
-    if (!strcmp(what, "ncsr")) {
+    if (!strcmp(what, "time") && argc >= 7) {
+        /* ncsr(THREADS) back to back until SECONDS have passed (the
+         * reference's test_ncsr times single calls the same way) */
+        const double limit = atof(argv[5]);
+        const UIN nt = (UIN)atoi(argv[6]);
+        FPT *y = (FPT *)calloc(csr.nrows + 1, sizeof(FPT));
+        struct timespec t0, t1;
+        unsigned long calls = 0;
+        double el = 0.0;
+        ncsr(nt, csr, x, y); /* first touch, untimed */
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        do {
+            ncsr(nt, csr, x, y);
+            ++calls;
+            clock_gettime(CLOCK_MONOTONIC, &t1);
+            el = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+        } while (el < limit);
+        dump_raw(out, "y", y, csr.nrows * sizeof(FPT));
+        fprintf(meta, "threads %u\ncalls %lu\nseconds %.9f\n", nt, calls, el);
+    } else if (!strcmp(what, "ncsr")) {
         FPT *y = (FPT *)calloc(csr.nrows + 1, sizeof(FPT));
         ncsr(1, csr, x, y);
         dump_raw(out, "y", y, csr.nrows * sizeof(FPT));
