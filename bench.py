@@ -80,6 +80,16 @@ def omp_threads():
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ref_dump_c64")
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline_reference(m, x, seconds: float):
     """The reference's own ncsr (its host code compiled in place into
     oracle/_ref), OpenMP over the host threads, repeated for `seconds`; None
@@ -94,9 +104,11 @@ def cpu_baseline_reference(m, x, seconds: float):
         mat, xf, out = (os.path.join(tmp, f) for f in ("m.bin", "x.bin", "o"))
         m.write(mat)
         np.ascontiguousarray(x[:m.nrows], dtype=np.float64).tofile(xf)
+        # the OpenMP settings of the reference's launcher (jobLauncher.sh:171-173)
+        env = dict(os.environ, OMP_WAIT_POLICY="active", OMP_DYNAMIC="false", OMP_PROC_BIND="true")
         try:
             subprocess.run([REF_BIN, mat, xf, out, "time", str(seconds), str(threads)], check=True,
-                           stdout=subprocess.DEVNULL, timeout=seconds * 6 + 120)
+                           stdout=subprocess.DEVNULL, timeout=seconds * 6 + 120, env=env)
         except (subprocess.SubprocessError, OSError) as e:
             log(f"cpu baseline: reference ncsr did not run ({e}); timing the oracle port")
             return None
@@ -107,8 +119,10 @@ def cpu_baseline_reference(m, x, seconds: float):
     alg = 12.0 * m.nnz + 4.0 * (m.nrows + 1) + 8.0 * m.ncols + 8.0 * m.nrows
     return y, dict(value=round(2.0 * m.nnz / per / 1e9, 3), unit="GFLOP/s", cores=threads, kind="reference",
                    sample=f"{calls} calls of the reference's ncsr (cudaSpmv.cu:451-466, OpenMP over rows, "
-                          f"built from its sources into oracle/_ref) on the full matrix, {el:.1f} s",
-                   ms_per_spmv=round(per * 1e3, 3), alg_gbs=round(alg / per / 1e9, 1))
+                          f"OMP_PROC_BIND=true, OMP_WAIT_POLICY=active, built from its sources into oracle/_ref) on the "
+                          f"full matrix, "
+                          f"{el:.1f} s",
+                   ms_per_spmv=round(per * 1e3, 3), alg_gbs=round(alg / per / 1e9, 1), cpu=cpu_model())
 
 
 def cpu_baseline(m, x, seconds: float):
@@ -131,7 +145,7 @@ def cpu_baseline(m, x, seconds: float):
     alg = 12.0 * m.nnz + 4.0 * (m.nrows + 1) + 8.0 * m.ncols + 8.0 * m.nrows
     return y, dict(value=round(2.0 * m.nnz / per / 1e9, 3), unit="GFLOP/s", cores=threads, kind="port",
                    sample=f"{calls} calls of ncsr (OpenMP static over rows) on the full matrix, {el:.1f} s",
-                   ms_per_spmv=round(per * 1e3, 3), alg_gbs=round(alg / per / 1e9, 1))
+                   ms_per_spmv=round(per * 1e3, 3), alg_gbs=round(alg / per / 1e9, 1), cpu=cpu_model())
 
 
 def summarize(info, nnz, sec, name, err, extra=None):
