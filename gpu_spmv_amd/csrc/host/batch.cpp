@@ -5,6 +5,7 @@
 #include <map>
 #include <memory>
 #include <numeric>
+#include <string>
 #include <tuple>
 
 #include "plan.hpp"
@@ -57,6 +58,15 @@ int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
     return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "batch upload");
 }
 
+// axspmv_csr_check of batch member i, its message prefixed with the index
+// (copied first: fail() writes the buffer that axspmv_last_error returns).
+int check_member(const axspmv_csr *a, uint32_t i) {
+    if (!a) return fail(AXSPMV_E_INVALID, "batch_create: matrix %u is NULL", i);
+    if (axspmv_csr_check(a) == AXSPMV_OK) return AXSPMV_OK;
+    const std::string why = axspmv_last_error();
+    return fail(AXSPMV_E_INVALID, "batch_create: matrix %u: %s", i, why.c_str());
+}
+
 // Algorithmic bytes of one SpMV of a: values and column indices, row
 // pointer, x and y once.
 uint64_t alg_bytes(const axspmv_csr *a, size_t vs) {
@@ -76,7 +86,7 @@ int batch_create_rct(const axspmv_csr *const *mats, uint32_t count, const axspmv
     const size_t vs = opts[0].vtype == AXSPMV_F32 ? 4 : 8;
     for (uint32_t i = 0; i < count; ++i) {
         const axspmv_csr *a = mats[i];
-        if (!a || !a->row || a->row[a->nrows] != a->nnz) return fail(AXSPMV_E_INVALID, "batch_create: matrix %u inconsistent", i);
+        AX_TRY(check_member(a, i));
         nrows += a->nrows;
         ncols += a->ncols;
         nnz += a->nnz;
@@ -152,7 +162,7 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
     for (uint32_t i = 0; i < count; ++i) {
         const axspmv_csr *a = mats[i];
         axspmv_opts o = opts[per_matrix ? i : 0];
-        if (!a || !a->row || a->row[a->nrows] != a->nnz) return fail(AXSPMV_E_INVALID, "batch_create: matrix %u inconsistent", i);
+        AX_TRY(check_member(a, i));
         if (o.format != AXSPMV_FMT_AXT || o.mode != AXSPMV_AXT_UNC)
             return fail(AXSPMV_E_UNSUPPORTED, "batch_create: matrix %u: batches use the AXT UNC layout", i);
         if (o.vtype != vtype) return fail(AXSPMV_E_INVALID, "batch_create: one value type per batch");

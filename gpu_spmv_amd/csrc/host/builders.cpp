@@ -18,6 +18,7 @@
 // and that plus thw (gathered x value).
 #include <algorithm>
 #include <numeric>
+#include <string>
 
 #include "common.hpp"
 
@@ -27,11 +28,12 @@ namespace {
 
 inline uint64_t ax_of_plane(uint64_t q, uint32_t hw) { return (q / hw) * 2 * hw + q % hw; }
 
+// The builders index x and their tables by column: the whole matrix is
+// checked first (axspmv_csr_check), its message prefixed with the caller.
 int check_input(const axspmv_csr *a, const char *who) {
-    if (!a || !a->row || (a->nnz && (!a->col || !a->val)))
-        return fail(AXSPMV_E_INVALID, "%s: NULL matrix or array", who);
-    if (a->row[a->nrows] != a->nnz) return fail(AXSPMV_E_INVALID, "%s: row[nrows] != nnz", who);
-    return AXSPMV_OK;
+    if (axspmv_csr_check(a) == AXSPMV_OK) return AXSPMV_OK;
+    const std::string why = axspmv_last_error();
+    return fail(AXSPMV_E_INVALID, "%s: %s", who, why.c_str());
 }
 
 void set_fd(axspmv_fmtdata *fd, const char *name, double mfp, double beta, double ct) {

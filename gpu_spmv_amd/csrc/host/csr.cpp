@@ -224,6 +224,7 @@ extern "C" int axspmv_csr_write(const axspmv_csr *a, const char *path) {
 extern "C" int axspmv_csr_transpose(const axspmv_csr *a, axspmv_csr **out) {
     if (!a || !out) return fail(AXSPMV_E_INVALID, "csr_transpose: NULL argument");
     *out = nullptr;
+    AX_TRY(axspmv_csr_check(a));
     axspmv_csr *t = nullptr;
     AX_TRY(axspmv_csr_alloc(a->ncols, a->nrows, a->nnz, &t));
     for (uint32_t j = 0; j < a->nnz; ++j) t->row[a->col[j] + 1]++;

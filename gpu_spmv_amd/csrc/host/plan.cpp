@@ -730,8 +730,10 @@ extern "C" int axspmv_opts_default(uint32_t format, axspmv_opts *o) {
 extern "C" int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream, axspmv_plan **out) {
     if (!a || !o || !out) return fail(AXSPMV_E_INVALID, "plan_create: NULL argument");
     *out = nullptr;
-    if (!a->row || (a->nnz && (!a->col || !a->val)) || a->row[a->nrows] != a->nnz)
-        return fail(AXSPMV_E_INVALID, "plan_create: inconsistent CSR");
+    // Every kernel gathers x[col] without a bounds check and the builders
+    // rely on sorted rows: the whole matrix is checked before anything is
+    // built (row offsets, columns < ncols, strictly increasing per row).
+    AX_TRY(axspmv_csr_check(a));
     if (o->vtype > AXSPMV_F32) return fail(AXSPMV_E_INVALID, "plan_create: unknown value type %u", o->vtype);
     auto p = std::make_unique<Plan>();
     p->o = *o;
