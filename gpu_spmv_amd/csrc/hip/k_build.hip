@@ -23,6 +23,20 @@ using axspmv::launch::DevCsr;
 
 constexpr uint32_t GROUP = 8;  // lanes per row in the fill
 
+// The device CSR's offsets and columns, checked before the fill reads col /
+// val through them: bit 1 offsets decrease, 2 row[0] != 0, 4 row[nrows] !=
+// nnz, 8 a column >= ncols.  One thread per row and per entry; reads only
+// row[0 .. nrows] and col[0 .. nnz).
+__global__ void __launch_bounds__(256) check_kernel(DevCsr a, uint32_t *__restrict__ flag) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint32_t bad = 0;
+    if (i < a.nrows && a.row[i + 1] < a.row[i]) bad |= 1u;
+    if (i == 0 && a.row[0] != 0) bad |= 2u;
+    if (i == 0 && a.row[a.nrows] != a.nnz) bad |= 4u;
+    if (i < a.nnz && a.col[i] >= a.ncols) bad |= 8u;
+    if (bad) atomicOr(flag, bad);
+}
+
 __global__ void __launch_bounds__(256) count_kernel(DevCsr a, uint32_t div, uint64_t *__restrict__ cnt) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= a.nrows) return;
@@ -90,6 +104,14 @@ __global__ void __launch_bounds__(256) fill_kernel(DevCsr a, const uint64_t *__r
 
 namespace axspmv {
 namespace launch {
+
+hipError_t build_check(const DevCsr &a, uint32_t *flag, hipStream_t s) {
+    const hipError_t e = hipMemsetAsync(flag, 0, 4, s);
+    if (e != hipSuccess) return e;
+    const uint64_t threads = std::max<uint64_t>(1, std::max<uint64_t>(a.nrows, a.nnz));
+    check_kernel<<<unsigned((threads + 255) / 256), 256, 0, s>>>(a, flag);
+    return hipGetLastError();
+}
 
 hipError_t build_count(const DevCsr &a, uint32_t div, uint64_t *cnt, hipStream_t s) {
     if (a.nrows) count_kernel<<<(a.nrows + 255) / 256, 256, 0, s>>>(a, div, cnt);

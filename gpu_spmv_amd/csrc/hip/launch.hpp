@@ -189,6 +189,8 @@ struct BuildOut {
     uint32_t *con;     // AXT con, AXC brp
 };
 // cnt[r] = ceil(rl_r / div) for r < nrows
+// *flag = 0, then the offsets / column checks of check_kernel (bits 1, 2, 4, 8)
+hipError_t build_check(const DevCsr &a, uint32_t *flag, hipStream_t s);
 hipError_t build_count(const DevCsr &a, uint32_t div, uint64_t *cnt, hipStream_t s);
 // rocPRIM exclusive scan; tmp == nullptr queries tmp_bytes
 hipError_t build_scan(const uint64_t *in, uint64_t *out, size_t n, void *tmp, size_t &tmp_bytes, hipStream_t s);

@@ -65,6 +65,12 @@ int run_build(const axspmv_dcsr *a, const double *d_x, int kind, const launch::B
     uint64_t *start = nullptr;
     uint64_t total = 0;
     float ms_scan = 0.0f;
+    // the CSR is checked on the device; its flag comes back with the scan
+    // total, before anything reads col / val through the offsets
+    auto *d_flag = static_cast<uint32_t *>(scratch.alloc(4));
+    if (!d_flag) return fail(AXSPMV_E_NOMEM, "dev build: scratch allocation failed");
+    uint32_t flag = 0;
+    HIP_TRY(launch::build_check(dc, d_flag, s), "dev build: check");
     if (kind <= 1) {
         auto *cnt = static_cast<uint64_t *>(scratch.alloc(8ull * (n + 1)));
         start = static_cast<uint64_t *>(scratch.alloc(8ull * (n + 1)));
@@ -78,9 +84,14 @@ int run_build(const axspmv_dcsr *a, const double *d_x, int kind, const launch::B
         HIP_TRY(launch::build_scan(cnt, start, size_t(n) + 1, tmp, tmp_bytes, s), "dev build: scan");
         HIP_TRY(hipEventRecord(ev.e[1], s), "dev build: event");
         HIP_TRY(hipMemcpyAsync(&total, start + n, 8, hipMemcpyDeviceToHost, s), "dev build: total");
-        HIP_TRY(hipStreamSynchronize(s), "dev build: sync");
-        HIP_TRY(hipEventElapsedTime(&ms_scan, ev.e[0], ev.e[1]), "dev build: elapsed");
     }
+    HIP_TRY(hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, s), "dev build: check flag");
+    HIP_TRY(hipStreamSynchronize(s), "dev build: sync");
+    if (flag)
+        return fail(AXSPMV_E_INVALID, "dev build: invalid device CSR:%s%s%s%s", flag & 1 ? " row offsets decrease;" : "",
+                    flag & 2 ? " row[0] != 0;" : "", flag & 4 ? " row[nrows] != nnz;" : "",
+                    flag & 8 ? " a column >= ncols;" : "");
+    if (kind <= 1) HIP_TRY(hipEventElapsedTime(&ms_scan, ev.e[0], ev.e[1]), "dev build: elapsed");
     uint64_t tileN, lenSEC, lenAX;
     if (kind == 0) {
         tileN = total;

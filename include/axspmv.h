@@ -417,7 +417,10 @@ typedef struct axspmv_dfmt {
  * 1004-1062, run on the host 250 times in the reference): the same arrays,
  * bit for bit, built by kernels from a device CSR.  d_x may be NULL (the
  * x-halves stay 0; axspmv_x_refresh fills them later).  Same parameter
- * rules and errors as the host builders. */
+ * rules and errors as the host builders.  The device CSR's offsets (row[0]
+ * = 0, non-decreasing, row[nrows] = nnz) and column bounds are checked on
+ * the device before anything is filled (AXSPMV_E_INVALID, nothing built);
+ * columns must be sorted within rows (not checked). */
 int axspmv_dev_build_axt(const axspmv_dcsr *a, const double *d_x, uint32_t bs, uint32_t thw, uint32_t th,
                          uint32_t mode, void *stream, axspmv_dfmt **out);
 int axspmv_dev_build_axc(const axspmv_dcsr *a, const double *d_x, uint32_t hbs, void *stream,

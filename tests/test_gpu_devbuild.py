@@ -80,3 +80,21 @@ def test_dev_build_rejects_what_the_host_rejects():
             ax.dev_build_axt(m, None, *args)
     with pytest.raises(ax.AxspmvError):
         ax.dev_build_axc(m, None, 0)
+
+
+@pytest.mark.parametrize("case,row,col,ncols,msg", [
+    ("column >= ncols", [0, 2, 3, 5], [0, 2, 1, 0, 3], 3, "column >= ncols"),
+    ("offsets decrease", [0, 2, 1, 5], [0, 2, 1, 0, 2], 3, "row offsets decrease"),
+    ("row[nrows] != nnz", [0, 2, 3, 4], [0, 2, 1, 0, 2], 3, "row\\[nrows\\] != nnz"),
+    ("row[0] != 0", [1, 2, 3, 5], [0, 2, 1, 0, 2], 3, "row\\[0\\] != 0"),
+])
+def test_dev_build_checks_the_device_csr(case, row, col, ncols, msg):
+    """A device CSR is checked on the device (offsets, column bounds) before
+    the fill reads through its offsets: an error, nothing is built."""
+    m = ax.CsrMatrix(np.array(row), np.array(col), np.ones(len(col)), ncols=ncols)
+    with pytest.raises(ax.AxspmvError, match="invalid device CSR:.*" + msg):
+        ax.dev_build_axt(m, None, 256, 4, 1, ax.AXT_UNC)
+    with pytest.raises(ax.AxspmvError, match="invalid device CSR:.*" + msg):
+        ax.dev_build_axt(m, None, 256, 64, 1, ax.AXT_COM)
+    with pytest.raises(ax.AxspmvError, match="invalid device CSR:.*" + msg):
+        ax.dev_build_axc(m, None, 4)
