@@ -363,6 +363,8 @@ def suite_full(res):
     mats = []
     for i in range(256):
         mats.append(ax.banded(int(rng.integers(1000, 200001)), int(rng.integers(1, 65)), 0.3, seed=5000 + i))
+    # the built-in candidates: AXT UNC per shape group against the RCT
+    # block-diagonal plan for the whole batch
     t0 = time.perf_counter()
     tuned, ngroups = ax.autotune_batch(mats, iters=5)
     tune_s = round(time.perf_counter() - t0, 1)
@@ -390,8 +392,10 @@ def suite_full(res):
                                       frac=round(info["alg_bytes"] / sec / 1e9 / PEAK_HBM_GBS, 4),
                                       max_rel_err_sampled=max(errs), build_s=build_s)
         if label == "autotuned":
-            res["batch256"][label].update(tuner_groups=ngroups, tuner_s=tune_s,
-                                          thw_th_used=sorted({(o.thw, o.th) for o in tuned}))
+            res["batch256"][label].update(
+                tuner_option_sets=ngroups, tuner_s=tune_s,
+                chosen="rct" if tuned[0].format == ax.FMT_RCT else
+                f"axt per group, (thw, th) used: {sorted({(o.thw, o.th) for o in tuned})}")
         log(f"batch256 {label}: {res['batch256'][label]}")
         del dxb, dyb, bt
 

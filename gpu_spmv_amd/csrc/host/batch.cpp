@@ -310,8 +310,9 @@ extern "C" int axspmv_batch_time(axspmv_batch *b, const double *d_x, double *d_y
 // ---------------------------------------------------------------- tuner
 namespace {
 
-// Built-in candidates.  Batches take AXT UNC only (the persistent batch
-// kernel runs that layout); single plans also try K1 and RCT.  SpMM: AXT
+// Built-in candidates.  Batches take AXT UNC per shape group and RCT (the
+// block-diagonal plan) for the whole batch; single plans also try K1 and
+// RCT.  SpMM: AXT
 // k-tiles, and RCT tiles whose rows x k-tile sums fill the LDS (one column
 // slice and the default slicing).
 std::vector<axspmv_opts> default_candidates(uint32_t op, bool for_batch = false) {
@@ -352,7 +353,11 @@ std::vector<axspmv_opts> default_candidates(uint32_t op, bool for_batch = false)
             c.push_back(o);
         }
     }
-    if (for_batch) return c;
+    if (for_batch) {
+        axspmv_opts_default(AXSPMV_FMT_RCT, &o);
+        c.push_back(o);
+        return c;
+    }
     axspmv_opts_default(AXSPMV_FMT_K1, &o);
     c.push_back(o);
     axspmv_opts_default(AXSPMV_FMT_RCT, &o);
@@ -489,20 +494,79 @@ extern "C" int axspmv_autotune_batch(const axspmv_csr *const *mats, uint32_t cou
         cand ? std::vector<axspmv_opts>(cand, cand + ncand) : default_candidates(AXSPMV_OP_SPMV, true);
     if (cs.empty()) return fail(AXSPMV_E_INVALID, "autotune_batch: no candidates");
     if (iters == 0) iters = 20;
+    // AXT candidates are chosen per shape group, RCT ones for the whole batch
+    std::vector<axspmv_opts> axt, rct;
+    for (const axspmv_opts &o : cs) {
+        if (o.format == AXSPMV_FMT_AXT)
+            axt.push_back(o);
+        else if (o.format == AXSPMV_FMT_RCT)
+            rct.push_back(o);
+        else
+            return fail(AXSPMV_E_UNSUPPORTED, "autotune_batch: candidates must be AXT or RCT, got format %u", o.format);
+    }
     std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> groups;  // shape -> members
+    uint64_t nrows = 0, ncols = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const axspmv_csr *a = mats[i];
         if (!a) return fail(AXSPMV_E_INVALID, "autotune_batch: matrix %u is NULL", i);
         const uint32_t lg = a->nrows ? ilog2(a->nrows) : 0;
         const double mean = a->nrows ? double(a->nnz) / a->nrows : 0.0;
         groups[std::make_pair(lg, uint32_t(mean / 4.0))].push_back(i);
+        nrows += a->nrows;
+        ncols += a->ncols;
     }
     for (const auto &g : groups) {
         uint32_t best = 0;
-        AX_TRY(tune_group(mats, g.second, cs, iters, stream, &best));
-        for (uint32_t i : g.second) out_opts[i] = cs[best];
+        if (!axt.empty()) AX_TRY(tune_group(mats, g.second, axt, iters, stream, &best));
+        for (uint32_t i : g.second) out_opts[i] = axt.empty() ? rct[0] : axt[best];
     }
-    if (ngroups) *ngroups = uint32_t(groups.size());
+    uint32_t chosen_sets = uint32_t(groups.size());
+    if (!rct.empty()) {
+        // The whole batch, timed with the per-group AXT choice and with each
+        // RCT option set (the block-diagonal plan); the fastest wins.
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        void *dx = nullptr, *dy = nullptr;
+        if (hipMalloc(&dx, 8 * std::max<uint64_t>(1, ncols)) != hipSuccess ||
+            hipMalloc(&dy, 8 * std::max<uint64_t>(1, nrows)) != hipSuccess) {
+            if (dx) (void)hipFree(dx);
+            return fail(AXSPMV_E_NOMEM, "autotune_batch: vector allocation failed");
+        }
+        struct Free {
+            void *a, *b;
+            ~Free() {
+                (void)hipFree(a);
+                (void)hipFree(b);
+            }
+        } guard{dx, dy};
+        if (launch::fill_f64(static_cast<double *>(dx), std::max<uint64_t>(1, ncols), 1.0, s) != hipSuccess)
+            return fail(AXSPMV_E_HIP, "autotune_batch: fill launch failed");
+        auto time_batch = [&](const axspmv_opts *o, int per_matrix, double *ms) {
+            axspmv_batch *b = nullptr;
+            AX_TRY(axspmv_batch_create(mats, count, o, per_matrix, stream, &b));
+            axspmv_timing tm;
+            const int rc = axspmv_batch_time(b, static_cast<double *>(dx), static_cast<double *>(dy), iters, 2,
+                                             AXSPMV_TIME_REGION, 1, stream, &tm);
+            axspmv_batch_destroy(b);
+            if (rc == AXSPMV_OK) *ms = tm.mean_ms;
+            return rc;
+        };
+        double best = INFINITY;
+        int which = -1;  // -1: the per-group AXT choice
+        if (!axt.empty()) AX_TRY(time_batch(out_opts, 1, &best));
+        for (size_t r = 0; r < rct.size(); ++r) {
+            double ms = INFINITY;
+            if (time_batch(&rct[r], 0, &ms) == AXSPMV_OK && ms < best) {
+                best = ms;
+                which = int(r);
+            }
+        }
+        if (!std::isfinite(best)) return fail(AXSPMV_E_UNSUPPORTED, "autotune_batch: no candidate ran");
+        if (which >= 0) {
+            for (uint32_t i = 0; i < count; ++i) out_opts[i] = rct[which];
+            chosen_sets = 1;
+        }
+    }
+    if (ngroups) *ngroups = chosen_sets;
     return AXSPMV_OK;
 }
 

@@ -477,20 +477,24 @@ int axspmv_graph_destroy(axspmv_graph *g);
  * reference sweeps block sizes by relaunching (jobLauncher.sh:145-176). */
 /* The built-in candidates: for SpMV AXT UNC th = 1 with thw 2..16 and th =
  * 2..8 with thw 64 at two block budgets, K1 (chunk 64) and RCT (default, 256
- * and 1024 threads); for SpMM the four AXT k-tiles and eight RCT plans
+ * and 1024 threads); for batches the AXT part and the default RCT options;
+ * for SpMM the four AXT k-tiles and eight RCT plans
  * (ktile 8 with 1024 and 2048 rows, 16 with 512, 32 with 256; one column
  * slice and the default slicing).  Writes min(cap, count) options to out and
  * the set's size to *count. */
 int axspmv_autotune_defaults(uint32_t op, axspmv_opts *out, uint32_t cap, uint32_t *count);
 int axspmv_autotune(const axspmv_csr *a, uint32_t op, uint32_t k, const axspmv_opts *cand,
                     uint32_t ncand, uint32_t iters, void *stream, uint32_t *best, double *times_ms);
-/* Per-matrix choice for a batch: matrices are grouped by shape (rows rounded
- * to a power of two, mean row length in steps of 4); each group is tuned on a
- * batch of its own members (repeated until it holds about 4M nonzeros, so a
- * candidate is timed at the throughput it reaches inside a large batch) and
- * the winner is written to out_opts[i] for every member.  *ngroups gets the
- * number of groups.
- * cand = NULL uses the AXT UNC part of the built-in set (batches run AXT UNC). */
+/* Options for a batch.  AXT UNC candidates are chosen per matrix: matrices
+ * are grouped by shape (rows rounded to a power of two, mean row length in
+ * steps of 4) and each group is tuned on a batch of its own members
+ * (repeated until it holds about 4M nonzeros, so a candidate is timed at the
+ * throughput it reaches inside a large batch).  RCT candidates apply to the
+ * whole batch (one block-diagonal plan): the whole batch is timed with the
+ * per-group AXT choice and with each RCT option set, and the fastest is
+ * written to out_opts (every entry the same option set if RCT wins).
+ * *ngroups gets the number of distinct option sets chosen.  cand = NULL uses
+ * the AXT UNC part of the built-in SpMV set plus the default RCT options. */
 int axspmv_autotune_batch(const axspmv_csr *const *mats, uint32_t count, const axspmv_opts *cand,
                           uint32_t ncand, uint32_t iters, void *stream, axspmv_opts *out_opts,
                           uint32_t *ngroups);

@@ -114,6 +114,29 @@ def test_autotune_batch_groups():
     check_batch(b, mats, ax.uniform(84, int(b.xoff[-1])))
 
 
+def test_autotune_batch_whole_batch_candidates():
+    """RCT candidates compete for the whole batch against the per-group AXT
+    choice; the result is one format, and the batch it builds is correct."""
+    mats = MATS[:12]
+    ol, ng = ax.autotune_batch(mats, candidates=[ax.opts(ax.FMT_AXT, thw=4), ax.opts(ax.FMT_RCT)], iters=3)
+    assert len({o.format for o in ol}) == 1
+    if ol[0].format == ax.FMT_RCT:
+        assert ng == 1
+    b = ax.Batch(mats, opts_list=ol)
+    check_batch(b, mats, ax.uniform(87, int(b.xoff[-1])))
+    # RCT candidates only: one of them for every matrix
+    rc = [ax.opts(ax.FMT_RCT, tile_rows=512), ax.opts(ax.FMT_RCT)]
+    ol, ng = ax.autotune_batch(mats, candidates=rc, iters=3)
+    assert ng == 1 and ol[0].format == ax.FMT_RCT and ol[0].tile_rows in (0, 512)
+    assert all(o.tile_rows == ol[0].tile_rows for o in ol)
+    # the built-in set (AXT per group + default RCT)
+    ol, ng = ax.autotune_batch(mats, iters=3)
+    b = ax.Batch(mats, opts_list=ol)
+    check_batch(b, mats, ax.uniform(88, int(b.xoff[-1])))
+    with pytest.raises(ax.AxspmvError, match="AXT or RCT"):
+        ax.autotune_batch(mats, candidates=[ax.opts(ax.FMT_K1)], iters=3)
+
+
 def test_batch_mixed_per_matrix_options():
     choices = [dict(thw=4), dict(thw=2), dict(thw=16), dict(thw=64, th=2), dict(thw=64, th=4), dict(thw=32, th=8),
                dict(thw=1)]
