@@ -1,0 +1,77 @@
+"""GPU: the benchmark configurations at their full sizes (the power-law
+matrix of 2^20 rows is covered in test_gpu_spmv.py).
+
+* 27-point stencil on a 128^3 grid, symmetric random permutation (seed 2):
+  n = 2,097,152, nnz = 55,742,968 (the nonzero count is a closed form of the
+  grid, checked first); every format against the oracle's ncsr, 1e-12.
+* Forward + adjoint on the same structure with values N(1, 0.1) (seed 4),
+  b ~ U[-1, 1]: y, r = y - b, x_bar = A^T r against the oracle (1e-12 /
+  1e-11), v_bar = r[row] * x[col] bit for bit against the oracle's product of
+  the r the kernel wrote.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import gpu_spmv_amd as ax
+from conftest import rel_err
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+NT = int(os.environ.get("OMP_NUM_THREADS", "8"))
+
+
+@pytest.fixture(scope="module")
+def stencil():
+    m = ax.stencil27(128, permute=True, seed=2)
+    # interior 26 neighbours, fewer on faces: sum over the grid of the
+    # product of the per-axis counts (1 + 2 interior, 2 at the ends)
+    per_axis = 3 * 128 - 2
+    assert (m.nrows, m.nnz) == (128 ** 3, per_axis ** 3) == (2097152, 55742968)
+    x = ax.uniform(1, m.ncols)
+    return m, x, orc.ncsr(m, x, nthreads=NT)
+
+
+FORMATS = [
+    ("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
+    ("csr_vector", ax.FMT_CSR, dict(kernel=ax.KER_STREAM)),
+    ("axt_unc_h1_thw4", ax.FMT_AXT, dict(thw=4, th=1)),
+    ("axt_unc_h4_thw64", ax.FMT_AXT, dict(thw=64, th=4)),
+    ("axt_com_h1_thw64_bs256", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=256)),
+    ("axt_com_h8_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=8)),
+    ("k1_c64", ax.FMT_K1, dict(chunk=64)),
+    ("axc_hbs4", ax.FMT_AXC, dict(hbs=4)),
+    ("rct_auto", ax.FMT_RCT, dict()),
+    ("rct_r8192_s1_split", ax.FMT_RCT, dict(tile_rows=8192, slices=1, flags=ax.PLAN_RCT_SPLIT)),
+]
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("name,fmt,kw", FORMATS, ids=[f[0] for f in FORMATS])
+def test_stencil_config3_full_size(stencil, name, fmt, kw):
+    m, x, ref = stencil
+    p = ax.Plan(m, fmt, **kw)
+    y = ax.spmv_host(p, x)
+    assert rel_err(ref, y) <= TOL, p.info()["name"]
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_RCT, dict())], ids=["axt", "rct"])
+def test_fwd_adj_config5_full_size(stencil, fmt, kw):
+    s, x, _ = stencil
+    m = ax.CsrMatrix(s.row, s.col, s.val.copy(), ncols=s.ncols)
+    ax.fill_values_normal(m, 1.0, 0.1, 4)
+    b = ax.uniform(4, m.nrows, stream=ax.VEC_STREAM + 1)
+    n, nc, nnz = m.nrows, m.ncols, m.nnz
+    p = ax.Plan(m, fmt, flags=ax.PLAN_TRANSPOSE, **kw)
+    dx, db = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer.from_array(b)
+    outs = [ax.DeviceBuffer(8 * k) for k in (n, n, nc, nnz)]
+    p.fwd_adj(dx, db, *outs, fused=True)
+    y, r, xbar, vbar = (o.download(np.float64, k) for o, k in zip(outs, (n, n, nc, nnz)))
+    y_ref = orc.ncsr(m, x, nthreads=NT)
+    assert rel_err(y_ref, y) <= TOL
+    assert rel_err(y_ref - b, r) <= TOL
+    assert rel_err(orc.ncsr_t(m, y_ref - b, nc), xbar) <= 1e-11
+    assert np.array_equal(p.vbar_csr(vbar), orc.vbar(m, r, x))
