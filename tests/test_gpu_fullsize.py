@@ -75,3 +75,28 @@ def test_fwd_adj_config5_full_size(stencil, fmt, kw):
     assert rel_err(y_ref - b, r) <= TOL
     assert rel_err(orc.ncsr_t(m, y_ref - b, nc), xbar) <= 1e-11
     assert np.array_equal(p.vbar_csr(vbar), orc.vbar(m, r, x))
+
+
+@pytest.fixture(scope="module")
+def powerlaw():
+    return ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("k", [8, 64])
+@pytest.mark.parametrize("vt", [ax.F64, ax.F32], ids=["f64", "f32val"])
+@pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_RCT, dict(tile_rows=1024)),
+                                    (ax.FMT_RCT, dict(tile_rows=256, slices=1))],
+                         ids=["axt", "rct_r1024", "rct_r256_s1"])
+def test_spmm_config4a_full_size(powerlaw, k, vt, fmt, kw):
+    """Config 4A: k right-hand sides (x ~ U[-1, 1], seed 3) on the power-law
+    matrix; FP32 values against the oracle on the float-rounded values."""
+    m = powerlaw
+    X = ax.uniform(3, m.ncols * k).reshape(m.ncols, k)
+    mv = m if vt == ax.F64 else ax.CsrMatrix(m.row, m.col, m.val.astype(np.float32).astype(np.float64),
+                                              ncols=m.ncols)
+    ref = orc.spmm(mv, X, nthreads=NT)
+    p = ax.Plan(m, fmt, vtype=vt, **kw)
+    dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * m.nrows * k)
+    p.spmm(k, dX, dY)
+    assert rel_err(ref, dY.download(np.float64, m.nrows * k).reshape(m.nrows, k)) <= TOL
