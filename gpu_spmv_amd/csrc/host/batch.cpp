@@ -58,13 +58,11 @@ int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
     return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "batch upload");
 }
 
-// axspmv_csr_check of batch member i, its message prefixed with the index
-// (copied first: fail() writes the buffer that axspmv_last_error returns).
+// check_csr_bounds of batch member i, its message prefixed with the index.
 int check_member(const axspmv_csr *a, uint32_t i) {
-    if (!a) return fail(AXSPMV_E_INVALID, "batch_create: matrix %u is NULL", i);
-    if (axspmv_csr_check(a) == AXSPMV_OK) return AXSPMV_OK;
-    const std::string why = axspmv_last_error();
-    return fail(AXSPMV_E_INVALID, "batch_create: matrix %u: %s", i, why.c_str());
+    char who[64];
+    std::snprintf(who, sizeof who, "batch_create: matrix %u", i);
+    return check_csr_bounds(a, who);
 }
 
 // Algorithmic bytes of one SpMV of a: values and column indices, row

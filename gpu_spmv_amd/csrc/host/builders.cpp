@@ -28,13 +28,9 @@ namespace {
 
 inline uint64_t ax_of_plane(uint64_t q, uint32_t hw) { return (q / hw) * 2 * hw + q % hw; }
 
-// The builders index x and their tables by column: the whole matrix is
-// checked first (axspmv_csr_check), its message prefixed with the caller.
-int check_input(const axspmv_csr *a, const char *who) {
-    if (axspmv_csr_check(a) == AXSPMV_OK) return AXSPMV_OK;
-    const std::string why = axspmv_last_error();
-    return fail(AXSPMV_E_INVALID, "%s: %s", who, why.c_str());
-}
+// The builders index x and their tables by column: offsets and columns are
+// checked first.
+int check_input(const axspmv_csr *a, const char *who) { return check_csr_bounds(a, who); }
 
 void set_fd(axspmv_fmtdata *fd, const char *name, double mfp, double beta, double ct) {
     if (!fd) return;

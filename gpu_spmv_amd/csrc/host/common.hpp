@@ -33,6 +33,14 @@ inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 // Wall-clock seconds (CLOCK_MONOTONIC, as the reference's GT macro).
 double now_s();
 
+// What every kernel and builder needs of a host CSR to stay inside its
+// arrays: non-NULL arrays, row[0] = 0, non-decreasing offsets, row[nrows] =
+// nnz and every column < ncols.  Unsorted rows and repeated columns are
+// accepted (the products sum every entry, as the reference's kernels do);
+// axspmv_csr_check also requires strictly increasing columns.  Messages
+// start with `who`.
+int check_csr_bounds(const axspmv_csr *a, const char *who);
+
 }  // namespace axspmv
 
 #define AX_TRY(expr)                        \

@@ -27,6 +27,20 @@ double now_s() {
     return double(t.tv_sec) + 1e-9 * double(t.tv_nsec);
 }
 
+int check_csr_bounds(const axspmv_csr *a, const char *who) {
+    if (!a || !a->row || (a->nnz && (!a->col || !a->val)))
+        return fail(AXSPMV_E_INVALID, "%s: NULL matrix or array", who);
+    if (a->row[0] != 0) return fail(AXSPMV_E_INVALID, "%s: row[0] = %u, expected 0", who, a->row[0]);
+    if (a->row[a->nrows] != a->nnz)
+        return fail(AXSPMV_E_INVALID, "%s: row[nrows] = %u but nnz = %u", who, a->row[a->nrows], a->nnz);
+    for (uint32_t i = 0; i < a->nrows; ++i)
+        if (a->row[i + 1] < a->row[i]) return fail(AXSPMV_E_INVALID, "%s: row offsets decrease at row %u", who, i);
+    for (uint32_t j = 0; j < a->nnz; ++j)
+        if (a->col[j] >= a->ncols)
+            return fail(AXSPMV_E_INVALID, "%s: col[%u] = %u >= ncols %u", who, j, a->col[j], a->ncols);
+    return AXSPMV_OK;
+}
+
 }  // namespace axspmv
 
 using namespace axspmv;
@@ -224,7 +238,7 @@ extern "C" int axspmv_csr_write(const axspmv_csr *a, const char *path) {
 extern "C" int axspmv_csr_transpose(const axspmv_csr *a, axspmv_csr **out) {
     if (!a || !out) return fail(AXSPMV_E_INVALID, "csr_transpose: NULL argument");
     *out = nullptr;
-    AX_TRY(axspmv_csr_check(a));
+    AX_TRY(check_csr_bounds(a, "csr_transpose"));
     axspmv_csr *t = nullptr;
     AX_TRY(axspmv_csr_alloc(a->ncols, a->nrows, a->nnz, &t));
     for (uint32_t j = 0; j < a->nnz; ++j) t->row[a->col[j] + 1]++;

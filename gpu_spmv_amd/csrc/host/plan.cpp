@@ -98,8 +98,9 @@ std::vector<launch::Window> block_windows(const axspmv_csr *a, const std::vector
         uint32_t cmin = UINT32_MAX, cmax = 0;
         for (; r < n && cum[r] < b.end; ++r) {
             if (a->row[r + 1] == a->row[r]) continue;
-            cmin = std::min(cmin, a->col[a->row[r]]);
-            cmax = std::max(cmax, a->col[a->row[r + 1] - 1]);
+            const auto mm = std::minmax_element(a->col + a->row[r], a->col + a->row[r + 1]);  // rows may be unsorted
+            cmin = std::min(cmin, *mm.first);
+            cmax = std::max(cmax, *mm.second);
         }
         if (cmin <= cmax) w[i] = launch::Window{cmin, cmax - cmin + 1};
     }
@@ -730,10 +731,9 @@ extern "C" int axspmv_opts_default(uint32_t format, axspmv_opts *o) {
 extern "C" int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream, axspmv_plan **out) {
     if (!a || !o || !out) return fail(AXSPMV_E_INVALID, "plan_create: NULL argument");
     *out = nullptr;
-    // Every kernel gathers x[col] without a bounds check and the builders
-    // rely on sorted rows: the whole matrix is checked before anything is
-    // built (row offsets, columns < ncols, strictly increasing per row).
-    AX_TRY(axspmv_csr_check(a));
+    // Every kernel gathers x[col] without a bounds check: offsets and
+    // columns are checked before anything is built.
+    AX_TRY(check_csr_bounds(a, "plan_create"));
     if (o->vtype > AXSPMV_F32) return fail(AXSPMV_E_INVALID, "plan_create: unknown value type %u", o->vtype);
     auto p = std::make_unique<Plan>();
     p->o = *o;

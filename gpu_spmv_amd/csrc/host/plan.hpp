@@ -75,8 +75,8 @@ int time_launches(const std::function<hipError_t(hipStream_t)> &fn, uint32_t ite
 // Captures `repeats` calls of fn into an executable graph.
 int capture_graph(const std::function<hipError_t(hipStream_t)> &fn, uint32_t repeats, axspmv_graph **out);
 std::vector<launch::Block> row_blocks(const std::vector<uint32_t> &units_per_row, uint32_t budget);
-// Column window of every block of row_blocks(units_per_row, ...): the blocks
-// hold whole consecutive rows, and columns are sorted within a row.
+// Column window (smallest to largest column) of every block of
+// row_blocks(units_per_row, ...): the blocks hold whole consecutive rows.
 std::vector<launch::Window> block_windows(const axspmv_csr *a, const std::vector<uint32_t> &units_per_row,
                                           const std::vector<launch::Block> &blocks);
 // True when at least half of the windows fit STAGE_COLS.

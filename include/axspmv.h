@@ -58,7 +58,7 @@ int axspmv_abi_version(void);
 typedef struct axspmv_csr {
     uint32_t nrows, ncols, nnz, owner;
     uint32_t *row;   /* nrows + 1 */
-    uint32_t *col;   /* nnz, sorted within each row */
+    uint32_t *col;   /* nnz, < ncols (any order within a row; see axspmv_csr_check) */
     double *val;     /* nnz */
     uint32_t *rl;    /* nrows (filled by axspmv_csr_stats) */
     uint32_t rmin, rmax, bw, pad_;
@@ -70,7 +70,11 @@ int axspmv_csr_alloc(uint32_t nrows, uint32_t ncols, uint32_t nnz, axspmv_csr **
 void axspmv_csr_free(axspmv_csr *a);
 
 /* Structural checks: row[0]==0, row nondecreasing, row[nrows]==nnz,
- * col < ncols, columns strictly increasing within a row. */
+ * col < ncols, columns strictly increasing within a row.  Plans, batches,
+ * builders and the transpose check everything here but the column order
+ * themselves (AXSPMV_E_INVALID before anything is built): unsorted rows and
+ * repeated columns are accepted, every entry is summed, as by the
+ * reference's kernels. */
 int axspmv_csr_check(const axspmv_csr *a);
 
 /* rl[i] = row[i+1]-row[i] and rmin/rave/rmax/rsd/bw.
@@ -384,9 +388,9 @@ int axspmv_time_op(axspmv_plan *p, uint32_t op, const axspmv_operands *ops, uint
                    axspmv_timing *out);
 
 /* ------------------------------------------------- builders on the device */
-/* A CSR matrix whose arrays are in device memory (caller-owned); it must
- * satisfy the rules of axspmv_csr_check (the device builders trust the row
- * pointer; a column >= ncols only leaves its x value 0). */
+/* A CSR matrix whose arrays are in device memory (caller-owned): the same
+ * rules as a host CSR (see axspmv_csr_check); the device builders check the
+ * offsets and column bounds on the device. */
 typedef struct axspmv_dcsr {
     uint32_t nrows, ncols, nnz, pad_;
     const uint32_t *row;  /* nrows + 1 */
@@ -420,7 +424,7 @@ typedef struct axspmv_dfmt {
  * rules and errors as the host builders.  The device CSR's offsets (row[0]
  * = 0, non-decreasing, row[nrows] = nnz) and column bounds are checked on
  * the device before anything is filled (AXSPMV_E_INVALID, nothing built);
- * columns must be sorted within rows (not checked). */
+ * any column order within a row is built as the host builders build it. */
 int axspmv_dev_build_axt(const axspmv_dcsr *a, const double *d_x, uint32_t bs, uint32_t thw, uint32_t th,
                          uint32_t mode, void *stream, axspmv_dfmt **out);
 int axspmv_dev_build_axc(const axspmv_dcsr *a, const double *d_x, uint32_t hbs, void *stream,
