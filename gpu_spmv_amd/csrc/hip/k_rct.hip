@@ -72,15 +72,24 @@ struct TileOf {
     }
 };
 
-template <class VT, bool PACKED>
-__global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__restrict__ x, double *__restrict__ y) {
+// VB: the same product on the plan of A^T, fused with the value gradient of
+// A: vb[e] = xr[row(e)] * x[col(e)] for every entry, in entry order (the
+// tile's xr values sit in LDS after its sums).  With x = r and xr = the
+// forward x, y = A^T r and vb holds r[i] * x[j] for every nonzero A(i, j).
+template <class VT, bool PACKED, bool VB>
+__global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__restrict__ x, double *__restrict__ y,
+                                                   const double *__restrict__ xr, double *__restrict__ vb) {
     extern __shared__ double yl[];
     TileOf tile;
     if (!tile.from(a, blockIdx.x)) return;  // the whole workgroup leaves together
     const uint32_t s = tile.s, rb = tile.rb;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t row0 = a.rb_start[rb], rows = a.rb_start[rb + 1] - row0;  // rows <= a.R
-    for (uint32_t k = tid; k < rows; k += nt) yl[k] = 0.0;
+    double *xs = yl + a.R;  // VB: xr of the tile's rows
+    for (uint32_t k = tid; k < rows; k += nt) {
+        yl[k] = 0.0;
+        if constexpr (VB) xs[k] = xr[row0 + k];
+    }
     __syncthreads();
     const uint32_t t = rb * a.S + s;  // tile index in storage order
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
@@ -106,22 +115,34 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
 #pragma unroll
             for (int j = 0; j < U; ++j) ent.load(min(e0 + j * nt, last), S.c[j], S.r[j], S.v[j]);
         };
-        auto add = [&](const Set &S, const double *xv, uint32_t e0) {
+        // VB stores are unpredicated too: a lane past the end holds the
+        // tile's last entry, so it stores that entry's own value again.  The
+        // LDS reads of xr go out with the x gathers, ahead of the wait.
+        auto add = [&](const Set &S, const double *xv, const double *xl, uint32_t e0) {
 #pragma unroll
-            for (int j = 0; j < U; ++j) atomicAdd(&yl[S.r[j]], e0 + j * nt < end ? S.v[j] * xv[j] : 0.0);
+            for (int j = 0; j < U; ++j) {
+                atomicAdd(&yl[S.r[j]], e0 + j * nt < end ? S.v[j] * xv[j] : 0.0);
+                if constexpr (VB) vb[min(e0 + j * nt, last)] = xl[j] * xv[j];
+            }
+        };
+        auto gather = [&](const Set &S, double *xv, double *xl) {
+#pragma unroll
+            for (int j = 0; j < U; ++j) xv[j] = x[S.c[j]];
+            if constexpr (VB) {
+#pragma unroll
+                for (int j = 0; j < U; ++j) xl[j] = xs[S.r[j]];
+            }
         };
         auto round = [&](const Set &S, Set &N, uint32_t e0) {
-            double xv[U];
-#pragma unroll
-            for (int j = 0; j < U; ++j) xv[j] = x[S.c[j]];
+            double xv[U], xl[U];
+            gather(S, xv, xl);
             fetch(N, e0 + span);
-            add(S, xv, e0);
+            add(S, xv, xl, e0);
         };
         auto final_round = [&](const Set &S, uint32_t e0) {
-            double xv[U];
-#pragma unroll
-            for (int j = 0; j < U; ++j) xv[j] = x[S.c[j]];
-            add(S, xv, e0);
+            double xv[U], xl[U];
+            gather(S, xv, xl);
+            add(S, xv, xl, e0);
         };
         Set A, B;
         uint32_t e = beg + tid;
@@ -300,12 +321,14 @@ hipError_t rct_vbar(const RctArgs &a, const double *ybar, const double *x, doubl
     return hipGetLastError();
 }
 
-hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid) {
+hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid, const double *xr,
+               double *vbar) {
     if (!a.nrows) return hipSuccess;
     const uint32_t per = 8 / a.S;
     const uint32_t groups = (a.nrb + per - 1) / per;
     const unsigned grid = groups * 8;
-    const size_t lds = size_t(a.R) * sizeof(double);
+    const bool vb = vbar != nullptr;
+    const size_t lds = size_t(a.R) * sizeof(double) * (vb ? 2 : 1);
     if (a.S > 1) {
         if (reinterpret_cast<uintptr_t>(y) % 16 == 0)
             rct_clear<true><<<(a.nrows / 2 + 256) / 256, 256, 0, s>>>(y, a.nrows);
@@ -317,17 +340,25 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
         }
     }
     const bool packed = a.key != nullptr;
+#define AX_RCT_LAUNCH(VT, P)                                                  \
+    do {                                                                      \
+        if (vb)                                                               \
+            rct_kernel<VT, P, true><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar);  \
+        else                                                                  \
+            rct_kernel<VT, P, false><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar); \
+    } while (0)
     if (a.vtype == 1) {
         if (packed)
-            rct_kernel<float, true><<<grid, a.block, lds, s>>>(a, x, y);
+            AX_RCT_LAUNCH(float, true);
         else
-            rct_kernel<float, false><<<grid, a.block, lds, s>>>(a, x, y);
+            AX_RCT_LAUNCH(float, false);
     } else {
         if (packed)
-            rct_kernel<double, true><<<grid, a.block, lds, s>>>(a, x, y);
+            AX_RCT_LAUNCH(double, true);
         else
-            rct_kernel<double, false><<<grid, a.block, lds, s>>>(a, x, y);
+            AX_RCT_LAUNCH(double, false);
     }
+#undef AX_RCT_LAUNCH
     return hipGetLastError();
 }
 

@@ -85,8 +85,11 @@ struct RctArgs {
     uint32_t nrows, R, S, nrb, vtype, block, rbits;
 };
 // S > 1: a fill kernel clears y, then the tile kernel adds into it; mid
-// (optional) is recorded on s between the two
-hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid = nullptr);
+// (optional) is recorded on s between the two.  vbar != nullptr (a plan of
+// A^T, 16 B of LDS per tile row): also vbar[e] = xr[row(e)] * x[col(e)] for
+// every entry, in entry order.
+hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid = nullptr,
+               const double *xr = nullptr, double *vbar = nullptr);
 // Y = A X (k interleaved right-hand sides) on a one-slice plan, kt (8, 16,
 // 32 or 64) right-hand sides per launch; needs R * kt * 8 bytes of LDS
 hipError_t rct_spmm(const RctArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s);

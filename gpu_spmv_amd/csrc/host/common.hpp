@@ -41,6 +41,10 @@ double now_s();
 // start with `who`.
 int check_csr_bounds(const axspmv_csr *a, const char *who);
 
+// axspmv_csr_transpose that also returns, for every position of A^T, the
+// position in A its entry came from (perm may be NULL).
+int csr_transpose_perm(const axspmv_csr *a, axspmv_csr **out, std::vector<uint32_t> *perm);
+
 }  // namespace axspmv
 
 #define AX_TRY(expr)                        \

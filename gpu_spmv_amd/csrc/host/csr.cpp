@@ -234,8 +234,9 @@ extern "C" int axspmv_csr_write(const axspmv_csr *a, const char *path) {
 }
 
 // Counting-sort transpose: row j of A^T lists the rows i of A with A(i,j) != 0
-// in increasing i, so columns of the result come out sorted.
-extern "C" int axspmv_csr_transpose(const axspmv_csr *a, axspmv_csr **out) {
+// in increasing i, so columns of the result come out sorted.  perm (optional)
+// receives, for every position q of A^T, the position in A it came from.
+int axspmv::csr_transpose_perm(const axspmv_csr *a, axspmv_csr **out, std::vector<uint32_t> *perm) {
     if (!a || !out) return fail(AXSPMV_E_INVALID, "csr_transpose: NULL argument");
     *out = nullptr;
     AX_TRY(check_csr_bounds(a, "csr_transpose"));
@@ -244,11 +245,13 @@ extern "C" int axspmv_csr_transpose(const axspmv_csr *a, axspmv_csr **out) {
     for (uint32_t j = 0; j < a->nnz; ++j) t->row[a->col[j] + 1]++;
     for (uint32_t c = 0; c < a->ncols; ++c) t->row[c + 1] += t->row[c];
     std::vector<uint32_t> next(t->row, t->row + a->ncols);
+    if (perm) perm->assign(a->nnz, 0);
     for (uint32_t i = 0; i < a->nrows; ++i)
         for (uint32_t j = a->row[i]; j < a->row[i + 1]; ++j) {
             const uint32_t q = next[a->col[j]]++;
             t->col[q] = i;
             t->val[q] = a->val[j];
+            if (perm) (*perm)[q] = j;
         }
     int rc = axspmv_csr_stats(t);
     if (rc) {
@@ -257,6 +260,10 @@ extern "C" int axspmv_csr_transpose(const axspmv_csr *a, axspmv_csr **out) {
     }
     *out = t;
     return AXSPMV_OK;
+}
+
+extern "C" int axspmv_csr_transpose(const axspmv_csr *a, axspmv_csr **out) {
+    return csr_transpose_perm(a, out, nullptr);
 }
 
 extern "C" int axspmv_rng_u64(uint64_t seed, uint64_t stream, uint64_t *out, size_t n) {

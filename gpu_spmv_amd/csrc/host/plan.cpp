@@ -543,7 +543,7 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     size_t lds_max = 65536;
     if (hipGetDeviceProperties(&prop, p.device) == hipSuccess && prop.sharedMemPerBlock > 0)
         lds_max = prop.sharedMemPerBlock;
-    const uint32_t lds_rows = uint32_t(std::min<size_t>(65536, lds_max / 8));
+    const uint32_t lds_rows = uint32_t(std::min<size_t>(65536, lds_max / p.lds_per_row));
     p.lds_max = lds_max;
     const uint32_t n = a->nrows;
     // Default tile height: as tall as the LDS allows (fewer row blocks, fewer
@@ -564,6 +564,12 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
             if (!banded)
                 while (S < 8 && ceil_div(a->ncols, S) > slice_cols) S *= 2;
         }
+        // The plan of A^T behind a fused value gradient holds 16 B per tile
+        // row; 8192-row tiles with half the slices measured fastest for it
+        // (forward + adjoint on the 2^20-row power-law matrix 241 us against
+        // 269 us at 10240 rows and 4 slices; 702 against 745 us on the
+        // permuted 27-point stencil on 128^3), and for the adjoint alone.
+        if (p.lds_per_row > 8 && S > 1) S /= 2;
     }
     if (S != 1 && S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_INVALID, "RCT: slices must be 1, 2, 4 or 8");
     // Narrow bands reuse x within a band whatever the tile height: 2048-row
@@ -572,6 +578,8 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     // block-diagonal plan: 1259 us against 1390 us at 16384 rows) and costs
     // at most 2.5 % on uniform ones.
     uint32_t Rt = p.o.tile_rows ? p.o.tile_rows : banded ? std::min<uint32_t>(2048, auto_rows(1)) : auto_rows(S);
+    if (p.lds_per_row > 8)  // a transposed plan's tiles are cut to fit (8192 rows when automatic)
+        Rt = std::min(Rt, p.o.tile_rows || banded ? lds_rows : std::min<uint32_t>(lds_rows, 8192));
     if (Rt == 0 || Rt > lds_rows)
         return fail(AXSPMV_E_UNSUPPORTED, "RCT: %u rows per tile do not fit %zu bytes of LDS", Rt, lds_max);
     const uint32_t nrb = uint32_t(ceil_div(n, Rt));
@@ -593,7 +601,7 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     std::vector<uint16_t> lrow(nnz ? nnz : 1);
     std::vector<double> val(nnz ? nnz : 1);
     // the value gradient comes out in entry order: keep each entry's CSR position
-    const bool keep_pos = p.o.flags & AXSPMV_PLAN_TRANSPOSE;
+    const bool keep_pos = p.keep_pos;
     std::vector<uint32_t> pos(keep_pos && nnz ? nnz : 1);
 #pragma omp parallel
     {
@@ -644,9 +652,8 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     auto *drb = static_cast<uint32_t *>(p.dalloc(4 * rbs.size()));
     if (!dv || !dt || !drb) return AXSPMV_E_NOMEM;
     if (keep_pos) {
-        p.d_rct_pos = static_cast<uint32_t *>(p.dalloc(4 * nnz));
-        if (!p.d_rct_pos) return AXSPMV_E_NOMEM;
-        AX_TRY(p.upload(p.d_rct_pos, pos.data(), 4 * nnz, s));
+        pos.resize(nnz);
+        p.rct_pos = std::move(pos);
     }
     AX_TRY(p.upload(drb, rbs.data(), 4 * rbs.size(), s));
     if (p.o.vtype == AXSPMV_F32) {
@@ -728,7 +735,12 @@ extern "C" int axspmv_opts_default(uint32_t format, axspmv_opts *o) {
     return AXSPMV_OK;
 }
 
-extern "C" int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream, axspmv_plan **out) {
+namespace {
+
+// axspmv_plan_create; `transposed` marks the plan of A^T that a plan with
+// AXSPMV_PLAN_TRANSPOSE builds (RCT: entry positions kept, 16 B of LDS per
+// tile row for the fused value gradient).
+int plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream, bool transposed, axspmv_plan **out) {
     if (!a || !o || !out) return fail(AXSPMV_E_INVALID, "plan_create: NULL argument");
     *out = nullptr;
     // Every kernel gathers x[col] without a bounds check: offsets and
@@ -737,6 +749,10 @@ extern "C" int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, voi
     if (o->vtype > AXSPMV_F32) return fail(AXSPMV_E_INVALID, "plan_create: unknown value type %u", o->vtype);
     auto p = std::make_unique<Plan>();
     p->o = *o;
+    if (transposed) {
+        p->keep_pos = true;
+        p->lds_per_row = 16;
+    }
     if (p->o.th == 0) p->o.th = 1;
     if (p->o.thw == 0) p->o.thw = p->o.th == 1 ? 4 : 64;
     if (p->o.hbs == 0) p->o.hbs = 64;
@@ -764,6 +780,12 @@ extern "C" int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, voi
     p->beta = p->slots ? double(a->nnz) / double(p->slots) : 0.0;
     *out = p.release();
     return AXSPMV_OK;
+}
+
+}  // namespace
+
+extern "C" int axspmv_plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream, axspmv_plan **out) {
+    return plan_create(a, o, stream, false, out);
 }
 
 extern "C" int axspmv_plan_destroy(axspmv_plan *p) {
@@ -837,7 +859,9 @@ launch::SegArgs fused_args(const Plan &p) {
 }
 
 hipError_t launch_vbar(Plan *p, const double *ybar, const double *x, double *vbar, hipStream_t s) {
-    if (p->kind == Plan::RCT) return launch::rct_vbar(p->rct, ybar, x, vbar, s);
+    // RCT: over the plan of A^T (rows = columns of A), whose entry order is
+    // the one the fused forward + adjoint writes: x rows in LDS, ybar gathered
+    if (p->kind == Plan::RCT) return launch::rct_vbar(p->T->rct, x, ybar, vbar, s);
     if (p->kind == Plan::AXT_UNC_H1)
         return launch::axt_unc_h1_vbar(seg_args(*p, p->d_sec), p->nunits, x, ybar, vbar,
                                        launch::VbarArgs{p->d_rowptr, p->d_first}, s);
@@ -857,6 +881,11 @@ hipError_t launch_fwd_adj(Plan *p, const double *x, const double *b, double *y, 
     }
     e = plan_launch(p, x, y, s);
     if (e == hipSuccess) e = launch::residual(p->nrows, y, b, r, s);
+    if (fused && p->kind == Plan::RCT) {
+        // x_bar = A^T r and v_bar in one sweep over the plan of A^T
+        if (e == hipSuccess) e = launch::rct(p->T->rct, r, xbar, s, nullptr, x, vbar);
+        return e;
+    }
     if (e == hipSuccess) e = plan_launch(p->T, r, xbar, s);
     if (e == hipSuccess) e = launch_vbar(p, r, x, vbar, s);
     return e;
@@ -883,14 +912,22 @@ namespace axspmv {
 
 int plan_add_transpose(Plan *p, const axspmv_csr *a, hipStream_t s) {
     axspmv_csr *t = nullptr;
-    AX_TRY(axspmv_csr_transpose(a, &t));
+    std::vector<uint32_t> perm;  // position in A^T -> position in A
+    AX_TRY(csr_transpose_perm(a, &t, &perm));
     axspmv_opts o = p->o;
     o.flags &= ~AXSPMV_PLAN_TRANSPOSE;
     axspmv_plan *pt = nullptr;
-    const int rc = axspmv_plan_create(t, &o, s, &pt);
+    const int rc = plan_create(t, &o, s, true, &pt);
     axspmv_csr_free(t);
     if (rc) return rc;
     p->T = pt;
+    if (pt->kind == Plan::RCT) {
+        // v_bar comes out in the transposed plan's entry order: the position
+        // in A of each of its entries
+        p->rct_pos.resize(pt->rct_pos.size());
+        for (size_t e = 0; e < pt->rct_pos.size(); ++e) p->rct_pos[e] = perm[pt->rct_pos[e]];
+        std::vector<uint32_t>().swap(pt->rct_pos);
+    }
     return AXSPMV_OK;
 }
 
@@ -918,8 +955,11 @@ hipError_t plan_op(Plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s
 
 int plan_op_check(const Plan *p, uint32_t op, const axspmv_operands &o) {
     if (!p) return fail(AXSPMV_E_INVALID, "NULL plan");
-    const bool needT = op == AXSPMV_OP_ADJOINT || op == AXSPMV_OP_FWD_ADJ || op == AXSPMV_OP_FWD_ADJ_SPLIT;
-    if (needT && !p->T) return fail(AXSPMV_E_INVALID, "the adjoint needs a plan created with AXSPMV_PLAN_TRANSPOSE");
+    const bool needT = op == AXSPMV_OP_ADJOINT || op == AXSPMV_OP_FWD_ADJ || op == AXSPMV_OP_FWD_ADJ_SPLIT ||
+                       (op == AXSPMV_OP_VBAR && p->kind == Plan::RCT);
+    if (needT && !p->T)
+        return fail(AXSPMV_E_INVALID, "%s needs a plan created with AXSPMV_PLAN_TRANSPOSE",
+                    op == AXSPMV_OP_VBAR ? "the RCT value gradient (it runs on the plan of A^T)" : "the adjoint");
     if ((op == AXSPMV_OP_VBAR || op == AXSPMV_OP_FWD_ADJ || op == AXSPMV_OP_FWD_ADJ_SPLIT) && !adjoint_kind(p))
         return fail(AXSPMV_E_UNSUPPORTED, "value gradient: plan %s is not CSR naive, AXT UNC th=1 or RCT", p->name);
     if (op == AXSPMV_OP_SPMM) {
@@ -995,10 +1035,10 @@ extern "C" int axspmv_plan_vbar_positions(const axspmv_plan *p, uint32_t *h_pos,
     if (!p || (count && !h_pos)) return fail(AXSPMV_E_INVALID, "vbar_positions: NULL argument");
     if (count != p->nnz) return fail(AXSPMV_E_INVALID, "vbar_positions: count %llu != nnz %u", (unsigned long long)count, p->nnz);
     if (p->kind == Plan::RCT) {
-        if (!p->d_rct_pos)
+        if (!p->T)
             return fail(AXSPMV_E_UNSUPPORTED, "vbar_positions: RCT plan created without AXSPMV_PLAN_TRANSPOSE");
-        const hipError_t e = hipMemcpy(h_pos, p->d_rct_pos, 4ull * count, hipMemcpyDeviceToHost);
-        return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "vbar_positions copy");
+        std::copy(p->rct_pos.begin(), p->rct_pos.end(), h_pos);
+        return AXSPMV_OK;
     }
     if (!adjoint_kind(p)) return fail(AXSPMV_E_UNSUPPORTED, "vbar_positions: plan %s has no value gradient", p->name);
     for (uint64_t k = 0; k < count; ++k) h_pos[k] = uint32_t(k);

@@ -37,7 +37,15 @@ struct axspmv_plan {
     axspmv::launch::ComArgs com{};
     axspmv::launch::RctArgs rct{};
     size_t lds_max = 0;             // LDS bytes per workgroup (RCT SpMM k-tile limit)
-    uint32_t *d_rct_pos = nullptr;  // CSR position of every RCT entry (plans with AXSPMV_PLAN_TRANSPOSE)
+    // RCT: keep_pos -- record the CSR position of every entry in rct_pos
+    // (set for the transposed plan of a plan with AXSPMV_PLAN_TRANSPOSE; the
+    // forward plan then holds the positions of the transposed plan's entries
+    // in its own CSR, the order in which v_bar comes out); lds_per_row --
+    // LDS bytes a tile row needs (16 for the transposed plan: the x_bar sums
+    // and the x values of the fused value gradient).
+    bool keep_pos = false;
+    uint32_t lds_per_row = 8;
+    std::vector<uint32_t> rct_pos;
     // Row-contiguous layouts (CSR, AXT UNC th == 1, AXC): CSR row pointer and
     // first plane slot of each row, for SpMM and the value gradient.
     uint32_t *d_rowptr = nullptr, *d_first = nullptr;

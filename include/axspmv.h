@@ -310,20 +310,24 @@ int axspmv_spmv(axspmv_plan *p, const double *d_x, double *d_y, void *stream);
 int axspmv_adjoint(axspmv_plan *p, const double *d_ybar, double *d_xbar, void *stream);
 /* Per-nonzero value gradient v_bar[k] = y_bar[row(k)] * x[col(k)].  Plans:
  * CSR naive and AXT UNC th == 1 write it in the CSR order of the matrix the
- * plan was built from; RCT writes it in its own entry order (tiles, column
- * sorted), and axspmv_plan_vbar_positions gives each entry's CSR position. */
+ * plan was built from; RCT (needs AXSPMV_PLAN_TRANSPOSE) writes it in the
+ * entry order of its plan of A^T, and axspmv_plan_vbar_positions gives each
+ * entry's CSR position. */
 int axspmv_vbar(axspmv_plan *p, const double *d_ybar, const double *d_x, double *d_vbar, void *stream);
 /* h_pos[e] (count = nnz entries) = CSR position of v_bar entry e: the
- * identity for CSR and AXT plans; the entry map of RCT plans created with
- * AXSPMV_PLAN_TRANSPOSE. */
+ * identity for CSR and AXT plans; for RCT plans created with
+ * AXSPMV_PLAN_TRANSPOSE the position of each entry of the plan of A^T. */
 int axspmv_plan_vbar_positions(const axspmv_plan *p, uint32_t *h_pos, uint64_t count);
 /* Forward + adjoint of the loss 0.5*||A x - b||^2: y = A x, r = y - b,
  * x_bar = A^T r, v_bar[k] = r[row(k)] * x[col(k)].  fused = 1: one sweep over
  * A produces y, r and v_bar, then one sweep over A^T; fused = 0: SpMV,
  * residual, adjoint and v_bar as four separate kernels.  Needs
- * AXSPMV_PLAN_TRANSPOSE; plans: CSR naive, AXT UNC th == 1 and RCT (RCT
- * always runs the four passes: r needs every column slice's sums; v_bar in
- * RCT entry order as for axspmv_vbar). */
+ * AXSPMV_PLAN_TRANSPOSE; plans: CSR naive, AXT UNC th == 1 and RCT.  RCT:
+ * r needs every column slice's sums, so the forward SpMV and the residual
+ * come first; fused = 1 then produces x_bar and v_bar in one sweep over the
+ * plan of A^T (whose tiles keep their x values in LDS beside the x_bar sums;
+ * with automatic tiling that plan uses 8192-row tiles and half the slices of
+ * the forward rule); v_bar in the order of axspmv_vbar. */
 int axspmv_fwd_adj(axspmv_plan *p, const double *d_x, const double *d_b, double *d_y, double *d_r,
                    double *d_xbar, double *d_vbar, int fused, void *stream);
 

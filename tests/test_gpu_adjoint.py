@@ -24,6 +24,9 @@ ADJ_KERNELS = [
     ("rct_auto", ax.FMT_RCT, dict()),
     ("rct_r512_s4", ax.FMT_RCT, dict(tile_rows=512, slices=4)),
     ("rct_r1024_s2_split", ax.FMT_RCT, dict(tile_rows=1024, slices=2, flags=ax.PLAN_RCT_SPLIT)),
+    # taller than the transposed plan's tiles may be (x_bar sums and x values
+    # of the fused gradient: 16 B of LDS per row): that plan is cut to fit
+    ("rct_r16384_s4", ax.FMT_RCT, dict(tile_rows=16384, slices=4)),
 ]
 
 
@@ -227,8 +230,35 @@ def test_unsupported_combinations_raise():
         pk.vbar(d, d, d)
     with pytest.raises(ax.AxspmvError):
         pk.vbar_positions()
-    pr = ax.Plan(m, ax.FMT_RCT)  # no transpose: no entry positions kept
+    pr = ax.Plan(m, ax.FMT_RCT)  # no transpose: no plan of A^T for the value gradient
     with pytest.raises(ax.AxspmvError):
         pr.vbar_positions()
+    with pytest.raises(ax.AxspmvError, match="TRANSPOSE"):
+        pr.vbar(d, d, d)
     assert np.array_equal(ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_TRANSPOSE).vbar_positions(),
                           np.arange(m.nnz, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("kw", [dict(tile_rows=16384, slices=1), dict(tile_rows=16384, slices=4), dict()])
+def test_rct_fused_gradient_tall_tiles(kw):
+    """30,000 rows with 16384-row tiles: the plan of A^T keeps its rows' x_bar
+    sums and x values in LDS (16 B a row), so its tiles are cut to fit; the
+    fused sweep (x_bar and v_bar together) equals the separate kernels' v_bar
+    bit for bit, and both match the oracle."""
+    m = ax.random_sparse(30000, 1, 12, seed=33)
+    n, nc, nnz = m.nrows, m.ncols, m.nnz
+    x, b = ax.uniform(34, nc), ax.uniform(35, n, stream=ax.VEC_STREAM + 1)
+    p = ax.Plan(m, ax.FMT_RCT, flags=ax.PLAN_TRANSPOSE, **kw)
+    dx, db = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer.from_array(b)
+    res = {}
+    for fused in (True, False):
+        outs = [ax.DeviceBuffer.from_array(np.full(k, np.nan)) for k in (n, n, nc, nnz)]
+        p.fwd_adj(dx, db, *outs, fused=fused)
+        res[fused] = [o.download(np.float64, k) for o, k in zip(outs, (n, n, nc, nnz))]
+    y, r, xbar, vbar = res[True]
+    r_ref = orc.ncsr(m, x) - b
+    assert rel_err(r_ref, r) <= TOL
+    assert rel_err(orc.ncsr_t(m, r_ref, nc), xbar) <= 1e-11
+    assert np.array_equal(p.vbar_csr(vbar), orc.vbar(m, r, x))
+    # the separate kernels: the same entry order, each v_bar from its own r
+    assert np.array_equal(p.vbar_csr(res[False][3]), orc.vbar(m, res[False][1], x))
