@@ -80,6 +80,46 @@ def omp_threads():
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ref_dump_c64")
 
 
+CMP_LIB = os.path.join(ROOT, "gpu_spmv_amd", "librocsparse_cmp.so")
+ROCSPARSE_ALGS = ("adaptive", "rowsplit", "lrb", "nnzsplit")
+
+
+def rocsparse_rows(m, dx, yref, iters: int = 50):
+    """rocSPARSE's CSR SpMV on the same matrix and x, each of its CSR
+    algorithms (the counterpart of the reference's cuSPARSE row,
+    cudaSpmv.cu:629-704): a comparator built into librocsparse_cmp.so,
+    never a library path.  None when that library is not built."""
+    import ctypes as C
+    if not os.path.exists(CMP_LIB):
+        return None
+    lib = C.CDLL(CMP_LIB)
+    u32, vp = C.c_uint32, C.c_void_p
+    lib.cmp_rocsparse_spmv.argtypes = [u32, u32, u32, vp, vp, vp, vp, vp, C.c_int, u32,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.cmp_rocsparse_spmv.restype = C.c_int
+    lib.cmp_last_error.restype = C.c_char_p
+    n, nc, nnz = m.nrows, m.ncols, m.nnz
+    drow, dcol, dval = (ax.DeviceBuffer.from_array(a) for a in (m.row, m.col, m.val))
+    dy = ax.DeviceBuffer(8 * n)
+    alg = 12.0 * nnz + 4.0 * (n + 1) + 8.0 * nc + 8.0 * n
+    rows = []
+    for i, name in enumerate(ROCSPARSE_ALGS):
+        an, t, api = C.c_double(), C.c_double(), C.c_int()
+        rc = lib.cmp_rocsparse_spmv(n, nc, nnz, drow.ptr, dcol.ptr, dval.ptr, dx.ptr, dy.ptr, i, iters,
+                                    C.byref(an), C.byref(t), C.byref(api))
+        if rc:
+            rows.append(dict(label=f"rocsparse_{name}", error=lib.cmp_last_error().decode()))
+            continue
+        y = dy.download(np.float64, n)
+        sec = t.value * 1e-3
+        rows.append(dict(label=f"rocsparse_{name}", us=round(sec * 1e6, 2), gflops=round(2.0 * nnz / sec / 1e9, 2),
+                         alg_gbs=round(alg / sec / 1e9, 1), frac=round(alg / sec / 1e9 / PEAK_HBM_GBS, 4),
+                         analysis_ms=round(an.value, 2), api=f"rocsparse v{api.value}",
+                         rel_err=rel_err(yref, y) if yref is not None else None))
+        log(f"rocsparse: {rows[-1]}")
+    return rows
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -335,6 +375,7 @@ def suite_full(res):
         c1.append(r)
         log(f"laplace256: {r}")
     res["laplace256"] = c1
+    res["laplace256_rocsparse"] = rocsparse_rows(m1, d1, y1, iters=200)
     # config 3: 27-point stencil 128^3, permuted
     m3 = ax.stencil27(128, permute=True, seed=2)
     x3 = ax.uniform(2, m3.ncols)
@@ -346,6 +387,7 @@ def suite_full(res):
         c3.append(r)
         log(f"stencil128p: {r}")
     res["stencil128p"] = c3
+    res["stencil128p_rocsparse"] = rocsparse_rows(m3, d3, y3, iters=20)
     # config 5: forward + adjoint on the stencil structure, values N(1, 0.1)
     ax.fill_values_normal(m3, 1.0, 0.1, 4)
     y5 = orc.ncsr(m3, x3, nthreads=omp_threads())
@@ -516,6 +558,7 @@ def main():
         plain = next(f["us"] for f in formats if f.get("label") == FLAGSHIP_AXT[0])
         extras = extras_same_matrix(m, x, dx, yref, plain)
         extras["conversion"] = conversion(m, x, dx)
+        extras["rocsparse"] = rocsparse_rows(m, dx, yref)
     if args.suite == "full":
         suite_full(extras)
 
