@@ -725,7 +725,7 @@ extern "C" int axspmv_opts_default(uint32_t format, axspmv_opts *o) {
     o->thw = 4;
     o->th = 1;
     o->bs = 256;
-    o->hbs = 64;
+    o->hbs = format == AXSPMV_FMT_CSR ? 0 : 64;  // CSR: the vector kernel picks its lanes per row
     o->chunk = 64;
     o->reorder = 1;
     o->vtype = AXSPMV_F64;
@@ -755,7 +755,7 @@ int plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream, bool tr
     }
     if (p->o.th == 0) p->o.th = 1;
     if (p->o.thw == 0) p->o.thw = p->o.th == 1 ? 4 : 64;
-    if (p->o.hbs == 0) p->o.hbs = 64;
+    if (p->o.hbs == 0 && p->o.format != AXSPMV_FMT_CSR) p->o.hbs = 64;
     if (p->o.chunk == 0) p->o.chunk = 64;
     if (p->o.bs == 0) p->o.bs = 256;
     if (hipGetDevice(&p->device) != hipSuccess) return fail(AXSPMV_E_HIP, "plan_create: no HIP device");

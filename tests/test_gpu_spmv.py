@@ -166,6 +166,20 @@ def test_rct_auto_slices(gen, slices):
     assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
 
 
+@pytest.mark.parametrize("gen,lanes", [(lambda: ax.laplace2d(128), 4),
+                                       (lambda: ax.powerlaw(1 << 14, 2.0, 1024, 20.0, seed=1), 16),
+                                       (lambda: ax.banded(20000, 32, 0.9, seed=4), 64)],
+                         ids=["laplace_mean5", "powerlaw_mean20", "banded_mean58"])
+def test_csr_vector_default_lanes(gen, lanes):
+    """The CSR vector kernel's default lanes per row follow the mean row
+    length (the power of two g with 1.5 g >= mean, within [2, 64])."""
+    m = gen()
+    p = ax.Plan(m, ax.FMT_CSR, kernel=ax.KER_STREAM)
+    assert p.info()["name"] == f"csr_vector_g{lanes}", (m.nnz / m.nrows, p.info()["name"])
+    x = ax.uniform(18, m.ncols)
+    assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
+
+
 @pytest.mark.parametrize("gen,split", [
     (lambda: ax.banded(300000, 16, 0.5, seed=9), False),    # span of a tile ~ 16k + 33 columns: 14 + 15 bits
     (lambda: ax.random_sparse(300000, 1, 12, seed=10), True),  # span ~ 300k columns: 14 + 19 bits
