@@ -88,7 +88,9 @@ def rocsparse_rows(m, dx, yref, iters: int = 50):
     """rocSPARSE's CSR SpMV on the same matrix and x, each of its CSR
     algorithms (the counterpart of the reference's cuSPARSE row,
     cudaSpmv.cu:629-704): a comparator built into librocsparse_cmp.so,
-    never a library path.  None when that library is not built."""
+    never a library path.  None when that library is not built.  Call it
+    from a process that has not imported torch (torch ships its own
+    rocSPARSE build, which librocsparse.so.1 could then resolve to)."""
     import ctypes as C
     if not os.path.exists(CMP_LIB):
         return None

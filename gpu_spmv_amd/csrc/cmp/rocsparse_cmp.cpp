@@ -154,7 +154,14 @@ extern "C" int cmp_rocsparse_spmv(uint32_t nrows, uint32_t ncols, uint32_t nnz, 
     rocsparse_spmat_descr A = nullptr;
     rocsparse_dnvec_descr X = nullptr, Y = nullptr;
     const rocsparse_datatype f64 = rocsparse_datatype_f64_r;
-    CMP_SP(rocsparse_create_handle(&h));
+    // an error left pending by earlier HIP calls of the process is reported
+    // (and cleared) rather than mistaken for rocSPARSE's own
+    const hipError_t stale = hipGetLastError();
+    if (rocsparse_create_handle(&h) != rocsparse_status_success) {
+        std::snprintf(g_err, sizeof g_err, "rocsparse_create_handle failed (pending HIP error before it: %d %s)",
+                      int(stale), hipGetErrorName(stale));
+        return -1;
+    }
     CMP_SP(rocsparse_create_csr_descr(&A, nrows, ncols, nnz, const_cast<uint32_t *>(d_row),
                                       const_cast<uint32_t *>(d_col), const_cast<double *>(d_val),
                                       rocsparse_indextype_i32, rocsparse_indextype_i32, rocsparse_index_base_zero,
