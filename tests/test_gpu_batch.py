@@ -238,3 +238,42 @@ def test_plan_graph_replays():
                 assert rel_err(y_eager, y) <= TOL, (fmt, rep)
             else:
                 assert np.array_equal(y, y_eager), (fmt, rep)
+
+
+@pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_RCT, dict(tile_rows=2048, slices=4)),
+                                    (ax.FMT_RCT, dict(tile_rows=1024, slices=1))], ids=["axt", "rct_s4", "rct_s1"])
+def test_plan_graph_spmm_and_fwd_adj(fmt, kw):
+    """Graphs of SpMM and of the forward + adjoint pass (fused and split):
+    every replay, with eager work in between, gives the eager result (RCT
+    within the tolerance: unordered sums; v_bar exactly r[row] * x[col] of
+    the r that replay wrote)."""
+    m = ax.powerlaw(1 << 14, 2.0, 1024, 20.0, seed=7)
+    n, nc, nnz = m.nrows, m.ncols, m.nnz
+    p = ax.Plan(m, fmt, flags=ax.PLAN_TRANSPOSE, **kw)
+    k = 16
+    X = ax.uniform(89, nc * k).reshape(nc, k)
+    dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * n * k)
+    Y_ref = orc.spmm(m, X)
+    g = ax.Graph.of_plan(p, ax._lib.OP_SPMM, repeats=2, k=k, X=dX, Y=dY)
+    for rep in range(3):
+        dY.upload(np.full(n * k, np.nan))
+        g.launch()
+        ax._lib.call("axspmv_device_sync")
+        assert rel_err(Y_ref, dY.download(np.float64, n * k).reshape(n, k)) <= TOL, rep
+    x, b = ax.uniform(90, nc), ax.uniform(91, n, stream=ax.VEC_STREAM + 1)
+    dx, db = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer.from_array(b)
+    outs = dict(y=ax.DeviceBuffer(8 * n), r=ax.DeviceBuffer(8 * n), xbar=ax.DeviceBuffer(8 * nc),
+                vbar=ax.DeviceBuffer(8 * nnz))
+    r_ref = orc.ncsr(m, x) - b
+    xbar_ref = orc.ncsr_t(m, r_ref, nc)
+    for op in (ax._lib.OP_FWD_ADJ, ax._lib.OP_FWD_ADJ_SPLIT):
+        g = ax.Graph.of_plan(p, op, repeats=2, x=dx, b=db, **outs)
+        for rep in range(3):
+            for name, size in (("y", n), ("r", n), ("xbar", nc), ("vbar", nnz)):
+                outs[name].upload(np.full(size, np.nan))
+            g.launch()
+            ax._lib.call("axspmv_device_sync")
+            r = outs["r"].download(np.float64, n)
+            assert rel_err(r_ref, r) <= TOL, (op, rep)
+            assert rel_err(xbar_ref, outs["xbar"].download(np.float64, nc)) <= 1e-11, (op, rep)
+            assert np.array_equal(p.vbar_csr(outs["vbar"].download(np.float64, nnz)), orc.vbar(m, r, x)), (op, rep)
