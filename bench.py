@@ -445,8 +445,9 @@ def suite_full(res):
 
 
 def fwd_adj_rct(m, dx, db, yref, b):
-    """Forward + adjoint on RCT plans: SpMV, residual, SpMV of the transposed
-    RCT plan, value gradient in entry order (four passes), against plain RCT."""
+    """Forward + adjoint on RCT plans (the fused form: SpMV, residual, one
+    sweep over the RCT plan of A^T adding x_bar and writing v_bar), against
+    plain RCT."""
     from oracle import oracle as orc
     n, nc, nnz = m.nrows, m.ncols, m.nnz
     p = ax.Plan(m, ax.FMT_RCT, flags=ax.PLAN_TRANSPOSE)
