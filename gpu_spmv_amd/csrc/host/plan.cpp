@@ -227,9 +227,13 @@ int plan_axt_com(Plan &p, const axspmv_csr *a, hipStream_t s) {
     AX_TRY(p.upload(p.d_sec, m->sec, 4ull * m->lenSEC, s));
     const uint32_t nnz = a->nnz;
     // Wave ranges: th == 1 in slots, a multiple of bs and of 256 (4 slots x
-    // 64 lanes); th > 1 in tile columns, 4 passes of 64 columns.
-    const uint32_t wr = th == 1 ? std::max<uint32_t>(p.o.bs, 1024) : 256;
+    // 64 lanes); th > 1 in tile columns, 4 passes of 64 columns.  A small
+    // matrix gets shorter ranges (down to one pass) until about 2048 waves
+    // spread over the 1024 SIMDs.
     const uint32_t units = th == 1 ? nnz : uint32_t(ceil_div(nnz, th));
+    uint32_t wr = th == 1 ? std::max<uint32_t>(p.o.bs, 1024) : 256;
+    const uint32_t wr_min = th == 1 ? std::max<uint32_t>(p.o.bs, 256) : 64;
+    while (wr > wr_min && units / wr < 2048) wr /= 2;
     std::vector<launch::ComWave> waves;
     for (uint64_t b = 0; b < units; b += wr) {
         launch::ComWave w{uint32_t(b), uint32_t(std::min<uint64_t>(b + wr, units)), 0, 0};
