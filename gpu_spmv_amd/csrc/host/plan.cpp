@@ -696,8 +696,15 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     }
     uint32_t block = p.o.block;
     // 512 threads (8 waves, one workgroup per CU with R = 16384) measured
-    // fastest on the 2^20-row power-law matrix; 1024 is 2-3 % slower
-    if (block == 0 || block > 1024 || block % 64) block = 512;
+    // fastest on the 2^20-row power-law matrix (82K entries per tile; 1024 is
+    // 2-3 % slower); tiles of few entries want fewer lanes (graph-timed: the
+    // Laplacian on 256^2, 1.3K per tile, 3.69 -> 3.05 us at 128; a 2^16-row
+    // power-law matrix, 5.1K per tile, 9.17 -> 7.83 us at 256).
+    if (block == 0) {
+        const uint64_t per_tile = nnz / std::max<uint64_t>(1, ntile);
+        block = per_tile < 2048 ? 128 : per_tile < 16384 ? 256 : 512;
+    }
+    if (block > 1024 || block % 64) block = 512;
     p.rct = launch::RctArgs{dv, dc, dl, dk, dcb, dt, drb, n, R, S, nrb, p.o.vtype, block, rbits};
     p.slots = nnz;
     // matrix stream, and with S > 1 the clear of y and S atomic adds per row

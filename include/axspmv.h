@@ -236,8 +236,9 @@ typedef struct axspmv_opts {
     uint32_t reorder; /* K1: 1 = sort rows by length (default), 0 = natural */
     uint32_t vtype;   /* enum axspmv_vtype: value storage */
     uint32_t block;   /* threads per workgroup (0 = kernel default; the
-                         RCT default options keep 0: 512 for SpMV, 1024
-                         for SpMM) */
+                         RCT default options keep 0: for SpMV 512, or 256 /
+                         128 for tiles of under 16K / 2K entries; 1024 for
+                         SpMM) */
     uint32_t budget;  /* units per wave block for segmented kernels (0 = auto) */
     uint32_t flags;   /* AXSPMV_PLAN_* */
     uint32_t ktile;   /* SpMM k-tile: right-hand sides per lane group, 8/16/32/64
