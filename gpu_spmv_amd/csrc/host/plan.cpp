@@ -351,16 +351,14 @@ int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
         if (rl == 0) empty.push_back(r);
         units[r] = th == 1 ? uint32_t(ceil_div(rl, thw) * (thw / K)) : uint32_t(ceil_div(rl, th));
     }
-    // Budget in units: 4 passes of 64 lanes per wave block by default; with
-    // th == 1 and fewer units than 2048 such blocks, 64, so that the small
-    // matrix still spreads over the 1024 SIMDs (the Laplacian on 256^2 with
-    // thw 2 / 4 / 8: 3.9-4.1 us a graph-timed SpMV at 256 units, 2.6-2.7 us
-    // at 64, 3.0-3.2 at 32 and at 128).
-    uint32_t budget = p.o.budget;
-    if (!budget) {
-        budget = 256;
-        if (th == 1 && std::accumulate(units.begin(), units.end(), uint64_t(0)) < 2048ull * 256) budget = 64;
-    }
+    // Budget in units per wave block: 64 (one pass of the wave) by default.
+    // Measured against 256 (four passes): the 2^20-row power-law matrix th =
+    // 1 / 4 / 8: 218 / 225 / 234 us against 227 / 240 / 271 us; the permuted
+    // 27-point stencil on 128^3 th = 1 / 4: 801 / 810 against 812 / 839 us;
+    // the 256^2 Laplacian (graph-timed) thw 4: 2.6 against 4.1 us.  Shorter
+    // blocks balance better and, on small matrices, spread over the 1024
+    // SIMDs; a banded matrix at th = 4 is the exception (5 % slower).
+    const uint32_t budget = p.o.budget ? p.o.budget : 64;
     const std::vector<launch::Block> blocks = row_blocks(units, budget);
     AX_TRY(upload_blocks(p, blocks, empty, s));
     // On request, banded / stencil structure: blocks whose columns fit a

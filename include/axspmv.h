@@ -239,7 +239,8 @@ typedef struct axspmv_opts {
                          RCT default options keep 0: for SpMV 512, or 256 /
                          128 for tiles of under 16K / 2K entries; 1024 for
                          SpMM) */
-    uint32_t budget;  /* units per wave block for segmented kernels (0 = auto) */
+    uint32_t budget;  /* units per wave block for the AXT UNC row-block kernels
+                         (0 = 64, one pass of the wave; batches: 256) */
     uint32_t flags;   /* AXSPMV_PLAN_* */
     uint32_t ktile;   /* SpMM k-tile: right-hand sides per lane group, 8/16/32/64
                          (0 = smallest of those >= min(k, 64)) */
