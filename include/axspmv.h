@@ -229,9 +229,10 @@ typedef struct axspmv_opts {
     uint32_t thw;     /* AXT tile half width (0 = 64, or 4 for UNC th==1) */
     uint32_t th;      /* AXT tile height (0 = 1) */
     uint32_t bs;      /* AXT COM th==1 block size (0 = 256) */
-    uint32_t hbs;     /* AXC brick size (0 = 64); CSR vector kernel: lanes
-                         per row, a power of two <= 64 (0 = the one nearest
-                         the mean row length, the CSR default) */
+    uint32_t hbs;     /* AXC brick size; CSR vector kernel: lanes per row.
+                         A power of two <= 64; 0 (the CSR and AXC default) =
+                         the power of two g in [2, 64] with 1.5 g >= the
+                         mean row length */
     uint32_t chunk;   /* K1 rows per chunk (0 = 64) */
     uint32_t reorder; /* K1: 1 = sort rows by length (default), 0 = natural */
     uint32_t vtype;   /* enum axspmv_vtype: value storage */

@@ -170,14 +170,15 @@ def test_rct_auto_slices(gen, slices):
                                        (lambda: ax.powerlaw(1 << 14, 2.0, 1024, 20.0, seed=1), 16),
                                        (lambda: ax.banded(20000, 32, 0.9, seed=4), 64)],
                          ids=["laplace_mean5", "powerlaw_mean20", "banded_mean58"])
-def test_csr_vector_default_lanes(gen, lanes):
-    """The CSR vector kernel's default lanes per row follow the mean row
-    length (the power of two g with 1.5 g >= mean, within [2, 64])."""
+def test_default_lanes_per_row(gen, lanes):
+    """The CSR vector kernel's lanes per row and AXC's brick size default to
+    the power of two g with 1.5 g >= the mean row length, within [2, 64]."""
     m = gen()
-    p = ax.Plan(m, ax.FMT_CSR, kernel=ax.KER_STREAM)
-    assert p.info()["name"] == f"csr_vector_g{lanes}", (m.nnz / m.nrows, p.info()["name"])
     x = ax.uniform(18, m.ncols)
-    assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
+    for p, name in ((ax.Plan(m, ax.FMT_CSR, kernel=ax.KER_STREAM), f"csr_vector_g{lanes}"),
+                    (ax.Plan(m, ax.FMT_AXC), f"axc_hbs{lanes}")):
+        assert p.info()["name"] == name, (m.nnz / m.nrows, p.info()["name"])
+        assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
 
 
 @pytest.mark.parametrize("gen,split", [
