@@ -556,6 +556,8 @@ bool narrow_bands(const axspmv_csr *a, uint32_t rows, uint64_t cols) {
 // RCT: row blocks of at most tile_rows rows and about equal nonzeros, column
 // slices, entries of a tile sorted by (column, row).
 int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
+    if (p.o.block > 1024 || p.o.block % 64)
+        return fail(AXSPMV_E_INVALID, "RCT: block %u is not a multiple of 64 up to 1024", p.o.block);
     hipDeviceProp_t prop;
     size_t lds_max = 65536;
     if (hipGetDeviceProperties(&prop, p.device) == hipSuccess && prop.sharedMemPerBlock > 0)
@@ -709,7 +711,7 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         const uint64_t per_tile = nnz / std::max<uint64_t>(1, ntile);
         block = per_tile < 2048 ? 128 : per_tile < 16384 ? 256 : 512;
     }
-    if (block > 1024 || block % 64) block = 512;
+
     p.rct = launch::RctArgs{dv, dc, dl, dk, dcb, dt, drb, n, R, S, nrb, p.o.vtype, block, rbits};
     p.slots = nnz;
     // matrix stream, and with S > 1 the clear of y and S atomic adds per row

@@ -36,7 +36,8 @@ from .core import (TIME_REGION, CsrMatrix, DeviceBuffer, Plan, banded, laplace2d
 BLOCKS = (32, 64, 128, 256, 512, 1024)
 
 # (label, format, options, block sensitive); block-sensitive candidates are
-# run once per block size, the others once (their kernels fix the workgroup).
+# run once per block size they accept (RCT: multiples of 64), the others
+# once (their kernels fix the workgroup).
 CANDIDATES = [
     ("csr_naive", L.FMT_CSR, dict(kernel=L.KER_NAIVE), True),
     ("csr_vector", L.FMT_CSR, dict(kernel=L.KER_STREAM), False),
@@ -88,7 +89,8 @@ def sweep_matrix(name: str, m: CsrMatrix, blocks=BLOCKS, candidates=CANDIDATES, 
     yref = check(m, x) if check is not None else None
     rows = []
     for label, fmt, kw, by_block in candidates:
-        for bs in (blocks if by_block else (0,)):
+        ok_blocks = tuple(b for b in blocks if fmt != L.FMT_RCT or b % 64 == 0) or (0,)
+        for bs in (ok_blocks if by_block else (0,)):
             opts = dict(kw)
             if bs:
                 opts["block"] = bs

@@ -166,6 +166,13 @@ def test_rct_auto_slices(gen, slices):
     assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
 
 
+def test_rct_rejects_invalid_block():
+    m = MATS["lap16"]
+    for b in (32, 96, 2048):
+        with pytest.raises(ax.AxspmvError, match="multiple of 64"):
+            ax.Plan(m, ax.FMT_RCT, block=b)
+
+
 @pytest.mark.parametrize("gen,lanes", [(lambda: ax.laplace2d(128), 4),
                                        (lambda: ax.powerlaw(1 << 14, 2.0, 1024, 20.0, seed=1), 16),
                                        (lambda: ax.banded(20000, 32, 0.9, seed=4), 64)],
