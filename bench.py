@@ -17,8 +17,11 @@ adjoint / SpMM / FP32-value forms against plain SpMV.
                                    stencil, the 256-matrix batch and the
                                    forward + adjoint on the stencil structure
   python bench.py --calibrate      copy bandwidth and the x-footprint sweep
-  python bench.py --pmc-probe      a 256 MB copy then 5 flagship SpMVs (for
-                                   rocprofv3 --pmc runs)
+  python bench.py --pmc-probe [stencil]
+                                   a 256 MB copy then 5 SpMVs of each flagship
+                                   kernel on the power-law matrix (or on the
+                                   permuted 27-point stencil), for rocprofv3
+                                   --pmc runs
 """
 from __future__ import annotations
 
@@ -339,15 +342,20 @@ def calibrate():
     return res
 
 
-def pmc_probe():
-    """A 256 MB copy (known bytes: the counter scale) then 5 flagship SpMVs
-    of each of the two flagship kernels."""
+def pmc_probe(matrix: str = "powerlaw"):
+    """A 256 MB copy (known bytes: the counter scale) then 5 SpMVs of each of
+    the two flagship kernels, on the 2^20-row power-law matrix or on the
+    permuted 27-point stencil on 128^3."""
     n = 32 << 20
     src, dst = ax.DeviceBuffer(8 * n), ax.DeviceBuffer(8 * n)
     L.call("axspmv_copy_f64", src.ptr, dst.ptr, n, None)
     L.call("axspmv_device_sync")
-    m = ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
-    x = ax.uniform(1, m.ncols)
+    if matrix == "stencil":
+        m = ax.stencil27(128, permute=True, seed=2)
+        x = ax.uniform(2, m.ncols)
+    else:
+        m = ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
+        x = ax.uniform(1, m.ncols)
     dx, dy = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer(8 * m.nrows)
     for _, fmt, kw in (FLAGSHIP_AXT, FLAGSHIP):
         p = ax.Plan(m, fmt, **kw)
@@ -497,7 +505,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
-    ap.add_argument("--pmc-probe", action="store_true")
+    ap.add_argument("--pmc-probe", nargs="?", const="powerlaw", choices=["powerlaw", "stencil"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: SpMV here runs on a single MI355X; "
@@ -507,7 +515,7 @@ def main():
         print(json.dumps(calibrate()), flush=True)
         return
     if args.pmc_probe:
-        pmc_probe()
+        pmc_probe(args.pmc_probe)
         return
 
     t0 = time.perf_counter()
