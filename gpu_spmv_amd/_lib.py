@@ -181,6 +181,7 @@ SIGNATURES = {
     "axspmv_device_count": (C.c_int, [P(C.c_int)]),
     "axspmv_copy_f64": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "axspmv_x_refresh": (C.c_int, [vp, u64, u32, u32, vp, vp, vp]),
+    "axspmv_wave_selftest": (C.c_int, [u32, vp, vp, vp, vp, u64, vp]),
 }
 
 _lib = None

@@ -1,6 +1,6 @@
 // Utility kernels: a plain copy (the bandwidth reference used to calibrate
-// the HBM byte counters), fills, and the x-half refresh of the interleaved
-// ax layouts.
+// the HBM byte counters), fills, the x-half refresh of the interleaved ax
+// layouts, and a lane-by-lane exercise of the wave64 primitives.
 #include "launch.hpp"
 #include "wave64.hpp"
 
@@ -57,6 +57,37 @@ unsigned grid_for(size_t work, unsigned block) {
     return g ? unsigned(g) : 1u;
 }
 
+// One lane per element, whole waves (n a multiple of 64): applies the wave64
+// primitive that op names (enum axspmv_wave_op) to v / h and stores the
+// lane's results, so each primitive can be checked against its definition.
+__global__ void __launch_bounds__(256) wave_selftest_kernel(uint32_t op, const double *__restrict__ v,
+                                                           const uint32_t *__restrict__ h, double *__restrict__ ov,
+                                                           uint32_t *__restrict__ oh, size_t n) {
+    using namespace axspmv::dev;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;  // n is a multiple of 64: whole waves leave together
+    double a = v[i];
+    uint32_t b = h[i];
+    switch (op) {  // op is uniform: every lane of the wave takes the same case
+    case 0: a = seg_scan_f64(a, b); break;
+    case 1: b = scan_max_u32(b); break;
+    case 2:
+        a = shift_up_f64(a, -1.0);
+        b = shift_up_u32(b, 0xffffffffu);
+        break;
+    case 3: b = shift_down_u32(b, 0xffffffffu); break;
+    case 4: a = group_sum_f64<1>(a); break;
+    case 5: a = group_sum_f64<2>(a); break;
+    case 6: a = group_sum_f64<4>(a); break;
+    case 7: a = group_sum_f64<8>(a); break;
+    case 8: a = group_sum_f64<16>(a); break;
+    case 9: a = group_sum_f64<32>(a); break;
+    default: a = group_sum_f64<64>(a); break;
+    }
+    ov[i] = a;
+    oh[i] = b;
+}
+
 }  // namespace
 
 namespace axspmv {
@@ -78,6 +109,12 @@ hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s) {
 hipError_t x_refresh(const uint32_t *map, uint64_t n, uint32_t hw_log, uint32_t nx, const double *x, double *ax,
                      hipStream_t s) {
     if (n) x_refresh_kernel<<<grid_for(n, 256), 256, 0, s>>>(map, n, hw_log, nx, x, ax);
+    return hipGetLastError();
+}
+
+hipError_t wave_selftest(uint32_t op, const double *v, const uint32_t *h, double *ov, uint32_t *oh, size_t n,
+                         hipStream_t s) {
+    if (n) wave_selftest_kernel<<<grid_for(n, 256), 256, 0, s>>>(op, v, h, ov, oh, n);
     return hipGetLastError();
 }
 

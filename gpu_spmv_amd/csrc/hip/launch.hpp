@@ -206,6 +206,9 @@ hipError_t copy_f64(const double *src, double *dst, size_t n, hipStream_t s);
 hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s);
 // reads p[0 .. n) (n even, p 16-byte aligned): evicts caches without dirtying them
 hipError_t touch(const double *p, size_t n, double *sink, hipStream_t s);
+// the wave64 primitive op (enum axspmv_wave_op) lane by lane; n a multiple of 64
+hipError_t wave_selftest(uint32_t op, const double *v, const uint32_t *h, double *ov, uint32_t *oh, size_t n,
+                         hipStream_t s);
 // y[rows[i]] = 0 for i < n
 hipError_t fill_rows(const uint32_t *rows, uint32_t n, double *y, hipStream_t s);
 

@@ -67,6 +67,16 @@ extern "C" int axspmv_x_refresh(const uint32_t *d_map, uint64_t nslots, uint32_t
     return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "x_refresh launch");
 }
 
+extern "C" int axspmv_wave_selftest(uint32_t op, const double *d_v, const uint32_t *d_h, double *d_ov,
+                                    uint32_t *d_oh, uint64_t n, void *stream) {
+    if (op > AXSPMV_WAVE_GROUP_SUM_64) return fail(AXSPMV_E_INVALID, "wave_selftest: unknown op %u", op);
+    if (n % 64) return fail(AXSPMV_E_INVALID, "wave_selftest: %llu elements are not whole waves of 64",
+                            (unsigned long long)n);
+    if (n && (!d_v || !d_h || !d_ov || !d_oh)) return fail(AXSPMV_E_INVALID, "wave_selftest: NULL argument");
+    const hipError_t e = launch::wave_selftest(op, d_v, d_h, d_ov, d_oh, n, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "wave_selftest launch");
+}
+
 namespace {
 
 struct Events {

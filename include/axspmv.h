@@ -538,6 +538,31 @@ int axspmv_copy_f64(const double *d_src, double *d_dst, size_t n, void *stream);
 int axspmv_x_refresh(const uint32_t *d_map, uint64_t nslots, uint32_t hw, uint32_t nx, const double *d_x,
                      double *d_ax, void *stream);
 
+/* The wave64 cross-lane primitives that replace the reference's 32-lane
+ * __shfl_* reduction trees (cudaSpmv.cu:1086-1090, 1586-1594) and its
+ * Hillis-Steele scans (cudaSpmv.cu:2061-2086, scan.cu:239-414), run lane by
+ * lane so that a caller can check them: element i is lane i % 64 of a wave,
+ * n a multiple of 64, and for each op
+ *   SEG_SCAN     ov = inclusive sum of v from the lane's segment start (the
+ *                last lane <= i of its wave with h != 0, else lane 0);
+ *                oh != 0 iff some lane <= i of its wave has h != 0
+ *   SCAN_MAX     oh = max of h over lanes 0..i of the wave; ov = v
+ *   SHIFT_UP     ov, oh = v, h of lane i - 1; lane 0 gets -1.0, 0xffffffff
+ *   SHIFT_DOWN   oh = h of lane i + 1; lane 63 gets 0xffffffff; ov = v
+ *   GROUP_SUM_G  ov = sum of v over the aligned group of G lanes holding
+ *                lane i (G = 1 << (op - AXSPMV_WAVE_GROUP_SUM_1)); oh = h
+ * Test hook only: no SpMV path calls it. */
+enum axspmv_wave_op {
+    AXSPMV_WAVE_SEG_SCAN = 0,
+    AXSPMV_WAVE_SCAN_MAX = 1,
+    AXSPMV_WAVE_SHIFT_UP = 2,
+    AXSPMV_WAVE_SHIFT_DOWN = 3,
+    AXSPMV_WAVE_GROUP_SUM_1 = 4,
+    AXSPMV_WAVE_GROUP_SUM_64 = 10
+};
+int axspmv_wave_selftest(uint32_t op, const double *d_v, const uint32_t *d_h, double *d_ov, uint32_t *d_oh,
+                         uint64_t n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

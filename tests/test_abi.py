@@ -43,3 +43,14 @@ def test_abi_version_and_error_path():
     rc = lib.axspmv_csr_check(None)
     assert rc == L.E_INVALID
     assert b"NULL" in lib.axspmv_last_error()
+
+
+def test_wave_selftest_rejects_bad_arguments():
+    """Checked before any device call: partial waves, unknown ops, NULL."""
+    lib = L.load()
+    assert lib.axspmv_wave_selftest(0, None, None, None, None, 65, None) == L.E_INVALID
+    assert b"whole waves" in lib.axspmv_last_error()
+    assert lib.axspmv_wave_selftest(11, None, None, None, None, 64, None) == L.E_INVALID
+    assert b"unknown op" in lib.axspmv_last_error()
+    assert lib.axspmv_wave_selftest(0, None, None, None, None, 64, None) == L.E_INVALID
+    assert b"NULL" in lib.axspmv_last_error()
