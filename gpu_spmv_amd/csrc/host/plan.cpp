@@ -666,7 +666,9 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         }
     const uint32_t rbits = bits(R), cbits = bits(span);
     const bool packed = rbits + cbits <= 32 && !(p.o.flags & AXSPMV_PLAN_RCT_SPLIT);
-    void *dv = p.dalloc(vs * nnz);
+    // one spare entry when nnz is odd: a kernel may load entries in aligned pairs
+    const uint64_t nnz_pad = nnz + (nnz & 1);
+    void *dv = p.dalloc(vs * nnz_pad);
     auto *dt = static_cast<uint32_t *>(p.dalloc(4 * tptr.size()));
     auto *drb = static_cast<uint32_t *>(p.dalloc(4 * rbs.size()));
     if (!dv || !dt || !drb) return AXSPMV_E_NOMEM;
@@ -689,14 +691,14 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
 #pragma omp parallel for schedule(dynamic, 1)
         for (uint64_t t = 0; t < ntile; ++t)
             for (uint64_t q = cnt[t]; q < cnt[t + 1]; ++q) key[q] = ((col[q] - cbase[t]) << rbits) | lrow[q];
-        dk = static_cast<uint32_t *>(p.dalloc(4 * nnz));
+        dk = static_cast<uint32_t *>(p.dalloc(4 * nnz_pad));
         dcb = static_cast<uint32_t *>(p.dalloc(4 * ntile));
         if (!dk || !dcb) return AXSPMV_E_NOMEM;
         AX_TRY(p.upload(dk, key.data(), 4 * nnz, s));
         AX_TRY(p.upload(dcb, cbase.data(), 4 * ntile, s));
     } else {
-        dc = static_cast<uint32_t *>(p.dalloc(4 * nnz));
-        dl = static_cast<uint16_t *>(p.dalloc(2 * nnz));
+        dc = static_cast<uint32_t *>(p.dalloc(4 * nnz_pad));
+        dl = static_cast<uint16_t *>(p.dalloc(2 * nnz_pad));
         if (!dc || !dl) return AXSPMV_E_NOMEM;
         AX_TRY(p.upload(dc, col.data(), 4 * nnz, s));
         AX_TRY(p.upload(dl, lrow.data(), 2 * nnz, s));
