@@ -136,6 +136,9 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
         // SpMV: each lane takes entries in aligned pairs (2q, 2q + 1), so a
         // value load moves 16 bytes and a key load 8 (1-4 % faster than one
         // entry per load on the power-law, stencil and banded matrices).
+        // The VB variant below keeps one entry per lane: paired, its value
+        // gradient stores (8 bytes at a 16-byte stride, as a pair may hold
+        // an entry of the next tile) made forward + adjoint 3 % slower.
         // Pairs pb .. pl cover the tile and each holds at least one of its
         // entries; an entry outside the tile borrows its partner's column and
         // row (both in the tile) and adds nothing.
