@@ -454,8 +454,8 @@ def suite_full(res):
 
 def fwd_adj_rct(m, dx, db, yref, b):
     """Forward + adjoint on RCT plans (the fused form: SpMV, residual, one
-    sweep over the RCT plan of A^T adding x_bar and writing v_bar), against
-    plain RCT."""
+    sweep over the RCT plan of A^T adding x_bar and writing v_bar; the split
+    form: SpMV, residual, adjoint, value gradient), against plain RCT."""
     from oracle import oracle as orc
     n, nc, nnz = m.nrows, m.ncols, m.nnz
     p = ax.Plan(m, ax.FMT_RCT, flags=ax.PLAN_TRANSPOSE)
@@ -463,9 +463,10 @@ def fwd_adj_rct(m, dx, db, yref, b):
     p.time(dx, dy, iters=3, warmup=0, mode=TIME_REGION)
     plain = p.time(dx, dy, iters=20, warmup=0, mode=TIME_REGION).mean_ms * 1e3
     bufs = dict(y=dy, r=ax.DeviceBuffer(8 * n), xbar=ax.DeviceBuffer(8 * nc), vbar=ax.DeviceBuffer(8 * nnz))
+    split = p.time_op(L.OP_FWD_ADJ_SPLIT, iters=20, warmup=2, x=dx, b=db, **bufs)
     t = p.time_op(L.OP_FWD_ADJ, iters=20, warmup=2, x=dx, b=db, **bufs)
     out = dict(plan=p.info()["name"], plain_spmv_us=round(plain, 2), fwd_adj_us=round(t.mean_ms * 1e3, 2),
-               fwd_adj_over_plain=round(t.mean_ms * 1e3 / plain, 3))
+               split_us=round(split.mean_ms * 1e3, 2), fwd_adj_over_plain=round(t.mean_ms * 1e3 / plain, 3))
     if yref is not None:
         r_ref = yref - b
         out["r_rel_err"] = rel_err(r_ref, bufs["r"].download(np.float64, n))
