@@ -24,7 +24,8 @@
 #include "launch.hpp"
 #include "wave64.hpp"
 
-// Entries per lane per round (each of the two register sets holds this many).
+// Entries per lane per round (each of the two register sets holds this many,
+// as RCT_UNROLL / 2 pairs in the tile kernel).
 #ifndef RCT_UNROLL
 #define RCT_UNROLL 4
 #endif
@@ -123,144 +124,114 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
     const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.rbits,
                                      PACKED ? a.cbase[t] : 0u};
-    // Rounds of U entries per lane, in two register sets A and B: a round
-    // issues its x gathers, then the loads of the next round into the other
-    // set, then waits for the gathers only and does its LDS adds.  Loads are
-    // never predicated (a lane past the end re-reads the tile's last entry)
-    // and the product of such a lane is zeroed.  The round count is the same
-    // for the whole workgroup, and the last round (which fetches nothing) is
-    // peeled, so every fetch is used on every path and the compiler has no
-    // reason to sink it behind the LDS adds.
-    constexpr int U = RCT_UNROLL;
-    if constexpr (!VB) {
-        // SpMV: each lane takes entries in aligned pairs (2q, 2q + 1), so a
-        // value load moves 16 bytes and a key load 8 (1-4 % faster than one
-        // entry per load on the power-law, stencil and banded matrices).
-        // The VB variant below keeps one entry per lane: paired, its value
-        // gradient stores (8 bytes at a 16-byte stride, as a pair may hold
-        // an entry of the next tile) made forward + adjoint 3 % slower.
-        // Pairs pb .. pl cover the tile and each holds at least one of its
-        // entries; an entry outside the tile borrows its partner's column and
-        // row (both in the tile) and adds nothing.
-        constexpr int UP = (U + 1) / 2;
-        struct PSet {
-            uint32_t c[UP][2], r[UP][2];
-            double v[UP][2];
-            bool ok[UP][2];
-        };
-        if (beg < end) {
-            const uint32_t pb = beg >> 1, pl = (end - 1) >> 1, span = UP * nt;
-            const uint32_t nr = (pl - pb + span) / span;
-            auto fetch = [&](PSet &S, uint32_t q0) {
-#pragma unroll
-                for (int j = 0; j < UP; ++j) {
-                    const uint32_t q = q0 + j * nt, qc = min(q, pl);
-                    ent.load2(qc, S.c[j], S.r[j], S.v[j]);
-                    const bool in0 = 2 * qc >= beg, in1 = 2 * qc + 1 < end;
-                    const uint32_t c0 = S.c[j][0], r0 = S.r[j][0], c1 = S.c[j][1], r1 = S.r[j][1];
-                    S.c[j][0] = in0 ? c0 : c1;
-                    S.r[j][0] = in0 ? r0 : r1;
-                    S.c[j][1] = in1 ? c1 : c0;
-                    S.r[j][1] = in1 ? r1 : r0;
-                    S.ok[j][0] = q == qc && in0;  // a lane past the end adds nothing
-                    S.ok[j][1] = q == qc && in1;
-                }
-            };
-            auto gather = [&](const PSet &S, double (&xv)[UP][2]) {
-#pragma unroll
-                for (int j = 0; j < UP; ++j) {
-                    xv[j][0] = x[S.c[j][0]];
-                    xv[j][1] = x[S.c[j][1]];
-                }
-            };
-            auto add = [&](const PSet &S, const double (&xv)[UP][2]) {
-#pragma unroll
-                for (int j = 0; j < UP; ++j)
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) atomicAdd(&yl[S.r[j][h]], S.ok[j][h] ? S.v[j][h] * xv[j][h] : 0.0);
-            };
-            auto round = [&](const PSet &S, PSet &N, uint32_t q0) {
-                double xv[UP][2];
-                gather(S, xv);
-                fetch(N, q0 + span);
-                add(S, xv);
-            };
-            auto final_round = [&](const PSet &S) {
-                double xv[UP][2];
-                gather(S, xv);
-                add(S, xv);
-            };
-            PSet A, B;
-            uint32_t q = pb + tid;
-            fetch(A, q);
-            uint32_t k = 1;  // rounds k - 1 and k run in one pass; both have a successor
-            for (; k + 1 < nr; k += 2) {
-                round(A, B, q);
-                round(B, A, q + span);
-                q += 2 * span;
-            }
-            if (k < nr) {
-                round(A, B, q);
-                final_round(B);
-            } else {
-                final_round(A);
-            }
-        }
-    }
-    struct Set {
-        uint32_t c[U], r[U];
-        double v[U];
+    // Each lane takes entries in aligned pairs (2q, 2q + 1): a value load
+    // moves 16 bytes and a key load 8, and the value gradient (VB) goes out
+    // as one 16-byte store per pair.  Pairs pb .. pl cover the tile and each
+    // holds at least one of its entries; an entry outside the tile borrows
+    // its partner's column and row (both in the tile) and adds nothing.
+    // Measured against one entry per lane: SpMV 1-4 % faster on the
+    // power-law, stencil and banded matrices, forward + adjoint 2-4 %.
+    //
+    // Rounds of UP pairs per lane, in two register sets A and B: a round
+    // issues its x gathers (and, for VB, its LDS reads of xr), then the loads
+    // of the next round into the other set, then waits for the gathers only
+    // and does its LDS adds.  Loads are never predicated (a lane past the end
+    // re-reads the tile's last pair) and the products of such a lane are
+    // zeroed.  The round count is the same for the whole workgroup, and the
+    // last round (which fetches nothing) is peeled, so every fetch is used on
+    // every path and the compiler has no reason to sink it behind the LDS
+    // adds.
+    //
+    // VB stores are unpredicated too: a lane's two value gradients go to
+    // entries of the tile (an entry outside it, or a lane past the end,
+    // stores its partner's own value again), as one 16-byte store when every
+    // lane of the wave holds a whole pair and vb is 16-byte aligned, else as
+    // two 8-byte stores.
+    constexpr int UP = (RCT_UNROLL + 1) / 2;
+    struct PSet {
+        uint32_t c[UP][2], r[UP][2];
+        uint32_t at[UP][2];  // VB: the entries the two stores write
+        double v[UP][2];
+        bool ok[UP][2];      // adds its product
     };
-    if (VB && beg < end) {
-        const uint32_t last = end - 1, span = U * nt;
-        const uint32_t nr = (end - beg + span - 1) / span;
-        auto fetch = [&](Set &S, uint32_t e0) {
+    if (beg < end) {
+        const uint32_t pb = beg >> 1, pl = (end - 1) >> 1, span = UP * nt;
+        const uint32_t nr = (pl - pb + span) / span;
+        const bool vb16 = (reinterpret_cast<uintptr_t>(vb) & 15) == 0;
+        auto fetch = [&](PSet &S, uint32_t q0) {
 #pragma unroll
-            for (int j = 0; j < U; ++j) ent.load(min(e0 + j * nt, last), S.c[j], S.r[j], S.v[j]);
-        };
-        // VB stores are unpredicated too: a lane past the end holds the
-        // tile's last entry, so it stores that entry's own value again.  The
-        // LDS reads of xr go out with the x gathers, ahead of the wait.
-        auto add = [&](const Set &S, const double *xv, const double *xl, uint32_t e0) {
-#pragma unroll
-            for (int j = 0; j < U; ++j) {
-                atomicAdd(&yl[S.r[j]], e0 + j * nt < end ? S.v[j] * xv[j] : 0.0);
-                if constexpr (VB) vb[min(e0 + j * nt, last)] = xl[j] * xv[j];
+            for (int j = 0; j < UP; ++j) {
+                const uint32_t q = q0 + j * nt, qc = min(q, pl);
+                ent.load2(qc, S.c[j], S.r[j], S.v[j]);
+                const bool in0 = 2 * qc >= beg, in1 = 2 * qc + 1 < end;
+                const uint32_t c0 = S.c[j][0], r0 = S.r[j][0], c1 = S.c[j][1], r1 = S.r[j][1];
+                S.c[j][0] = in0 ? c0 : c1;
+                S.r[j][0] = in0 ? r0 : r1;
+                S.c[j][1] = in1 ? c1 : c0;
+                S.r[j][1] = in1 ? r1 : r0;
+                S.ok[j][0] = q == qc && in0;  // a lane past the end adds nothing
+                S.ok[j][1] = q == qc && in1;
+                if constexpr (VB) {
+                    S.at[j][0] = in0 ? 2 * qc : 2 * qc + 1;
+                    S.at[j][1] = in1 ? 2 * qc + 1 : 2 * qc;
+                }
             }
         };
-        auto gather = [&](const Set &S, double *xv, double *xl) {
+        auto gather = [&](const PSet &S, double (&xv)[UP][2], double (&xl)[UP][2]) {
 #pragma unroll
-            for (int j = 0; j < U; ++j) xv[j] = x[S.c[j]];
+            for (int j = 0; j < UP; ++j) {
+                xv[j][0] = x[S.c[j][0]];
+                xv[j][1] = x[S.c[j][1]];
+            }
             if constexpr (VB) {
 #pragma unroll
-                for (int j = 0; j < U; ++j) xl[j] = xs[S.r[j]];
+                for (int j = 0; j < UP; ++j) {
+                    xl[j][0] = xs[S.r[j][0]];
+                    xl[j][1] = xs[S.r[j][1]];
+                }
             }
         };
-        auto round = [&](const Set &S, Set &N, uint32_t e0) {
-            double xv[U], xl[U];
-            gather(S, xv, xl);
-            fetch(N, e0 + span);
-            add(S, xv, xl, e0);
+        auto add = [&](const PSet &S, const double (&xv)[UP][2], const double (&xl)[UP][2]) {
+#pragma unroll
+            for (int j = 0; j < UP; ++j) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) atomicAdd(&yl[S.r[j][h]], S.ok[j][h] ? S.v[j][h] * xv[j][h] : 0.0);
+                if constexpr (VB) {
+                    const double p0 = xl[j][0] * xv[j][0], p1 = xl[j][1] * xv[j][1];
+                    if (__all(vb16 && S.at[j][1] == S.at[j][0] + 1)) {  // whole pairs (at[j][0] even)
+                        reinterpret_cast<double2 *>(vb)[S.at[j][0] >> 1] = double2{p0, p1};
+                    } else {
+                        vb[S.at[j][0]] = p0;
+                        vb[S.at[j][1]] = p1;
+                    }
+                }
+            }
         };
-        auto final_round = [&](const Set &S, uint32_t e0) {
-            double xv[U], xl[U];
+        auto round = [&](const PSet &S, PSet &N, uint32_t q0) {
+            double xv[UP][2], xl[UP][2];
             gather(S, xv, xl);
-            add(S, xv, xl, e0);
+            fetch(N, q0 + span);
+            add(S, xv, xl);
         };
-        Set A, B;
-        uint32_t e = beg + tid;
-        fetch(A, e);
+        auto final_round = [&](const PSet &S) {
+            double xv[UP][2], xl[UP][2];
+            gather(S, xv, xl);
+            add(S, xv, xl);
+        };
+        PSet A, B;
+        uint32_t q = pb + tid;
+        fetch(A, q);
         uint32_t k = 1;  // rounds k - 1 and k run in one pass; both have a successor
         for (; k + 1 < nr; k += 2) {
-            round(A, B, e);
-            round(B, A, e + span);
-            e += 2 * span;
+            round(A, B, q);
+            round(B, A, q + span);
+            q += 2 * span;
         }
         if (k < nr) {
-            round(A, B, e);
-            final_round(B, e + span);
+            round(A, B, q);
+            final_round(B);
         } else {
-            final_round(A, e);
+            final_round(A);
         }
     }
     __syncthreads();

@@ -262,3 +262,23 @@ def test_rct_fused_gradient_tall_tiles(kw):
     assert np.array_equal(p.vbar_csr(vbar), orc.vbar(m, r, x))
     # the separate kernels: the same entry order, each v_bar from its own r
     assert np.array_equal(p.vbar_csr(res[False][3]), orc.vbar(m, res[False][1], x))
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_rct_vbar_at_8_byte_offset(fused):
+    """The fused sweep writes v_bar pairs with 16-byte stores when v_bar is
+    16-byte aligned; at an 8-byte offset it falls back to 8-byte stores and
+    writes the same values, and nothing outside v_bar[0 .. nnz)."""
+    m = ax.random_sparse(20000, 1, 12, seed=36)
+    n, nc, nnz = m.nrows, m.ncols, m.nnz
+    x, b = ax.uniform(37, nc), ax.uniform(38, n, stream=ax.VEC_STREAM + 1)
+    p = ax.Plan(m, ax.FMT_RCT, flags=ax.PLAN_TRANSPOSE)
+    dx, db = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer.from_array(b)
+    outs = [ax.DeviceBuffer.from_array(np.full(k, np.nan)) for k in (n, n, nc)]
+    big = ax.DeviceBuffer.from_array(np.full(nnz + 2, np.nan))
+    p.fwd_adj(dx, db, *outs, big.ptr + 8, fused=fused)
+    r = outs[1].download(np.float64, n)
+    full = big.download(np.float64, nnz + 2)
+    assert np.isnan(full[0]) and np.isnan(full[-1])
+    assert np.array_equal(p.vbar_csr(full[1:-1]), orc.vbar(m, r, x))
+    assert rel_err(orc.ncsr(m, x) - b, r) <= TOL
