@@ -208,6 +208,31 @@ def test_rct_key_packing(gen, split):
     assert rel_err(orc.spmm(m, X), dY.download(np.float64, m.nrows * 8).reshape(m.nrows, 8)) <= TOL
 
 
+def small_csr(nrows, ncols, nnz, seed):
+    """nnz entries at random (row, column) positions, rows sorted, values
+    small integers (exact sums)."""
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.integers(0, nrows, nnz))
+    row = np.zeros(nrows + 1, np.uint32)
+    np.add.at(row, rows + 1, 1)
+    row = np.cumsum(row).astype(np.uint32)
+    col = rng.integers(0, ncols, nnz).astype(np.uint32)
+    return ax.CsrMatrix(row, col, rng.integers(-9, 10, nnz).astype(np.float64), ncols=ncols)
+
+
+@pytest.mark.parametrize("nnz", [1, 2, 3, 5, 64, 127, 129, 1023, 4097])
+@pytest.mark.parametrize("slices,block,vtype", [(1, 64, ax.F64), (2, 128, ax.F64), (8, 64, ax.F64), (4, 192, ax.F32)])
+def test_rct_entry_pairs(nnz, slices, block, vtype):
+    """The tile kernel loads entries in aligned pairs: tiles that start or end
+    at odd entries, one-entry tiles, an odd entry count (the plan's spare
+    entry is read and ignored), lanes past the end.  Integer values and x
+    make every sum exact, so y must equal the oracle bit for bit."""
+    m = small_csr(300, 700, nnz, seed=nnz)
+    x = np.random.default_rng(nnz + 1).integers(-9, 10, m.ncols).astype(np.float64)
+    p = ax.Plan(m, ax.FMT_RCT, tile_rows=64, slices=slices, block=block, vtype=vtype)
+    np.testing.assert_array_equal(ax.spmv_host(p, x), orc.ncsr(m, x))
+
+
 @pytest.mark.parametrize("kw", [dict(slices=4, tile_rows=512), dict(slices=1, tile_rows=512)])
 def test_rct_unaligned_y(kw):
     """y at an 8-byte offset (the clear of y falls back to 8-byte stores)."""
