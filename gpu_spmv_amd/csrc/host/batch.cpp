@@ -311,8 +311,8 @@ namespace {
 // Built-in candidates.  Batches take AXT UNC per shape group and RCT (the
 // block-diagonal plan) for the whole batch; single plans also try K1 and
 // RCT.  SpMM: AXT
-// k-tiles, and RCT tiles whose rows x k-tile sums fill the LDS (one column
-// slice and the default slicing).
+// k-tiles, and RCT tiles with 64 KB of rows x k-tile sums (one column slice
+// and the default slicing) or 32 KB (one slice).
 std::vector<axspmv_opts> default_candidates(uint32_t op, bool for_batch = false) {
     std::vector<axspmv_opts> c;
     axspmv_opts o;
@@ -335,6 +335,12 @@ std::vector<axspmv_opts> default_candidates(uint32_t op, bool for_batch = false)
                     c.push_back(w);
                 }
             }
+            // half the height, one slice (k = 32 on the 2^20-row power-law
+            // matrix: 256 rows at ktile 16 took 1019 us, 512 rows 1060-1080)
+            axspmv_opts h = o;
+            h.tile_rows = 4096 / kt;
+            h.slices = 1;
+            c.push_back(h);
         }
         return c;
     }

@@ -491,10 +491,11 @@ int axspmv_graph_destroy(axspmv_graph *g);
 /* The built-in candidates: for SpMV AXT UNC th = 1 with thw 2..16 and th =
  * 2..8 with thw 64 at two block budgets, K1 (chunk 64) and RCT (default, 256
  * and 1024 threads); for batches the AXT part and the default RCT options;
- * for SpMM the four AXT k-tiles and eight RCT plans
- * (ktile 8 with 1024 and 2048 rows, 16 with 512, 32 with 256; one column
- * slice and the default slicing).  Writes min(cap, count) options to out and
- * the set's size to *count. */
+ * for SpMM the four AXT k-tiles and eleven RCT plans
+ * (ktile 8 with 1024 and 2048 rows, 16 with 512, 32 with 256, each with one
+ * column slice and with the default slicing; and ktile 8 / 16 / 32 with 512
+ * / 256 / 128 rows and one slice).  Writes min(cap, count) options to out
+ * and the set's size to *count. */
 int axspmv_autotune_defaults(uint32_t op, axspmv_opts *out, uint32_t cap, uint32_t *count);
 int axspmv_autotune(const axspmv_csr *a, uint32_t op, uint32_t k, const axspmv_opts *cand,
                     uint32_t ncand, uint32_t iters, void *stream, uint32_t *best, double *times_ms);
