@@ -332,7 +332,8 @@ int axspmv_plan_vbar_positions(const axspmv_plan *p, uint32_t *h_pos, uint64_t c
  * come first; fused = 1 then produces x_bar and v_bar in one sweep over the
  * plan of A^T (whose tiles keep their x values in LDS beside the x_bar sums;
  * with automatic tiling that plan uses 8192-row tiles and half the slices of
- * the forward rule); v_bar in the order of axspmv_vbar. */
+ * the forward rule); v_bar in the order of axspmv_vbar, written with 16-byte
+ * stores where d_vbar is 16-byte aligned (8-byte aligned is enough). */
 int axspmv_fwd_adj(axspmv_plan *p, const double *d_x, const double *d_b, double *d_y, double *d_r,
                    double *d_xbar, double *d_vbar, int fused, void *stream);
 
