@@ -64,28 +64,31 @@ __global__ void __launch_bounds__(256) wave_selftest_kernel(uint32_t op, const d
                                                            const uint32_t *__restrict__ h, double *__restrict__ ov,
                                                            uint32_t *__restrict__ oh, size_t n) {
     using namespace axspmv::dev;
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;  // n is a multiple of 64: whole waves leave together
-    double a = v[i];
-    uint32_t b = h[i];
-    switch (op) {  // op is uniform: every lane of the wave takes the same case
-    case 0: a = seg_scan_f64(a, b); break;
-    case 1: b = scan_max_u32(b); break;
-    case 2:
-        a = shift_up_f64(a, -1.0);
-        b = shift_up_u32(b, 0xffffffffu);
-        break;
-    case 3: b = shift_down_u32(b, 0xffffffffu); break;
-    case 4: a = group_sum_f64<1>(a); break;
-    case 5: a = group_sum_f64<2>(a); break;
-    case 6: a = group_sum_f64<4>(a); break;
-    case 7: a = group_sum_f64<8>(a); break;
-    case 8: a = group_sum_f64<16>(a); break;
-    case 9: a = group_sum_f64<32>(a); break;
-    default: a = group_sum_f64<64>(a); break;
+    // grid stride (a multiple of 64) over whole waves: n is a multiple of 64,
+    // so the lanes of a wave stay together and leave together
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double a = v[i];
+        uint32_t b = h[i];
+        switch (op) {  // op is uniform: every lane of the wave takes the same case
+        case 0: a = seg_scan_f64(a, b); break;
+        case 1: b = scan_max_u32(b); break;
+        case 2:
+            a = shift_up_f64(a, -1.0);
+            b = shift_up_u32(b, 0xffffffffu);
+            break;
+        case 3: b = shift_down_u32(b, 0xffffffffu); break;
+        case 4: a = group_sum_f64<1>(a); break;
+        case 5: a = group_sum_f64<2>(a); break;
+        case 6: a = group_sum_f64<4>(a); break;
+        case 7: a = group_sum_f64<8>(a); break;
+        case 8: a = group_sum_f64<16>(a); break;
+        case 9: a = group_sum_f64<32>(a); break;
+        default: a = group_sum_f64<64>(a); break;
+        }
+        ov[i] = a;
+        oh[i] = b;
     }
-    ov[i] = a;
-    oh[i] = b;
 }
 
 }  // namespace

@@ -115,3 +115,17 @@ def test_group_sum(data, log_g):
         want[s:s + g] = sum(v[s:s + g])
     np.testing.assert_array_equal(ov, want)
     np.testing.assert_array_equal(oh, h)
+
+
+def test_group_sum_beyond_one_grid():
+    """More elements than one launch grid covers (the grid is capped at
+    16,384 workgroups of 256 lanes): the kernel strides over whole waves."""
+    rng = np.random.default_rng(65)
+    n = 64 * 70000  # 4,480,000 > 16,384 * 256
+    v = rng.integers(-1000, 1001, n).astype(np.float64)
+    h = np.zeros(n, np.uint32)
+    ov, oh = run(GROUP_SUM_1 + 6, v, h)
+    np.testing.assert_array_equal(ov, np.repeat(v.reshape(-1, 64).sum(1), 64))
+    ov, _ = run(SHIFT_UP, v, h)
+    want = np.concatenate([np.full((n // 64, 1), -1.0), v.reshape(-1, 64)[:, :-1]], axis=1).ravel()
+    np.testing.assert_array_equal(ov, want)
