@@ -26,6 +26,10 @@ class CsrMatrix:
         self.row = np.ascontiguousarray(row, dtype=np.uint32)
         self.col = np.ascontiguousarray(col, dtype=np.uint32)
         self.val = np.ascontiguousarray(val, dtype=np.float64)
+        if self.row.ndim != 1 or self.row.size == 0:
+            raise ValueError("CsrMatrix: row must be a 1-D array of nrows + 1 offsets (at least one)")
+        if self.col.size != self.val.size:  # nnz is taken from col; the library reads that many values
+            raise ValueError(f"CsrMatrix: {self.col.size} columns but {self.val.size} values")
         nrows = self.row.size - 1
         if ncols is None:
             ncols = max(nrows, int(self.col.max()) + 1 if self.col.size else 0)

@@ -26,9 +26,9 @@ __global__ void __launch_bounds__(256) csr_vector_kernel(uint32_t nrows, const V
                                                          const uint32_t *__restrict__ col,
                                                          const uint32_t *__restrict__ row, uint32_t long_len,
                                                          const double *__restrict__ x, double *__restrict__ y) {
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t r = gl / G, g = gl % G;
-    if (r >= nrows) return;  // whole groups leave together (G divides 64)
+    const size_t gl = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gl / G >= nrows) return;  // whole groups leave together (G divides 64)
+    const uint32_t r = uint32_t(gl / G), g = uint32_t(gl % G);
     const uint32_t b = row[r], e = row[r + 1];
     if (e - b > long_len) return;  // csr_long's row
     double acc = 0.0;

@@ -42,7 +42,8 @@ __global__ void __launch_bounds__(256) k1_kernel(const K1Item *__restrict__ item
     for (; j + 4 <= w.j1; j += 4) {
         const uint64_t q = base + uint64_t(j) * chunk;
         const VT v0 = val[q], v1 = val[q + chunk], v2 = val[q + 2 * chunk], v3 = val[q + 3 * chunk];
-        const double x0 = x[col[q]], x1 = x[col[q + chunk]], x2 = x[col[q + 2 * chunk]], x3 = x[col[q + 3 * chunk]];
+        const double x0 = xpad(x, col[q]), x1 = xpad(x, col[q + chunk]), x2 = xpad(x, col[q + 2 * chunk]),
+                     x3 = xpad(x, col[q + 3 * chunk]);
         acc = fma(double(v0), x0, acc);
         acc = fma(double(v1), x1, acc);
         acc = fma(double(v2), x2, acc);
@@ -50,7 +51,7 @@ __global__ void __launch_bounds__(256) k1_kernel(const K1Item *__restrict__ item
     }
     for (; j < w.j1; ++j) {
         const uint64_t q = base + uint64_t(j) * chunk;
-        acc = fma(double(val[q]), x[col[q]], acc);
+        acc = fma(double(val[q]), xpad(x, col[q]), acc);
     }
     if (w.slot == NOROW) {
         if (gid < nrows) y[permi[gid]] = acc;
@@ -80,13 +81,13 @@ __global__ void __launch_bounds__(256) axc_kernel(uint32_t nrows, const uint32_t
                                                   const VT *__restrict__ vp, const uint32_t *__restrict__ mapx,
                                                   uint32_t long_slots, const double *__restrict__ x,
                                                   double *__restrict__ y) {
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t r = gl / G, g = gl % G;
-    if (r >= nrows) return;  // whole groups leave together (G divides 64)
+    const size_t gl = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gl / G >= nrows) return;  // whole groups leave together (G divides 64)
+    const uint32_t r = uint32_t(gl / G), g = uint32_t(gl % G);
     const uint32_t b = brp[r] >> 1, e = brp[r + 1] >> 1;
     if (e - b > long_slots) return;
     double acc = 0.0;
-    for (uint32_t q = b + g; q < e; q += G) acc = fma(double(vp[q]), x[mapx[q]], acc);
+    for (uint32_t q = b + g; q < e; q += G) acc = fma(double(vp[q]), xpad(x, mapx[q]), acc);
     acc = group_sum_f64<G>(acc);
     if (g == 0) y[r] = acc;
 }
@@ -101,7 +102,7 @@ __global__ void __launch_bounds__(256) axc_long(const uint32_t *__restrict__ row
     const uint32_t r = rows[wave];
     const uint32_t b = brp[r] >> 1, e = brp[r + 1] >> 1;
     double acc = 0.0;
-    for (uint32_t q = b + lane; q < e; q += 64) acc = fma(double(vp[q]), x[mapx[q]], acc);
+    for (uint32_t q = b + lane; q < e; q += 64) acc = fma(double(vp[q]), xpad(x, mapx[q]), acc);
     acc = group_sum_f64<64>(acc);
     if (lane == 0) y[r] = acc;
 }

@@ -21,10 +21,10 @@ __global__ void __launch_bounds__(256) spmm_rows_kernel(uint32_t nrows, const ui
                                                         const uint32_t *__restrict__ rowptr, const VT *__restrict__ val,
                                                         const uint32_t *__restrict__ col, uint32_t k, uint32_t long_slots,
                                                         const double *__restrict__ X, double *__restrict__ Y) {
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t r = gl / KT, j = gl % KT;
+    const size_t gl = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gl / KT >= nrows) return;
+    const uint32_t r = uint32_t(gl / KT), j = uint32_t(gl % KT);
     const uint32_t kc = blockIdx.y * KT + j;  // right-hand side of this lane
-    if (r >= nrows) return;
     const uint32_t len = rowptr[r + 1] - rowptr[r];
     if (len > long_slots) return;
     const uint32_t q0 = first[r];

@@ -17,11 +17,11 @@ using axspmv::launch::Block;
 using axspmv::launch::Window;
 
 // x as the kernels read it: straight from HBM, or from a wave's LDS copy of
-// the columns [cmin, cmin + width) its block touches.  A column outside the
-// window belongs to a padding slot (value 0) and reads 0.
+// the columns [cmin, cmin + width) its block touches.  Padding slots carry
+// the column 0xffffffff and read 0 either way (outside every window).
 struct XGlobal {
     const double *__restrict__ p;
-    __device__ __forceinline__ double operator()(uint32_t c) const { return p[c]; }
+    __device__ __forceinline__ double operator()(uint32_t c) const { return xpad(p, c); }
 };
 struct XWindow {
     const double *xw;
@@ -131,7 +131,7 @@ struct UncH1Src {
         VecLoad<VT, K>::get(vp + q, v);
         load_cols<K>(cp + q, c);
 #pragma unroll
-        for (int j = 0; j < K; ++j) xv[j] = x[c[j]];
+        for (int j = 0; j < K; ++j) xv[j] = xpad(x, c[j]);
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < K; ++j) acc = fma(double(v[j]), xv[j], acc);
@@ -141,7 +141,7 @@ struct UncH1Src {
         uint32_t c[K];
         load_cols<K>(cp + size_t(u) * K, c);
 #pragma unroll
-        for (int j = 0; j < K; ++j) xv[j] = x[c[j]];
+        for (int j = 0; j < K; ++j) xv[j] = xpad(x, c[j]);
     }
 };
 

@@ -14,6 +14,13 @@ namespace dev {
 
 constexpr uint32_t NOROW = 0xffffffffu;
 
+// x[c] for a slot of a padded layout: the plans mark padding slots with the
+// column 0xffffffff (AXSPMV_PAD_COL), which reads 0.0, so a non-finite x
+// entry cannot reach a row through a padding slot (0 * inf = NaN).
+__device__ __forceinline__ double xpad(const double *__restrict__ x, uint32_t c) {
+    return c == 0xffffffffu ? 0.0 : x[c];
+}
+
 // DPP control words (GFX9 encoding).
 enum : int {
     DPP_ROW_SHR1 = 0x111,

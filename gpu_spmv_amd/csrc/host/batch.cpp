@@ -175,7 +175,8 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
             m = hit->second.get();
         } else {
             axspmv_axt *fresh = nullptr;
-            AX_TRY(axspmv_build_axt(a, nullptr, 256, o.thw, o.th, AXSPMV_AXT_UNC, AXSPMV_BUILD_PLANES, &fresh, nullptr));
+            AX_TRY(axspmv_build_axt(a, nullptr, 256, o.thw, o.th, AXSPMV_AXT_UNC,
+                                    AXSPMV_BUILD_PLANES | AXSPMV_BUILD_PAD_COL, &fresh, nullptr));
             own.reset(fresh);
             m = fresh;
             if (seen[a] > 1) built.emplace(key, std::move(own));

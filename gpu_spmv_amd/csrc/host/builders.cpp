@@ -159,6 +159,7 @@ extern "C" int axspmv_build_axc(const axspmv_csr *a, const double *x, uint32_t h
         axspmv_axc_free(m);
         return AXSPMV_E_NOMEM;
     }
+    if (flags & AXSPMV_BUILD_PAD_COL) std::fill(m->mapx, m->mapx + m->lenMAPX, AXSPMV_PAD_COL);
 #pragma omp parallel for schedule(static)
     for (uint32_t r = 0; r < n; ++r) {
         const uint32_t b = a->row[r], l = rlen(a, r), s = m->brp[r] / 2;
@@ -281,6 +282,7 @@ extern "C" int axspmv_build_axt(const axspmv_csr *a, const double *x, uint32_t b
         axspmv_axt_free(m);
         return AXSPMV_E_NOMEM;
     }
+    if ((flags & AXSPMV_BUILD_PAD_COL) && m->cplane) std::fill(m->cplane, m->cplane + lenAX / 2, AXSPMV_PAD_COL);
     auto put = [&](uint64_t q, uint32_t p) {
         if (m->vplane) {
             m->vplane[q] = a->val[p];

@@ -254,13 +254,17 @@ int time_kernel(axspmv_plan *p, const axspmv_operands &o, uint32_t iters, uint32
     float tot = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&tot, k[0], k[iters]);
     double main_sum = 0.0;
+    std::vector<double> per(iters);  // SpMV i: k[i] .. k[i+1]
     for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) {
         float t = 0.f;
         e = hipEventElapsedTime(&t, marks ? m[i] : k[i], k[i + 1]);
         main_sum += t;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, k[i], k[i + 1]);
+        per[i] = t;
     }
     if (e != hipSuccess) return hip_fail(e, "timing (kernel)");
-    out->mean_ms = out->median_ms = out->min_ms = out->max_ms = double(tot) / iters;
+    summarize(per, 1.0, out);  // median, min and max of the SpMVs in the region
+    out->mean_ms = double(tot) / iters;
     out->main_ms = main_sum / iters;
     return AXSPMV_OK;
 }

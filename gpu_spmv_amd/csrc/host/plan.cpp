@@ -340,7 +340,8 @@ int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
         return fail(AXSPMV_E_UNSUPPORTED, "AX layout: AXT UNC th = 1 with FP64 values only");
     if (p.o.mode == AXSPMV_AXT_COM) return plan_axt_com(p, a, s);
     axspmv_axt *m = nullptr;
-    AX_TRY(axspmv_build_axt(a, nullptr, p.o.bs, thw, th, p.o.mode, AXSPMV_BUILD_PLANES, &m, nullptr));
+    AX_TRY(axspmv_build_axt(a, nullptr, p.o.bs, thw, th, p.o.mode, AXSPMV_BUILD_PLANES | AXSPMV_BUILD_PAD_COL, &m,
+                           nullptr));
     std::unique_ptr<axspmv_axt, void (*)(axspmv_axt *)> guard(m, axspmv_axt_free);
     if (axl)
         AX_TRY(upload_ax_layout(p, a, m, s));
@@ -422,6 +423,15 @@ int plan_k1(Plan &p, const axspmv_csr *a, hipStream_t s) {
     axspmv_k1 *m = nullptr;
     AX_TRY(axspmv_build_k1(a, C, p.o.reorder ? 1 : 0, &m, nullptr));
     std::unique_ptr<axspmv_k1, void (*)(axspmv_k1 *)> guard(m, axspmv_k1_free);
+    // Padding slots (steps past a row's length, and the lanes past the last
+    // row) get the column AXSPMV_PAD_COL, which the kernel reads as 0.0.
+#pragma omp parallel for schedule(static)
+    for (uint32_t c = 0; c < m->chunkNum; ++c)
+        for (uint32_t r = 0; r < C; ++r) {
+            const uint64_t i = uint64_t(c) * C + r;
+            const uint32_t len = i < a->nrows ? a->row[m->permi[i] + 1] - a->row[m->permi[i]] : 0;
+            for (uint32_t k = len; k < m->nmc[c]; ++k) m->col[uint64_t(m->chp[c]) + uint64_t(k) * C + r] = AXSPMV_PAD_COL;
+        }
     const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
     std::vector<launch::K1Item> items;
     std::vector<uint32_t> split;
@@ -482,7 +492,7 @@ int plan_axc(Plan &p, const axspmv_csr *a, hipStream_t s) {
         return AXSPMV_OK;
     }
     axspmv_axc *m = nullptr;
-    AX_TRY(axspmv_build_axc(a, nullptr, G, AXSPMV_BUILD_PLANES, &m, nullptr));
+    AX_TRY(axspmv_build_axc(a, nullptr, G, AXSPMV_BUILD_PLANES | AXSPMV_BUILD_PAD_COL, &m, nullptr));
     std::unique_ptr<axspmv_axc, void (*)(axspmv_axc *)> guard(m, axspmv_axc_free);
     const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
     const uint32_t long_slots = std::max<uint32_t>(64 * G, 256);
@@ -775,6 +785,15 @@ int plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream, bool tr
     // columns are checked before anything is built.
     AX_TRY(check_csr_bounds(a, "plan_create"));
     if (o->vtype > AXSPMV_F32) return fail(AXSPMV_E_INVALID, "plan_create: unknown value type %u", o->vtype);
+    // A group of lanes per row (CSR vector, AXC): the launch counts nrows x
+    // lanes work-items, which must fit the 32-bit grid size of a dispatch.
+    if ((o->format == AXSPMV_FMT_CSR && o->kernel == AXSPMV_KER_STREAM) ||
+        (o->format == AXSPMV_FMT_AXC && !(o->flags & AXSPMV_PLAN_AX_LAYOUT))) {
+        const uint64_t g = o->hbs ? o->hbs : lanes_for_mean(a);
+        if (uint64_t(a->nrows) * g >= (1ull << 32))
+            return fail(AXSPMV_E_OVERFLOW, "plan_create: %u rows x %llu lanes per row exceed a 32-bit grid", a->nrows,
+                        (unsigned long long)g);
+    }
     auto p = std::make_unique<Plan>();
     p->o = *o;
     if (transposed) {
@@ -996,6 +1015,8 @@ int plan_op_check(const Plan *p, uint32_t op, const axspmv_operands &o) {
         if (o.k == 0) return fail(AXSPMV_E_INVALID, "SpMM: k must be > 0");
         const uint32_t kt = pick_ktile(p, o.k);
         if (kt != 8 && kt != 16 && kt != 32 && kt != 64) return fail(AXSPMV_E_INVALID, "SpMM: ktile %u not in 8/16/32/64", kt);
+        if (p->kind != Plan::RCT && uint64_t(p->nrows) * kt >= (1ull << 32))
+            return fail(AXSPMV_E_OVERFLOW, "SpMM: %u rows x ktile %u exceed a 32-bit grid", p->nrows, kt);
         if (p->kind == Plan::RCT && uint64_t(p->rct.R) * kt * 8 > p->lds_max)
             return fail(AXSPMV_E_UNSUPPORTED,
                         "SpMM: RCT tiles of %u rows x ktile %u need %llu bytes of LDS, %zu available (plan with "

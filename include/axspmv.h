@@ -141,6 +141,13 @@ typedef struct axspmv_fmtdata {
 /* Builder flags. */
 #define AXSPMV_BUILD_AX 1u     /* emit the reference's interleaved ax[] (needs x) */
 #define AXSPMV_BUILD_PLANES 2u /* emit the value plane / column plane        */
+#define AXSPMV_BUILD_PAD_COL 4u /* padding slots of the column plane (AXT) and
+                                   of mapx (AXC) hold AXSPMV_PAD_COL instead
+                                   of the reference's 0.  The device plans are
+                                   built this way: their kernels read 0.0 for
+                                   such a slot, so a non-finite x[0] does not
+                                   reach rows that have padding (0 * inf).  */
+#define AXSPMV_PAD_COL 0xffffffffu
 
 /* K1 (str_matK1, cudaSpmv.cu:712; builders cudaSpmv.cu:716-846).
  * chunk = rows per chunk (the reference's CHUNK_SIZE; 64 = one wavefront).
@@ -366,7 +373,9 @@ typedef struct axspmv_timing {
                                  kernel starts: main_ms is that kernel's mean
                                  time.  Only RCT with slices has a kernel
                                  before its main one (the clear of y); other
-                                 paths report main_ms = the SpMV's time. */
+                                 paths report main_ms = the SpMV's time.
+                                 median/min/max: of the intervals between the
+                                 starts of consecutive SpMVs in the region. */
 /* Time `iters` SpMVs after `warmup` untimed ones.  AXSPMV_TIME_EVENTS is the
  * reference's loop (cudaSpmv.cu:563-581): mean and median of per-launch
  * times.  AXSPMV_TIME_GRAPH reports per-SpMV times of graph replays (for

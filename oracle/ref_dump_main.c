@@ -15,10 +15,13 @@
  *   ref_dump M.bin X.bin OUT k1
  *   ref_dump M.bin X.bin OUT axc HBS
  *   ref_dump M.bin X.bin OUT axt BS THW TH UNC|COM
+ *   ref_dump M.csr - OUT parse
  *
  * M.bin is read by the reference's matrixReading (.bin layout), X.bin holds
  * nrows doubles.  Each array is written raw to OUT.<name>; OUT.meta holds
- * "name value" lines (lengths, log, mfp, beta).
+ * "name value" lines (lengths, log, mfp, beta).  `parse` reads a matrix file
+ * of either layout (.csr text or .bin) with matrixReading and writes the
+ * arrays it parsed (row, col, val); it needs no x.
  */
 
 static void dump_raw(const char *prefix, const char *name, const void *p, size_t bytes) {
@@ -36,6 +39,18 @@ int main(int argc, char **argv) {
         return 2;
     }
     str_matCSR csr = matrixReading(argv[1]);
+    if (!strcmp(argv[4], "parse")) {
+        char metap[512];
+        snprintf(metap, sizeof metap, "%s.meta", argv[3]);
+        FILE *meta = fopen(metap, "w");
+        if (!meta) { perror(metap); return 2; }
+        fprintf(meta, "nrows %u\nnnz %u\n", csr.nrows, csr.nnz);
+        fclose(meta);
+        dump_raw(argv[3], "row", csr.row, (size_t)(csr.nrows + 1) * sizeof(UIN));
+        dump_raw(argv[3], "col", csr.col, csr.nnz * sizeof(UIN));
+        dump_raw(argv[3], "val", csr.val, csr.nnz * sizeof(FPT));
+        return 0;
+    }
     printMatrixStats(argv[1], &csr);
     FPT *x = (FPT *)calloc(csr.nrows + 1, sizeof(FPT));
     FILE *fx = fopen(argv[2], "rb");

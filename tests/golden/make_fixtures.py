@@ -5,7 +5,8 @@ reference's cudaSpmv.cu compiled by `make -C oracle ref` -- on small
 matrices and stores, per matrix, the inputs (row/col/val, x) and the
 reference's outputs: ncsr's y, rl[] and the .sta statistics, and the K1, AXC
 and AXT arrays at the reference's 32-lane parameters and at the 64-lane
-parameters the kernels use.
+parameters the kernels use; and, in csrtext.npz, two .csr text files with
+the arrays the reference's matrixReading parses from them.
 
 Needs /root/reference (present only where the fixtures are made); the tests
 read the committed .npz files and never run the reference.
@@ -82,6 +83,52 @@ def load_arrays(out, names):
     return res
 
 
+def varied_csr_text(seed: int = 31) -> str:
+    """A .csr text file in the reference's layout ("nrows nnz", nnz lines
+    "val col", nrows + 1 offsets) written with varied number formats and
+    whitespace: 17-digit, exponent, signed, integer-valued, denormal and
+    tiny values, tabs, blank lines, offsets spread over several lines."""
+    rng = np.random.default_rng(seed)
+    rls = [3, 0, 1, 5, 2, 0, 4, 6, 1, 3]
+    row = np.concatenate([[0], np.cumsum(rls)])
+    fmts = ["{:.17g}", "{:.6e}", "{:+.3f}", "{:g}", "{:.20e}", "{:.1f}", "{!r}"]
+    vals = list(rng.standard_normal(sum(rls)) * 10.0 ** rng.integers(-300, 300, sum(rls)))
+    vals[0], vals[1], vals[2], vals[3] = 5e-324, -0.0, 1.0, 2.5e-310
+    lines = [f"  {len(rls)}\t{sum(rls)}"]
+    k = 0
+    for r, l in enumerate(rls):
+        for c in sorted(rng.choice(40, l, replace=False)):
+            sep = ["  ", "\t", " "][k % 3]
+            lines.append(fmts[k % len(fmts)].format(float(vals[k])) + sep + str(int(c)) + ("\n" if k % 7 == 3 else ""))
+            k += 1
+    offs = [str(int(v)) for v in row]
+    lines.append(" ".join(offs[:4]))
+    lines.append("\t".join(offs[4:]))
+    return "\n".join(lines) + "\n"
+
+
+def csr_text_fixture(tmp):
+    """tests/golden/csrtext.npz: .csr text files and the arrays the
+    reference's matrixReading (cudaSpmv.cu:265-328) parses from them --
+    one with varied formatting, one written by the library's .csr writer."""
+    store = {}
+    writer_m = ax.random_sparse(40, 0, 8, seed=21)
+    path = os.path.join(tmp, "t.csr")
+    writer_m.write(path)
+    texts = {"varied": varied_csr_text(), "writer": open(path).read()}
+    for name, text in texts.items():
+        with open(path, "w") as f:
+            f.write(text)
+        out = os.path.join(tmp, "p")
+        run("ref_dump_c32", path, "-", out, "parse")
+        store[f"{name}.text"] = np.array(text)
+        store.update({f"{name}.{k}": v for k, v in load_arrays(
+            out, [("row", np.uint32), ("col", np.uint32), ("val", np.float64)]).items()})
+    dst = os.path.join(HERE, "csrtext.npz")
+    np.savez_compressed(dst, **store)
+    print(f"csrtext: {', '.join(texts)} -> {os.path.getsize(dst)} bytes")
+
+
 def main():
     if not (os.path.exists(os.path.join(REF, "ref_dump_c32")) and os.path.exists(os.path.join(REF, "ref_dump_c64"))):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
@@ -130,6 +177,7 @@ def main():
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **store)
         print(f"{name}: n={n} nnz={m.nnz} -> {os.path.getsize(path)} bytes")
+    csr_text_fixture(tmp)
     for f in os.listdir(tmp):
         os.remove(os.path.join(tmp, f))
     os.rmdir(tmp)

@@ -26,3 +26,69 @@ def test_c_driver_host(tmp_path):
 def test_c_driver_gpu(tmp_path):
     out = subprocess.run([build(tmp_path), "gpu"], check=True, capture_output=True, text=True, timeout=120).stdout
     assert out.count("gpu: ") == 2, out
+
+
+DRIVER = os.path.join(LIB, "axspmv_main")
+
+
+def test_main_driver_arguments():
+    """The C host driver (in place of the reference's main) rejects bad
+    arguments, and without a HIP device it stops instead of running the
+    kernels' rows on the CPU."""
+    import gpu_spmv_amd as ax
+    if not os.path.exists(DRIVER):
+        pytest.skip("axspmv_main not built")
+    r = subprocess.run([DRIVER], capture_output=True, text=True)
+    assert r.returncode != 0 and "usage" in r.stderr
+    r = subprocess.run([DRIVER, "laplace2d:8", "-q"], capture_output=True, text=True)
+    assert r.returncode != 0 and "unknown argument" in r.stderr
+    if ax.device_count() == 0:
+        r = subprocess.run([DRIVER, "laplace2d:8"], capture_output=True, text=True)
+        assert r.returncode != 0 and "no HIP device" in r.stderr
+
+
+def parse_tables(out):
+    """The two tables axspmv_main prints: formats (name -> mfp MB, beta,
+    ct, device ct) and kernels (name -> exeTime, Gflops, ordTime, errAbs,
+    errRel, rowInd, sErrRel)."""
+    fm, km, sect = {}, {}, None
+    for line in out.splitlines():
+        if line.startswith("formats' data"):
+            sect = "f"
+        elif line.startswith("SpMV kernels' results"):
+            sect = "k"
+        elif sect and line.strip() and not line.startswith("---") and "memory [Mbytes]" not in line \
+                and "exeTime" not in line:
+            parts = line.split()
+            if sect == "f":
+                fm[" ".join(parts[:-4])] = parts[-4:]
+            else:
+                km[" ".join(parts[:-7])] = [float(v) for v in parts[-7:]]
+    return fm, km
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spec", ["laplace2d:64", "random:3001:0:40", "file"])
+def test_main_driver_tables(tmp_path, spec):
+    """axspmv_main on a generated matrix and on a .bin file: every format's
+    row (mfp, beta, conversion times) and every kernel's row, each kernel
+    within 1e-12 (signed) of the CPU CSR product."""
+    import gpu_spmv_amd as ax
+    if spec == "file":
+        m = ax.powerlaw(5000, 2.0, 256, 12.0, seed=4)
+        spec = str(tmp_path / "plaw.bin")
+        m.write(spec)
+        args = [DRIVER, spec, "-i", "20", "-x", "index", "-s"]
+    else:
+        args = [DRIVER, spec, "-i", "20", "-b", "128"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    fm, km = parse_tables(r.stdout)
+    assert len(fm) == 16 and len(km) == 21, (fm.keys(), km.keys())
+    assert any(k.startswith("faxtch1hw64bs") for k in fm)
+    for name, row in km.items():
+        et, gf, ot, ea, er, pos, ser = row
+        assert et > 0 and gf > 0 and ser <= 1e-12, (name, row)
+        assert (ot > 0) == ("ax layout" in name), (name, row)
+    if spec.endswith(".bin"):
+        assert os.path.exists(str(tmp_path / "plaw.sta"))

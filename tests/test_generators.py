@@ -1,9 +1,12 @@
 """The seeded RNG and the synthetic benchmark matrices: published RNG values,
 an independent Python restatement of xoshiro256**, closed-form sizes."""
+import hashlib
+
 import numpy as np
 import pytest
 
 import gpu_spmv_amd as ax
+from gpu_spmv_amd import workloads as W
 from oracle import oracle as orc
 
 M64 = (1 << 64) - 1
@@ -86,9 +89,42 @@ def test_stencil27_closed_form(grid, permute):
         assert np.allclose(np.sort(np.linalg.eigvalsh(d)), np.sort(np.linalg.eigvalsh(base.to_dense())))
 
 
-def test_stencil27_config3_size():
-    # n = 128^3 would take a while; check the closed form the generator uses
-    assert (3 * 128 - 2) ** 3 == 55742968
+# Fingerprints of the benchmark inputs: a change to a generator, the RNG or
+# the seeds changes the benchmark and fails here first.  nnz of configs 1
+# and 3 are closed forms of the grid (5 g^2 - 4 g, (3 g - 2)^3); config 2's
+# count and every digest are those of the inputs the benchmark ran on.
+def test_config1_laplace_pinned():
+    m, x = W.config1()
+    assert (m.nrows, m.nnz) == (65536, 5 * 256 ** 2 - 4 * 256) == (65536, 326656)
+    assert W.digest(m) == "25f9449b4f39ef36"
+
+
+def test_config2_powerlaw_pinned():
+    m, x = W.config2()
+    assert (m.nrows, m.nnz) == (1 << 20, 20970465)
+    assert W.digest(m) == "dc05322526159495"
+    assert hashlib.sha256(x.tobytes()).hexdigest()[:16] == "a38ec121af095151"
+
+
+def test_config3_stencil_pinned():
+    m, _ = W.config3()
+    assert (m.nrows, m.nnz) == (128 ** 3, (3 * 128 - 2) ** 3) == (2097152, 55742968)
+    assert W.digest(m) == "5fad6478cc6e7f4f"
+
+
+def test_config4b_batch_pinned():
+    """256 banded members, generated one at a time: the total nonzero count
+    and a digest of the members' digests."""
+    shapes = W.config4b_shapes()
+    assert len(shapes) == 256
+    assert all(1000 <= n <= 200000 and 1 <= h <= 64 for n, h, _ in shapes)
+    tot, h = 0, hashlib.sha256()
+    for s in shapes:
+        m = W.config4b_matrix(*s)
+        tot += m.nnz
+        h.update(W.digest(m).encode())
+    assert tot == 457872800
+    assert h.hexdigest()[:16] == "66096c682f8cd455"
 
 
 def test_powerlaw_properties():
@@ -126,6 +162,33 @@ def test_io_roundtrip(tmp_path):
         ax.CsrMatrix.read(str(tmp_path / "m.txt"))
     with pytest.raises(ax.AxspmvError):
         ax.CsrMatrix.read(str(tmp_path / "missing.bin"))
+
+
+@pytest.mark.parametrize("name", ["varied", "writer"])
+def test_csr_text_reader_matches_reference(name, tmp_path):
+    """The .csr text reader against the reference's matrixReading
+    (cudaSpmv.cu:268-293, fscanf "%lf %d"): the arrays it parsed from a file
+    with varied number formats and whitespace (denormals, -0.0, 17- and
+    21-digit mantissas, tabs, blank lines), and from a file the library's
+    writer produced, are the library's parse bit for bit."""
+    import os
+    from conftest import GOLDEN
+    z = np.load(os.path.join(GOLDEN, "csrtext.npz"), allow_pickle=False)
+    p = str(tmp_path / "m.csr")
+    with open(p, "w") as f:
+        f.write(str(z[f"{name}.text"]))
+    m = ax.CsrMatrix.read(p)
+    np.testing.assert_array_equal(m.row, z[f"{name}.row"])
+    np.testing.assert_array_equal(m.col, z[f"{name}.col"])
+    assert m.val.tobytes() == z[f"{name}.val"].tobytes()  # -0.0 and denormals included
+    if name == "writer":
+        # the writer's text is what the reference parsed back to the original arrays
+        w = ax.random_sparse(40, 0, 8, seed=21)
+        q = str(tmp_path / "w.csr")
+        w.write(q)
+        assert open(q).read() == str(z["writer.text"])
+        assert w.val.tobytes() == z["writer.val"].tobytes()
+        np.testing.assert_array_equal(w.col, z["writer.col"])
 
 
 def test_stats_and_sta_match_reference(fixture_matrix, tmp_path):

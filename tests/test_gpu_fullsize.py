@@ -8,6 +8,12 @@ matrix of 2^20 rows is covered in test_gpu_spmv.py).
   b ~ U[-1, 1]: y, r = y - b, x_bar = A^T r against the oracle (1e-12 /
   1e-11), v_bar = r[row] * x[col] bit for bit against the oracle's product of
   the r the kernel wrote.
+* Config 4A: k = 8, 32 and 64 right-hand sides on the power-law matrix, FP64
+  and FP32 values.
+* Config 4B: the 256 banded matrices (n ~ U{1000..200000}, 457,872,800
+  nonzeros) in one launch, as the AXT persistent batch (uniform tiles and the
+  autotuner's per-group tiles) and as the RCT block-diagonal plan; every
+  member against the oracle at 1e-12.
 """
 import os
 
@@ -15,6 +21,7 @@ import numpy as np
 import pytest
 
 import gpu_spmv_amd as ax
+from gpu_spmv_amd import workloads as W
 from conftest import rel_err
 from oracle import oracle as orc
 
@@ -83,7 +90,7 @@ def powerlaw():
 
 
 @pytest.mark.timeout(300)
-@pytest.mark.parametrize("k", [8, 64])
+@pytest.mark.parametrize("k", [8, 32, 64])
 @pytest.mark.parametrize("vt", [ax.F64, ax.F32], ids=["f64", "f32val"])
 @pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_RCT, dict(tile_rows=1024)),
                                     (ax.FMT_RCT, dict(tile_rows=256, slices=1))],
@@ -100,3 +107,42 @@ def test_spmm_config4a_full_size(powerlaw, k, vt, fmt, kw):
     dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * m.nrows * k)
     p.spmm(k, dX, dY)
     assert rel_err(ref, dY.download(np.float64, m.nrows * k).reshape(m.nrows, k)) <= TOL
+
+
+@pytest.fixture(scope="module")
+def batch256():
+    mats = W.config4b()
+    assert sum(m.nnz for m in mats) == 457872800
+    return mats
+
+
+def check_members(b, mats, x):
+    dx = ax.DeviceBuffer.from_array(x)
+    ny = int(b.yoff[-1])
+    dy = ax.DeviceBuffer.from_array(np.full(ny, np.nan))
+    b.spmv(dx, dy)
+    y = dy.download(np.float64, ny)
+    worst = 0.0
+    for i, m in enumerate(mats):
+        e = rel_err(orc.ncsr(m, x[b.xoff[i]:b.xoff[i + 1]], nthreads=NT), y[b.yoff[i]:b.yoff[i + 1]])
+        assert e <= TOL, (i, m.nrows, e)
+        worst = max(worst, e)
+    return worst
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("how", ["axt_thw4", "axt_autotuned", "rct"])
+def test_batch_config4b_full_size(batch256, how):
+    mats = batch256
+    if how == "axt_thw4":
+        b = ax.Batch(mats, thw=4)
+    elif how == "axt_autotuned":
+        tuned, _ = ax.autotune_batch(mats, iters=3)
+        b = ax.Batch(mats, opts_list=tuned)
+    else:
+        b = ax.Batch(mats, fmt=ax.FMT_RCT)
+    info = b.info()
+    assert info["count"] == 256 and info["nnz"] == 457872800
+    x = ax.uniform(6, int(b.xoff[-1]))
+    check_members(b, mats, x)
+    b.close()

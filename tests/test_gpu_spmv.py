@@ -313,3 +313,31 @@ def test_timing_modes():
     assert rel_err(orc.ncsr(m, x), dy.download(np.float64, m.nrows)) <= TOL
     with pytest.raises(ax.AxspmvError):
         r.time(dx, dy, iters=5000, warmup=0, mode=ax.core.TIME_KERNEL)
+
+
+NONFINITE_KERNELS = [k for k in KERNELS if not k[0].startswith("csr_vector_g")] + [
+    ("unc_h1_thw4_ax", ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_AX_LAYOUT)),
+    ("unc_h1_thw4_f32", ax.FMT_AXT, dict(thw=4, vtype=ax.F32)),
+    ("unc_h4_thw64_staged", ax.FMT_AXT, dict(thw=64, th=4, flags=ax.PLAN_STAGE_X))]
+
+
+@pytest.mark.parametrize("mname", ["band300", "random_odd", "long_rows"])
+@pytest.mark.parametrize("kname,fmt,kw", NONFINITE_KERNELS, ids=[k[0] for k in NONFINITE_KERNELS])
+def test_nonfinite_x_stays_in_its_rows(mname, kname, fmt, kw):
+    """x[0] = inf and x[1] = nan: only the rows holding column 0 or 1 become
+    non-finite, as in the CPU product.  Padding slots (value 0) carry the
+    column 0xffffffff in the plans and read 0.0, so 0 * inf never reaches a
+    row through padding."""
+    m = MATS[mname]
+    x = ax.uniform(19, m.ncols)
+    x[0], x[1] = np.inf, np.nan
+    if kw.get("vtype") == ax.F32:
+        ref = orc.ncsr_f32val(m, x)
+    else:
+        ref = orc.ncsr(m, x)
+    y = ax.spmv_host(ax.Plan(m, fmt, **kw), x)
+    fin = np.isfinite(ref)
+    assert not fin.all() and fin.any()
+    np.testing.assert_array_equal(np.isfinite(y), fin)
+    np.testing.assert_array_equal(y[~fin], ref[~fin])  # the same inf / nan
+    assert rel_err(ref[fin], y[fin]) <= TOL

@@ -80,3 +80,27 @@ def test_k1_length_and_reorder():
 def test_generators():
     overflow(ax.laplace2d, 30000, match="grid too large")  # 5 * 30000^2 > 2^32
     overflow(ax.stencil27, 600, match="grid too large")  # 27 * 600^3 > 2^32
+
+
+def test_lane_group_grids():
+    # CSR vector and AXC launch nrows x lanes work-items, and the dispatch
+    # grid is 32 bits: 2^26 rows at 64 lanes per row is rejected before any
+    # device work (this container has no GPU)
+    m = empty_rows(1 << 26)
+    with pytest.raises(L.AxspmvError, match="32-bit grid") as e:
+        ax.Plan(m, ax.FMT_CSR, kernel=ax.KER_STREAM, hbs=64)
+    assert e.value.status == L.E_OVERFLOW
+    with pytest.raises(L.AxspmvError, match="32-bit grid") as e:
+        ax.Plan(m, ax.FMT_AXC, hbs=64)
+    assert e.value.status == L.E_OVERFLOW
+
+
+def test_csr_matrix_shape_checks():
+    with pytest.raises(ValueError, match="offsets"):
+        ax.CsrMatrix(np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0))
+    with pytest.raises(ValueError, match="values"):
+        ax.CsrMatrix(np.array([0, 1], np.uint32), np.zeros(1, np.uint32), np.zeros(2))
+    # inconsistent offsets are the library's to reject (before any kernel)
+    bad = ax.CsrMatrix(np.array([0, 2], np.uint32), np.zeros(1, np.uint32), np.zeros(1))
+    with pytest.raises(L.AxspmvError, match="row"):
+        bad.check()
