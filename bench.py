@@ -1,21 +1,29 @@
 """SpMV benchmark on one MI355X.
 
-Workload (one step = one SpMV y = A x, inputs resident in HBM): the
-synthetic power-law matrix, n = 2^20 rows, row lengths Zipf(alpha = 2) on
-{1..2048} rescaled to mean 20 (nnz ~ 21.0M), uniform sorted distinct
-columns, values N(0,1), x ~ U[-1,1], seed 1 -- FP64.
+Workload of the headline line (one step = one SpMV y = A x, inputs resident
+in HBM): the synthetic power-law matrix, n = 2^20 rows, row lengths Zipf(alpha
+= 2) on {1..2048} rescaled to mean 20 (nnz = 20,970,465), uniform sorted
+distinct columns, values N(0,1), x ~ U[-1,1], seed 1 -- FP64.
 
-Prints one JSON line: GFLOP/s of the flagship kernel (2*nnz per SpMV), its
-roofline against 8 TB/s of HBM (algorithmic bytes per SpMV / kernel time
-from HIP events), the CPU baseline (the reference's own CSR product, ncsr,
+Prints one JSON line: GFLOP/s of the flagship plan (2*nnz per SpMV, host
+clock over the timed region), its roofline against 8 TB/s of HBM
+(algorithmic bytes per SpMV / the whole SpMV's time from HIP events, with
+the tile kernel alone beside it, the spread over the region and the
+cold-cache time), the CPU baseline (the reference's own CSR product, ncsr,
 built from its sources into oracle/_ref, else the oracle's restatement of
-it, on this host's cores), every format on the same matrix, and the
-adjoint / SpMM / FP32-value forms against plain SpMV.
+it, on this job's host cores), and, per configuration, the reference's two
+result tables: formats (name, memory footprint, occupancy, conversion time
+on the host and on the device) and kernels (time, GFLOP/s, GB/s, fraction
+of the roofline, ordering time, errAbs / errRel / rowInd as the reference's
+get_errors computes them and the signed error).  By default every
+configuration runs: the power-law matrix (every format, forward + adjoint,
+SpMM at k = 8 / 32 / 64 in FP64 and FP32 values with the error against the
+FP64 oracle, conversion times, rocSPARSE), the 256^2 Laplacian
+(graph-timed), the permuted 27-point stencil on 128^3, forward + adjoint on
+its structure, and the 256-matrix banded batch in one launch.
 
   python bench.py [--steps K] [--warmup W] [--formats flagship|all]
-  python bench.py --suite full     also the Laplacian, the permuted 27-point
-                                   stencil, the 256-matrix batch and the
-                                   forward + adjoint on the stencil structure
+                  [--suite all|headline]
   python bench.py --calibrate      copy bandwidth and the x-footprint sweep
   python bench.py --pmc-probe [stencil]
                                    a 256 MB copy then 5 SpMVs of each flagship
@@ -38,6 +46,7 @@ sys.path.insert(0, ROOT)
 
 import gpu_spmv_amd as ax  # noqa: E402
 from gpu_spmv_amd import _lib as L  # noqa: E402
+from gpu_spmv_amd import workloads as W  # noqa: E402
 from gpu_spmv_amd.core import TIME_COLD, TIME_GRAPH, TIME_KERNEL, TIME_REGION  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
@@ -125,14 +134,34 @@ def rocsparse_rows(m, dx, yref, iters: int = 50):
     return rows
 
 
-def cpu_model() -> str:
+def cpu_info() -> dict:
+    """The host's CPU model, its physical cores and logical CPUs, and the
+    CPUs this process may run on (its share of the host)."""
+    model, cores = "unknown", set()
+    phys = core = None
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None and core is not None:
+                cores.add((phys, core))
+                phys = core = None
     except OSError:
         pass
-    return "unknown"
+    if phys is not None and core is not None:
+        cores.add((phys, core))
+    try:
+        share = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        share = os.cpu_count() or 1
+    return dict(cpu=model, host_physical_cores=len(cores) or None, host_logical_cpus=os.cpu_count(),
+                cpus_available_to_this_job=share)
 
 
 def cpu_baseline_reference(m, x, seconds: float):
@@ -167,7 +196,7 @@ def cpu_baseline_reference(m, x, seconds: float):
                           f"OMP_PROC_BIND=true, OMP_WAIT_POLICY=active, built from its sources into oracle/_ref) on the "
                           f"full matrix, "
                           f"{el:.1f} s",
-                   ms_per_spmv=round(per * 1e3, 3), alg_gbs=round(alg / per / 1e9, 1), cpu=cpu_model())
+                   ms_per_spmv=round(per * 1e3, 3), alg_gbs=round(alg / per / 1e9, 1), **cpu_info())
 
 
 def cpu_baseline(m, x, seconds: float):
@@ -190,28 +219,93 @@ def cpu_baseline(m, x, seconds: float):
     alg = 12.0 * m.nnz + 4.0 * (m.nrows + 1) + 8.0 * m.ncols + 8.0 * m.nrows
     return y, dict(value=round(2.0 * m.nnz / per / 1e9, 3), unit="GFLOP/s", cores=threads, kind="port",
                    sample=f"{calls} calls of ncsr (OpenMP static over rows) on the full matrix, {el:.1f} s",
-                   ms_per_spmv=round(per * 1e3, 3), alg_gbs=round(alg / per / 1e9, 1), cpu=cpu_model())
+                   ms_per_spmv=round(per * 1e3, 3), alg_gbs=round(alg / per / 1e9, 1), **cpu_info())
+
+
+def sub_rows(m, rows):
+    """The rows `rows` of m as a CSR matrix with m's columns."""
+    rows = np.asarray(rows)
+    lens = (m.row[rows + 1] - m.row[rows]).astype(np.int64)
+    idx = np.concatenate([np.arange(m.row[r], m.row[r + 1]) for r in rows]) if rows.size else np.zeros(0, np.int64)
+    row = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    return ax.CsrMatrix(row, m.col[idx], m.val[idx], ncols=m.ncols)
+
+
+def errors(ref, y):
+    """The kernels table's error columns: errAbs / errRel / rowInd as the
+    reference's get_errors computes them (cudaSpmv.cu:474-495: it compares
+    |y| with |y_ref|) and the signed norm-wise relative error the parity
+    tests use (rel_err)."""
+    if ref is None:
+        return dict(rel_err=None)
+    from oracle import oracle as orc
+    a, r, pos = orc.get_errors(ref, y, sign_blind=True)
+    return dict(rel_err=rel_err(ref, y), errAbs=a, errRel=r, rowInd=int(pos))
 
 
 def summarize(info, nnz, sec, name, err, extra=None):
     gbs = info["alg_bytes"] / sec / 1e9
     r = dict(kernel=info["name"], label=name, us=round(sec * 1e6, 2), gflops=round(2.0 * nnz / sec / 1e9, 2),
              alg_gbs=round(gbs, 1), frac=round(gbs / PEAK_HBM_GBS, 4), beta=round(info["beta"], 4),
-             stream_gbs=round(info["stream_bytes"] / sec / 1e9, 1), launches=info["launches"], rel_err=err)
+             stream_gbs=round(info["stream_bytes"] / sec / 1e9, 1), launches=info["launches"])
+    r.update(err if isinstance(err, dict) else dict(rel_err=err))
     if extra:
         r.update(extra)
     return r
 
 
 def measure(m, name, fmt, kw, dx, dy, iters, warmup, yref, mode=TIME_REGION, batch=1):
+    """One row of the kernels table: plan, time (the plan's build time on the
+    host, upload included, in `plan_s`), errors, and for the AX layout the
+    per-SpMV x refresh as the reference's ordering time `ot`."""
+    t0 = time.perf_counter()
     p = ax.Plan(m, fmt, **kw)
+    plan_s = time.perf_counter() - t0
     info = p.info()
     p.spmv(dx, dy)
     L.call("axspmv_device_sync")
-    err = rel_err(yref, dy.download(np.float64, m.nrows)) if yref is not None else None
+    err = errors(yref, dy.download(np.float64, m.nrows))
     p.time(dx, dy, iters=max(warmup, 1), warmup=0, mode=mode, batch=batch)
     t = p.time(dx, dy, iters=iters, warmup=0, mode=mode, batch=batch)
-    return p, info, summarize(info, m.nnz, t.mean_ms * 1e-3, name, err)
+    extra = dict(plan_s=round(plan_s, 3), ot_us=0.0, device_mb=round(info["device_bytes"] / 1e6, 2))
+    if kw.get("flags", 0) & ax.PLAN_AX_LAYOUT:
+        extra["ot_us"] = round(p.time_op(L.OP_XREFRESH, iters=iters, warmup=2, x=dx).mean_ms * 1e3, 2)
+    return p, info, summarize(info, m.nnz, t.mean_ms * 1e-3, name, err, extra)
+
+
+def formats_table(m, x, dx, devbuild=True):
+    """The reference's formats table (cudaSpmv.cu:2420-2436) for this matrix:
+    each reference format's name, memory footprint mfp [MB], occupancy beta
+    and conversion time -- the host builders' `ct` (OpenMP threads of this
+    job) as the reference measures it, and the device builders' time where
+    there is one."""
+    rows = [dict(format="fcsr", mfp_mb=round((12.0 * m.nnz + 4.0 * (m.nrows + 1) + 8.0 * m.nrows) / 1e6, 2),
+                 beta=1.0, ct_s=0.0)]
+
+    def add(fd, dev=None, label=None):
+        r = dict(format=label or fd["name"], mfp_mb=round(fd["mfp"] / 1e6, 2), beta=round(fd["beta"], 4),
+                 ct_s=round(fd["ct"], 6))
+        if dev is not None:
+            r["device_ct_s"] = round(dev["ms"] * 1e-3, 6)
+        rows.append(r)
+
+    for chunk, reorder in ((64, True), (64, False)):
+        k = ax.build_k1(m, chunk, reorder)
+        add(k["fd"], label=f"{k['fd']['name']}_c{chunk}")
+        del k
+    for hbs in (4, 64):
+        h = ax.build_axc(m, x, hbs, flags=L.BUILD_AX)
+        d = ax.dev_build_axc(m, x, hbs) if devbuild else None
+        add(h["fd"], d, label=f"faxc_hbs{hbs}")
+        del h, d
+    for bs, thw, th, mode in ((256, 16, 1, ax.AXT_UNC), (256, 8, 1, ax.AXT_UNC), (256, 4, 1, ax.AXT_UNC),
+                              (256, 64, 4, ax.AXT_UNC), (256, 64, 8, ax.AXT_UNC), (256, 64, 1, ax.AXT_COM),
+                              (256, 64, 4, ax.AXT_COM), (256, 64, 8, ax.AXT_COM)):
+        h = ax.build_axt(m, x, bs, thw, th, mode, flags=L.BUILD_AX)
+        d = ax.dev_build_axt(m, x, bs, thw, th, mode) if devbuild else None
+        add(h["fd"], d)
+        del h, d
+    return rows
 
 
 def extras_same_matrix(m, x, dx, yref, plain_us):
@@ -250,6 +344,7 @@ def extras_same_matrix(m, x, dx, yref, plain_us):
         dX = ax.DeviceBuffer.from_array(X)
         dY = ax.DeviceBuffer(8 * n * k)
         rows = np.arange(0, n, max(1, n // 2000))
+        Yref = orc.spmm(sub_rows(m, rows), X, nthreads=omp_threads())  # FP64 oracle on the sampled rows
         for vt in (ax.F64, ax.F32):
             # the library's SpMM candidates (AXT k-tiles, RCT tiles), in this value type
             cands = ax.autotune_defaults(L.OP_SPMM)
@@ -262,12 +357,13 @@ def extras_same_matrix(m, x, dx, yref, plain_us):
             kt = c.ktile
             t = pm.time_op(L.OP_SPMM, iters=10, warmup=2, k=k, X=dX, Y=dY)
             Y = dY.download(np.float64, n * k).reshape(n, k)
-            # accuracy on sampled rows against the FP64 oracle
-            Yref = orc.spmm(m, X, nthreads=omp_threads())[rows] if k == 8 else None
-            err = rel_err(Yref, Y[rows]) if Yref is not None else None
+            # accuracy on the sampled rows against the FP64 oracle (for FP32
+            # values: the error of storing the values in FP32)
+            err = rel_err(Yref, Y[rows])
             alg = (12 if vt == ax.F64 else 8) * nnz + 4 * (n + 1) + 16 * n * k
             us = t.mean_ms * 1e3
             sp.append(dict(k=k, vtype="f64" if vt == ax.F64 else "f32val", kernel=pm.info()["name"], ktile=kt,
+                           rows_checked=int(rows.size),
                            us=round(us, 1),
                            gflops=round(2.0 * nnz * k / (us * 1e-6) / 1e9, 1),
                            alg_gbs=round(alg / (us * 1e-6) / 1e9, 1),
@@ -365,62 +461,75 @@ def pmc_probe(matrix: str = "powerlaw"):
         log(f"pmc-probe: {p.info()}")
 
 
-def suite_full(res):
-    """Laplacian (graph-timed), permuted stencil, 256-matrix batch, forward +
-    adjoint on the stencil structure."""
+def suite_c1(tables, extras):
+    """Config 1: the 2D Laplacian 256^2 -- below launch latency, so every
+    kernel is timed as graph replays of 50 SpMVs."""
     from oracle import oracle as orc
-    # config 1: 2D Laplacian 256^2, below launch latency -> graph batches
-    m1 = ax.laplace2d(256)
-    x1 = ax.uniform(0, m1.ncols)
+    m1, x1 = W.config1()
     y1 = orc.ncsr(m1, x1)
     d1, e1 = ax.DeviceBuffer.from_array(x1), ax.DeviceBuffer(8 * m1.nrows)
-    c1 = []
+    rows = []
     for name, fmt, kw in (("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
                           ("csr_vector", ax.FMT_CSR, dict(kernel=ax.KER_STREAM)),
                           ("axt_unc_h1_thw4", ax.FMT_AXT, dict(thw=4)), ("axt_unc_h4_thw64", ax.FMT_AXT, dict(thw=64, th=4)),
+                          ("axt_unc_h1_thw4_ax", ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_AX_LAYOUT)),
                           ("axc_hbs4", ax.FMT_AXC, dict(hbs=4)), ("k1_c64", ax.FMT_K1, dict(chunk=64)),
+                          ("k1_c64_natural", ax.FMT_K1, dict(chunk=64, reorder=0)),
+                          ("axt_com_h1_thw64_bs256", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=256)),
                           ("axt_com_h4_thw64", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)),
                           ("rct_auto", ax.FMT_RCT, dict())):
         _, _, r = measure(m1, name, fmt, kw, d1, e1, 20, 2, y1, mode=TIME_GRAPH, batch=50)
-        c1.append(r)
+        rows.append(r)
         log(f"laplace256: {r}")
-    res["laplace256"] = c1
-    res["laplace256_rocsparse"] = rocsparse_rows(m1, d1, y1, iters=200)
-    # config 3: 27-point stencil 128^3, permuted
-    m3 = ax.stencil27(128, permute=True, seed=2)
-    x3 = ax.uniform(2, m3.ncols)
+    tables["laplace256"] = dict(config="C1: 2D 5-point Laplacian 256^2, x ~ U[-1,1] seed 0; graph-timed (50 SpMVs "
+                                       "per replay)", nrows=m1.nrows, nnz=m1.nnz,
+                                formats=formats_table(m1, x1, d1), kernels=rows)
+    extras["laplace256_rocsparse"] = rocsparse_rows(m1, d1, y1, iters=200)
+
+
+def suite_c3_c5(tables, extras):
+    """Config 3 (the permuted 27-point stencil on 128^3, every format) and
+    config 5 (forward + adjoint on its structure with values N(1, 0.1))."""
+    from oracle import oracle as orc
+    m3, x3 = W.config3()
     y3 = orc.ncsr(m3, x3, nthreads=omp_threads())
     d3, e3 = ax.DeviceBuffer.from_array(x3), ax.DeviceBuffer(8 * m3.nrows)
-    c3 = []
-    for name, fmt, kw in [FLAGSHIP, FLAGSHIP_AXT] + OTHERS[1:14]:
+    rows = []
+    c3 = [o for o in OTHERS if not o[0].startswith("rct_r") and not o[0].endswith("f32val")]
+    for name, fmt, kw in [FLAGSHIP] + c3:
         _, _, r = measure(m3, name, fmt, kw, d3, e3, 20, 3, y3)
-        c3.append(r)
+        rows.append(r)
         log(f"stencil128p: {r}")
-    res["stencil128p"] = c3
-    res["stencil128p_rocsparse"] = rocsparse_rows(m3, d3, y3, iters=20)
+    tables["stencil128p"] = dict(config="C3: 27-point stencil 128^3, symmetric random permutation (seed 2), "
+                                        "x ~ U[-1,1] seed 2", nrows=m3.nrows, nnz=m3.nnz,
+                                 formats=formats_table(m3, x3, d3), kernels=rows)
+    extras["stencil128p_rocsparse"] = rocsparse_rows(m3, d3, y3, iters=20)
+    del e3
     # config 5: forward + adjoint on the stencil structure, values N(1, 0.1)
-    ax.fill_values_normal(m3, 1.0, 0.1, 4)
-    y5 = orc.ncsr(m3, x3, nthreads=omp_threads())
-    pl = ax.Plan(m3, ax.FMT_AXT, thw=4)
-    pl.time(d3, e3, iters=3, warmup=0, mode=TIME_REGION)
-    plain = pl.time(d3, e3, iters=20, warmup=0, mode=TIME_REGION).mean_ms * 1e3
-    del pl
-    res["stencil128p_fwd_adj"] = fwd_adj_only(m3, x3, d3, y5, plain)
-    b5 = ax.uniform(4, m3.nrows, stream=ax.VEC_STREAM + 1)
-    res["stencil128p_fwd_adj_rct"] = fwd_adj_rct(m3, d3, ax.DeviceBuffer.from_array(b5), y5, b5)
-    log(f"stencil fwd_adj_rct: {res['stencil128p_fwd_adj_rct']}")
-    del m3, d3, e3
-    # config 4B: 256 uneven banded matrices in one persistent launch
-    rng = np.random.default_rng(5)
-    mats = []
-    for i in range(256):
-        mats.append(ax.banded(int(rng.integers(1000, 200001)), int(rng.integers(1, 65)), 0.3, seed=5000 + i))
-    # the built-in candidates: AXT UNC per shape group against the RCT
-    # block-diagonal plan for the whole batch
+    m5, x5, b5 = W.config5(m3)
+    del m3
+    y5 = orc.ncsr(m5, x5, nthreads=omp_threads())
+    d5 = ax.DeviceBuffer.from_array(x5)
+    e5 = ax.DeviceBuffer(8 * m5.nrows)
+    pl = ax.Plan(m5, ax.FMT_AXT, thw=4)
+    pl.time(d5, e5, iters=3, warmup=0, mode=TIME_REGION)
+    plain = pl.time(d5, e5, iters=20, warmup=0, mode=TIME_REGION).mean_ms * 1e3
+    del pl, e5
+    extras["stencil128p_fwd_adj"] = fwd_adj_only(m5, x5, d5, y5, plain, b5)
+    extras["stencil128p_fwd_adj_rct"] = fwd_adj_rct(m5, d5, ax.DeviceBuffer.from_array(b5), y5, b5)
+    log(f"stencil fwd_adj_rct: {extras['stencil128p_fwd_adj_rct']}")
+
+
+def suite_c4b(extras):
+    """Config 4B: 256 uneven banded matrices in one launch -- the AXT
+    persistent batch with uniform tiles, with the autotuner's per-group tiles
+    (or its RCT choice), and the RCT block-diagonal plan."""
+    from oracle import oracle as orc
+    mats = W.config4b()
     t0 = time.perf_counter()
     tuned, ngroups = ax.autotune_batch(mats, iters=5)
     tune_s = round(time.perf_counter() - t0, 1)
-    res["batch256"] = {}
+    out = {}
     for label, make in (("uniform_thw4", lambda: ax.Batch(mats, thw=4)),
                         ("autotuned", lambda: ax.Batch(mats, opts_list=tuned)),
                         ("rct_block_diagonal", lambda: ax.Batch(mats, fmt=ax.FMT_RCT))):
@@ -438,18 +547,20 @@ def suite_full(res):
         t = bt.time(dxb, dyb, iters=10, warmup=2, mode=TIME_REGION)
         tg = bt.time(dxb, dyb, iters=5, warmup=1, mode=TIME_GRAPH, batch=10)
         sec = t.mean_ms * 1e-3
-        res["batch256"][label] = dict(matrices=256, nnz=info["nnz"], items=info["nitems"], waves=info["nwaves"],
-                                      us=round(sec * 1e6, 1), graph_us=round(tg.mean_ms * 1e3, 1),
-                                      alg_gbs=round(info["alg_bytes"] / sec / 1e9, 1),
-                                      frac=round(info["alg_bytes"] / sec / 1e9 / PEAK_HBM_GBS, 4),
-                                      max_rel_err_sampled=max(errs), build_s=build_s)
+        out[label] = dict(matrices=256, nnz=info["nnz"], items=info["nitems"], waves=info["nwaves"],
+                          us=round(sec * 1e6, 1), graph_us=round(tg.mean_ms * 1e3, 1),
+                          gflops=round(2.0 * info["nnz"] / sec / 1e9, 1),
+                          alg_gbs=round(info["alg_bytes"] / sec / 1e9, 1),
+                          frac=round(info["alg_bytes"] / sec / 1e9 / PEAK_HBM_GBS, 4),
+                          max_rel_err_sampled=max(errs), members_checked=len(errs), build_s=build_s)
         if label == "autotuned":
-            res["batch256"][label].update(
+            out[label].update(
                 tuner_option_sets=ngroups, tuner_s=tune_s,
                 chosen="rct" if tuned[0].format == ax.FMT_RCT else
                 f"axt per group, (thw, th) used: {sorted({(o.thw, o.th) for o in tuned})}")
-        log(f"batch256 {label}: {res['batch256'][label]}")
+        log(f"batch256 {label}: {out[label]}")
         del dxb, dyb, bt
+    extras["batch256"] = out
 
 
 def fwd_adj_rct(m, dx, db, yref, b):
@@ -474,10 +585,13 @@ def fwd_adj_rct(m, dx, db, yref, b):
     return out
 
 
-def fwd_adj_only(m, x, dx, yref, plain_us):
+def fwd_adj_only(m, x, dx, yref, plain_us, b=None):
+    """Forward + adjoint on the AXT UNC th = 1 plan: fused (y, r and v_bar in
+    one sweep, then x_bar = A^T r) and split (four kernels)."""
     from oracle import oracle as orc
     n, nc, nnz = m.nrows, m.ncols, m.nnz
-    b = ax.uniform(4, n, stream=ax.VEC_STREAM + 1)
+    if b is None:
+        b = ax.uniform(4, n, stream=ax.VEC_STREAM + 1)
     p = ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_TRANSPOSE)
     db = ax.DeviceBuffer.from_array(b)
     bufs = dict(y=ax.DeviceBuffer(8 * n), r=ax.DeviceBuffer(8 * n), xbar=ax.DeviceBuffer(8 * nc),
@@ -489,7 +603,7 @@ def fwd_adj_only(m, x, dx, yref, plain_us):
     r = bufs["r"].download(np.float64, n)
     xbar = bufs["xbar"].download(np.float64, nc)
     r_ref = yref - b
-    out = dict(plain_spmv_us=round(plain_us, 2), fused_us=fa["fused"], split_us=fa["split"],
+    out = dict(plan=p.info()["name"], plain_spmv_us=round(plain_us, 2), fused_us=fa["fused"], split_us=fa["split"],
                fused_over_plain=round(fa["fused"] / plain_us, 3), r_rel_err=rel_err(r_ref, r),
                xbar_rel_err=rel_err(orc.ncsr_t(m, r_ref, nc), xbar))
     log(f"stencil fwd_adj: {out}")
@@ -502,7 +616,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--formats", choices=["flagship", "all"], default="all")
-    ap.add_argument("--suite", choices=["headline", "full"], default="headline")
+    ap.add_argument("--suite", choices=["all", "headline"], default="all")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
@@ -519,10 +633,9 @@ def main():
         pmc_probe(args.pmc_probe)
         return
 
-    t0 = time.perf_counter()
-    m = ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
-    x = ax.uniform(1, m.ncols)
-    log(f"matrix: n={m.nrows} nnz={m.nnz} ({time.perf_counter() - t0:.1f} s)")
+    t_start = time.perf_counter()
+    m, x = W.config2()
+    log(f"matrix: n={m.nrows} nnz={m.nnz} ({time.perf_counter() - t_start:.1f} s)")
 
     yref, cpu = None, None
     if not args.no_cpu:
@@ -536,10 +649,10 @@ def main():
     log(f"flagship: {flag}")
 
     # The timed region: K SpMVs back to back, device-synchronized on both
-    # sides.  HIP events on the launch stream mark every SpMV and the end of
-    # its first (main) kernel, so the main kernel's own mean comes out of the
-    # same region (TIME_KERNEL); for RCT that is the tile kernel, the second
-    # launch being the slice merge.
+    # sides.  HIP events on the launch stream mark the start of every SpMV
+    # and, for a sliced RCT plan (a clear of y, then the tile kernel), the
+    # start of its tile kernel, so the whole SpMV's mean, its spread and the
+    # tile kernel's own mean come out of the same region (TIME_KERNEL).
     plan.time(dx, dy, iters=max(args.warmup, 1), warmup=0, mode=TIME_REGION)
     L.call("axspmv_device_sync")
     w0 = time.perf_counter()
@@ -548,17 +661,18 @@ def main():
     wall = time.perf_counter() - w0
     kern_s = t.main_ms * 1e-3
     path_s = t.mean_ms * 1e-3
-    achieved = info["alg_bytes"] / kern_s / 1e9
+    achieved = info["alg_bytes"] / path_s / 1e9
     gflops = 2.0 * m.nnz / (wall / args.steps) / 1e9
     # the same SpMV with the Infinity Cache and L2 flushed before each launch
     tc = plan.time(dx, dy, iters=20, warmup=0, mode=TIME_COLD)
     cold = dict(spmv_us_mean=round(tc.mean_ms * 1e3, 2), spmv_us_median=round(tc.median_ms * 1e3, 2),
-                alg_gbs=round(info["alg_bytes"] / (tc.median_ms * 1e-3) / 1e9, 1))
+                alg_gbs=round(info["alg_bytes"] / (tc.median_ms * 1e-3) / 1e9, 1),
+                frac=round(info["alg_bytes"] / (tc.median_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4))
     log(f"cold: {cold}")
     del plan
 
     formats = [flag]
-    extras = {}
+    extras, tables = {}, {}
     if args.formats == "all":
         for (n2, f2, k2) in OTHERS:
             try:
@@ -567,22 +681,33 @@ def main():
                 r = dict(label=n2, error=str(e))
             log(f"{n2}: {r}")
             formats.append(r)
+        tables["powerlaw"] = dict(config="C2: power-law n = 2^20, Zipf(2) row lengths on {1..2048}, mean 20, "
+                                         "x ~ U[-1,1] seed 1", nrows=m.nrows, nnz=m.nnz,
+                                  formats=formats_table(m, x, dx), kernels=formats)
         plain = next(f["us"] for f in formats if f.get("label") == FLAGSHIP_AXT[0])
-        extras = extras_same_matrix(m, x, dx, yref, plain)
+        extras.update(extras_same_matrix(m, x, dx, yref, plain))
         extras["conversion"] = conversion(m, x, dx)
         extras["rocsparse"] = rocsparse_rows(m, dx, yref)
-    if args.suite == "full":
-        suite_full(extras)
+    del dx, dy
+    if args.suite == "all":
+        del m
+        suite_c1(tables, extras)
+        suite_c3_c5(tables, extras)
+        suite_c4b(extras)
 
     traffic, traffic_src, traffic_kernel = None, None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_powerlaw_flagship.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
         entry = tj.get(info["name"], {})
-        traffic = entry.get("hbm_bytes_per_launch")
-        traffic_kernel = entry.get("kernel")
-        traffic_src = os.path.relpath(tpath, ROOT) if traffic is not None else None
+        if "hbm_bytes_per_launch" in entry:
+            # every launch of one SpMV: the tile kernel and, with slices, the y clear
+            clear = entry.get("clear_kernel", {})
+            traffic = entry["hbm_bytes_per_launch"] + clear.get("fetch_bytes", 0) + clear.get("write_bytes", 0)
+            traffic_kernel = entry.get("kernel")
+            traffic_src = os.path.relpath(tpath, ROOT)
 
+    us = lambda v: round(v * 1e3, 2)  # noqa: E731  (ms -> us)
     out = {
         "metric": "SpMV GFLOP/s + HBM GB/s (% of 8 TB/s) per format; fwd+adjoint vs plain",
         "value": round(gflops, 2),
@@ -597,19 +722,22 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded power-law matrix generated on the host)",
         "config": {"workload": "powerlaw n=2^20 Zipf(2) rl<=2048 mean 20 seed 1, FP64 SpMV",
-                   "nrows": m.nrows, "nnz": m.nnz, "kernel": info["name"],
+                   "nrows": info["nrows"], "nnz": info["nnz"], "kernel": info["name"],
                    "alg_bytes_per_spmv": info["alg_bytes"], "parity_rel_err": flag["rel_err"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "scope": "whole SpMV (every launch of one SpMV: y clear + tile kernel for a sliced RCT plan)",
+                     "spmv_us": round(path_s * 1e6, 2), "spmv_us_median": us(t.median_ms),
+                     "spmv_us_min": us(t.min_ms), "spmv_us_max": us(t.max_ms),
                      "kernel": traffic_kernel or info["name"], "kernel_us": round(kern_s * 1e6, 2),
-                     "alg_bytes_per_launch": info["alg_bytes"], "launches_per_spmv": info["launches"],
-                     "spmv_us": round(path_s * 1e6, 2),
-                     "spmv_frac": round(info["alg_bytes"] / path_s / 1e9 / PEAK_HBM_GBS, 4),
-                     "cold_caches": cold,
-                     "traffic_source": traffic_src},
+                     "kernel_frac": round(info["alg_bytes"] / kern_s / 1e9 / PEAK_HBM_GBS, 4),
+                     "alg_bytes_per_spmv": info["alg_bytes"], "launches_per_spmv": info["launches"],
+                     "cold_caches": cold, "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "formats": formats,
+        "tables": tables,
         "extras": extras,
+        "bench_s": round(time.perf_counter() - t_start, 1),
     }
     print(json.dumps(out), flush=True)
 

@@ -39,6 +39,7 @@ PLAN_TRANSPOSE = 1
 PLAN_RCT_SPLIT = 2
 PLAN_AX_LAYOUT = 4
 PLAN_STAGE_X = 8
+PLAN_DETERMINISTIC = 16
 
 
 class Csr(C.Structure):

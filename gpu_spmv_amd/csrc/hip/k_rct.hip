@@ -243,20 +243,6 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     }
 }
 
-// y[0 .. n) = 0 with 16-byte stores (y 16-byte aligned; VEC = false: 8-byte)
-template <bool VEC>
-__global__ void __launch_bounds__(256) rct_clear(double *__restrict__ y, size_t n) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if constexpr (VEC) {
-        if (2 * i + 1 < n)
-            reinterpret_cast<double2 *>(y)[i] = double2{0.0, 0.0};
-        else if (2 * i < n)
-            y[2 * i] = 0.0;
-    } else {
-        if (i < n) y[i] = 0.0;
-    }
-}
-
 // Value gradient in entry order: vbar[e] = ybar[row(e)] * x[col(e)].  The
 // tile's ybar rows sit in LDS; x is gathered in the tile's column order.
 template <bool PACKED>
@@ -371,11 +357,8 @@ namespace launch {
 hipError_t rct_spmm(const RctArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s) {
     if (!a.nrows || !k) return hipSuccess;
     if (a.S > 1) {
-        const size_t n = size_t(a.nrows) * k;
-        if (reinterpret_cast<uintptr_t>(Y) % 16 == 0)
-            rct_clear<true><<<unsigned((n / 2 + 256) / 256), 256, 0, s>>>(Y, n);
-        else
-            rct_clear<false><<<unsigned((n + 255) / 256), 256, 0, s>>>(Y, n);
+        const hipError_t e = clear_f64(Y, size_t(a.nrows) * k, s);
+        if (e != hipSuccess) return e;
     }
     const bool packed = a.key != nullptr;
     if (a.vtype == 1)
@@ -404,10 +387,8 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
     const bool vb = vbar != nullptr;
     const size_t lds = size_t(a.R) * sizeof(double) * (vb ? 2 : 1);
     if (a.S > 1) {
-        if (reinterpret_cast<uintptr_t>(y) % 16 == 0)
-            rct_clear<true><<<(a.nrows / 2 + 256) / 256, 256, 0, s>>>(y, a.nrows);
-        else
-            rct_clear<false><<<(a.nrows + 255) / 256, 256, 0, s>>>(y, a.nrows);
+        const hipError_t e = clear_f64(y, a.nrows, s);
+        if (e != hipSuccess) return e;
         if (mid) {
             const hipError_t e = hipEventRecord(mid, s);
             if (e != hipSuccess) return e;

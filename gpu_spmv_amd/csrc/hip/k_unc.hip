@@ -49,6 +49,50 @@ __global__ void __launch_bounds__(256) rowblock_kernel(Src src, const Block *__r
     rowblock_process(src, b, XGlobal{x}, y);
 }
 
+// Column-sliced AXT UNC th == 1: each column slice of the matrix is its own
+// AXT layout (planes concatenated); the host lists the wave blocks so that
+// the four waves of workgroup g all take blocks of slice (g % 8) % S, and
+// workgroups go to the 8 XCDs round-robin, so each XCD's L2 holds only its
+// slice of x.  A block adds its rows' slice sums into y (cleared first)
+// with FP64 atomics: S adds per row at most.
+template <class Src>
+__global__ void __launch_bounds__(256) rowblock_slices_kernel(Src src, const Block *__restrict__ blocks,
+                                                              uint32_t nblocks, const double *__restrict__ x,
+                                                              double *__restrict__ y) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave >= nblocks) return;
+    rowblock_process<Src, XGlobal, true>(src, blocks[wave], XGlobal{x}, y);
+}
+
+// The deterministic form: slice s's row sums are stored (not added) into
+// the partial vector part + s * nrows (s = (g % 8) % S for workgroup g, as
+// the dispatch list assigns), rows without entries in a slice get 0 there
+// (`empty` lists s * nrows + r), and part_merge adds the S partials of every
+// row in slice order.
+template <class Src>
+__global__ void __launch_bounds__(256) rowblock_partial_kernel(Src src, const Block *__restrict__ blocks,
+                                                               uint32_t nblocks, const uint32_t *__restrict__ empty,
+                                                               uint32_t nempty, uint32_t S, uint32_t nrows,
+                                                               const double *__restrict__ x,
+                                                               double *__restrict__ part) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = wave * 64 + lane; i < nempty; i += nwaves * 64) part[empty[i]] = 0.0;
+    if (wave >= nblocks) return;
+    double *ys = part + size_t((blockIdx.x % 8) % S) * nrows;
+    rowblock_process<Src, XGlobal, false>(src, blocks[wave], XGlobal{x}, ys);
+}
+
+__global__ void __launch_bounds__(256) part_merge(const double *__restrict__ part, uint32_t S, uint32_t nrows,
+                                                  double *__restrict__ y) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    double v = part[r];
+    for (uint32_t k = 1; k < S; ++k) v += part[size_t(k) * nrows + r];
+    y[r] = v;
+}
+
 // ------------------------------------------------------------------ adjoint
 // Value gradient of the AXT UNC th == 1 layout: plane slot q of row r is CSR
 // position rowptr[r] + q - first[r] (first[r] = first plane slot of row r);
@@ -147,6 +191,22 @@ hipError_t run(const Src &src, const axspmv::launch::SegArgs &a, const double *x
     const uint32_t block = 256;  // 4 waves: the LDS windows are indexed by wave
     uint32_t waves = a.nblocks ? a.nblocks : 1;
     const unsigned grid = (waves + block / 64 - 1) / (block / 64);
+    if (a.part) {  // column slices, deterministic: partials, then their sum in slice order
+        rowblock_partial_kernel<Src><<<grid, block, 0, s>>>(src, a.blocks, a.nblocks, a.empty, a.nempty, a.S,
+                                                            a.nrows_add, x, a.part);
+        if (a.nrows_add) part_merge<<<(a.nrows_add + 255) / 256, 256, 0, s>>>(a.part, a.S, a.nrows_add, y);
+        return hipGetLastError();
+    }
+    if (a.nrows_add) {  // column slices: clear y, then add every slice's row sums
+        const hipError_t c = axspmv::launch::clear_f64(y, a.nrows_add, s);
+        if (c != hipSuccess) return c;
+        if (a.mid) {
+            const hipError_t e = hipEventRecord(a.mid, s);
+            if (e != hipSuccess) return e;
+        }
+        rowblock_slices_kernel<Src><<<grid, block, 0, s>>>(src, a.blocks, a.nblocks, x, y);
+        return hipGetLastError();
+    }
     if (a.win)
         rowblock_kernel<Src, true><<<grid, block, 0, s>>>(src, a.blocks, a.nblocks, a.empty, a.nempty, a.win, x, y);
     else

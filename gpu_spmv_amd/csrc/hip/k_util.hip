@@ -20,6 +20,22 @@ __global__ void __launch_bounds__(256) fill_kernel(double *__restrict__ dst, siz
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
 }
 
+// y[0 .. n) = 0, one lane per 16 bytes (VEC: y 16-byte aligned) or per
+// element: the clear before the atomic slice adds of the sliced plans (one
+// pass over y, no grid-stride loop).
+template <bool VEC>
+__global__ void __launch_bounds__(256) clear_kernel(double *__restrict__ y, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if constexpr (VEC) {
+        if (2 * i + 1 < n)
+            reinterpret_cast<double2 *>(y)[i] = double2{0.0, 0.0};
+        else if (2 * i < n)
+            y[2 * i] = 0.0;
+    } else {
+        if (i < n) y[i] = 0.0;
+    }
+}
+
 __global__ void __launch_bounds__(256) fill_rows_kernel(const uint32_t *__restrict__ rows, uint32_t n, double *__restrict__ y) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[rows[i]] = 0.0;
@@ -123,6 +139,15 @@ hipError_t wave_selftest(uint32_t op, const double *v, const uint32_t *h, double
 
 hipError_t touch(const double *p, size_t n, double *sink, hipStream_t s) {
     if (n / 2) touch_kernel<<<grid_for(n / 2, 256), 256, 0, s>>>(reinterpret_cast<const double2 *>(p), n / 2, sink);
+    return hipGetLastError();
+}
+
+hipError_t clear_f64(double *y, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    if (reinterpret_cast<uintptr_t>(y) % 16 == 0)
+        clear_kernel<true><<<unsigned((n / 2 + 256) / 256), 256, 0, s>>>(y, n);
+    else
+        clear_kernel<false><<<unsigned((n + 255) / 256), 256, 0, s>>>(y, n);
     return hipGetLastError();
 }
 

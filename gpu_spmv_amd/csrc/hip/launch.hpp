@@ -33,6 +33,16 @@ struct SegArgs {
     uint32_t vtype;           // 0 = f64 values, 1 = f32 values
     uint32_t block;           // threads per workgroup
     const Window *win;        // per block (nullptr: x from HBM for every block)
+    uint32_t nrows_add;       // AXT UNC th == 1 in column slices: y (this many
+                              // rows) is cleared, then every block adds its
+                              // rows' sums; 0: blocks store whole rows
+    hipEvent_t mid;           // column slices with atomics: recorded between
+                              // the clear and the block kernel (nullptr: none)
+    double *part;             // column slices, deterministic: S x nrows_add
+                              // partial sums (empty lists s * nrows + r of the
+                              // rows without entries in slice s), summed in
+                              // slice order by a merge kernel
+    uint32_t S;               // column slices (with part)
 };
 hipError_t axt_unc_h1(const SegArgs &a, const double *x, double *y, hipStream_t s);
 // As axt_unc_h1 on the interleaved layout: a.vplane is the FP64 ax array
@@ -204,6 +214,8 @@ hipError_t build_fill(const DevCsr &a, int kind, const uint64_t *start, const Bu
 
 hipError_t copy_f64(const double *src, double *dst, size_t n, hipStream_t s);
 hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s);
+// y[0 .. n) = 0 in one pass (16-byte stores where y is 16-byte aligned)
+hipError_t clear_f64(double *y, size_t n, hipStream_t s);
 // reads p[0 .. n) (n even, p 16-byte aligned): evicts caches without dirtying them
 hipError_t touch(const double *p, size_t n, double *sink, hipStream_t s);
 // the wave64 primitive op (enum axspmv_wave_op) lane by lane; n a multiple of 64

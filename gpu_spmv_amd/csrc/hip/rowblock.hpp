@@ -202,7 +202,17 @@ struct UncHSrc {
     __device__ __forceinline__ uint32_t row(uint32_t g) const { return sec[g]; }
 };
 
-template <class Src, class XA>
+// ADD: the block holds the row sums of one column slice; they are added to
+// y (cleared beforehand) with FP64 atomics instead of stored.
+template <bool ADD>
+__device__ __forceinline__ void put_row(double *__restrict__ y, uint32_t r, double v) {
+    if constexpr (ADD)
+        unsafeAtomicAdd(y + r, v);
+    else
+        y[r] = v;
+}
+
+template <class Src, class XA, bool ADD = false>
 __device__ __forceinline__ void rowblock_process(const Src &src, const Block b, const XA &x, double *__restrict__ y) {
     const uint32_t lane = threadIdx.x & 63;
     uint32_t crow = NOROW;  // row of the segment still open at the end of the previous 64 units
@@ -219,7 +229,7 @@ __device__ __forceinline__ void rowblock_process(const Src &src, const Block b, 
         // The carried segment ends here if these units start with another row.
         const uint32_t r0 = readfirst_u32(r);
         if (crow != NOROW && r0 != crow) {
-            if (lane == 0) y[crow] = carry;
+            if (lane == 0) put_row<ADD>(y, crow, carry);
             crow = NOROW;
             carry = 0.0;
         }
@@ -229,7 +239,7 @@ __device__ __forceinline__ void rowblock_process(const Src &src, const Block b, 
         const bool last = u0 + 64 >= b.end;
         bool end = act && shift_down_u32(r, NOROW) != r;
         if (!last && lane == 63) end = false;
-        if (end) y[r] = v;
+        if (end) put_row<ADD>(y, r, v);
         if (!last) {
             crow = readlane_u32(r, 63);
             carry = readlane_f64(v, 63);

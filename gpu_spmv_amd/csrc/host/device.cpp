@@ -240,7 +240,7 @@ int time_kernel(axspmv_plan *p, const axspmv_operands &o, uint32_t iters, uint32
     }
     if (e == hipSuccess) e = hipEventCreate(&k[iters]);
     if (e != hipSuccess) return hip_fail(e, "timing: event creation");
-    const bool marks = p->kind == Plan::RCT && p->rct.S > 1;
+    const bool marks = (p->kind == Plan::RCT && p->rct.S > 1) || (p->kind == Plan::AXT_UNC_H1_SLICED && !p->d_part);
     for (uint32_t i = 0; i < warmup && e == hipSuccess; ++i) e = plan_launch(p, o.x, o.y, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) {

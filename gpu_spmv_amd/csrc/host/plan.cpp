@@ -44,6 +44,7 @@ int axspmv_plan::upload(void *dst, const void *src, size_t bytes, hipStream_t s)
 
 axspmv_plan::~axspmv_plan() {
     delete T;
+    delete Tdet;
     for (void *p : bufs) (void)hipFree(p);
 }
 
@@ -332,12 +333,123 @@ int upload_ax_layout(Plan &p, const axspmv_csr *a, const axspmv_axt *m, hipStrea
     return AXSPMV_OK;
 }
 
+// AXT UNC th == 1 in S column slices: slice k holds the entries of columns
+// [k C, (k + 1) C) (C = ceil(ncols / S)) and is the AXT layout of that part
+// of the matrix, built by the same builder (global column indices, so x is
+// indexed directly).  The slices' planes and sec arrays are concatenated;
+// every slice gets its own row-aligned wave blocks (unit offsets moved to the
+// concatenated planes).  The dispatch list gives workgroup g (four waves,
+// four blocks) slice (g % 8) % S, and workgroups run on XCD g % 8, so an XCD
+// only gathers from its slice of x -- 2 MB at S = 4 on the 2^20-row matrix,
+// resident in its 4 MB L2.  Rows accumulate across slices with FP64 atomic
+// adds into a y cleared first.
+int plan_axt_sliced(Plan &p, const axspmv_csr *a, hipStream_t s) {
+    const uint32_t thw = p.o.thw, S = p.o.slices, n = a->nrows;
+    if (S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_UNSUPPORTED, "AXT column slices must be 2, 4 or 8, got %u", S);
+    const uint64_t C = ceil_div(std::max<uint32_t>(a->ncols, 1), S);
+    const uint32_t K = std::min<uint32_t>(4, thw);
+    const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
+    // entries of each row by slice
+    std::vector<uint32_t> cnt(size_t(S) * n, 0);
+    for (uint32_t r = 0; r < n; ++r)
+        for (uint32_t j = a->row[r]; j < a->row[r + 1]; ++j) cnt[size_t(a->col[j] / C) * n + r]++;
+    std::vector<double> vcat;
+    std::vector<uint32_t> ccat, scat;
+    std::vector<std::vector<launch::Block>> sblocks(S);
+    uint64_t units_before = 0;
+    for (uint32_t k = 0; k < S; ++k) {
+        axspmv_csr sub{};
+        sub.nrows = n;
+        sub.ncols = a->ncols;
+        std::vector<uint32_t> row(size_t(n) + 1, 0), col;
+        std::vector<double> val;
+        for (uint32_t r = 0; r < n; ++r) row[r + 1] = row[r] + cnt[size_t(k) * n + r];
+        col.resize(row[n]);
+        val.resize(row[n]);
+        for (uint32_t r = 0; r < n; ++r) {
+            uint32_t q = row[r];
+            for (uint32_t j = a->row[r]; j < a->row[r + 1]; ++j)
+                if (a->col[j] / C == k) {
+                    col[q] = a->col[j];
+                    val[q++] = a->val[j];
+                }
+        }
+        sub.nnz = row[n];
+        sub.row = row.data();
+        sub.col = col.data();
+        sub.val = val.data();
+        axspmv_axt *m = nullptr;
+        AX_TRY(axspmv_build_axt(&sub, nullptr, p.o.bs, thw, 1, AXSPMV_AXT_UNC,
+                               AXSPMV_BUILD_PLANES | AXSPMV_BUILD_PAD_COL, &m, nullptr));
+        std::unique_ptr<axspmv_axt, void (*)(axspmv_axt *)> guard(m, axspmv_axt_free);
+        if (vcat.size() + m->lenAX / 2 >= (1ull << 32))
+            return fail(AXSPMV_E_OVERFLOW, "AXT slices: %llu slots exceed 32 bits",
+                        (unsigned long long)(vcat.size() + m->lenAX / 2));
+        vcat.insert(vcat.end(), m->vplane, m->vplane + m->lenAX / 2);
+        ccat.insert(ccat.end(), m->cplane, m->cplane + m->lenAX / 2);
+        scat.insert(scat.end(), m->sec, m->sec + m->lenSEC);
+        std::vector<uint32_t> units(n);
+        for (uint32_t r = 0; r < n; ++r) units[r] = uint32_t(ceil_div(cnt[size_t(k) * n + r], thw) * (thw / K));
+        sblocks[k] = row_blocks(units, p.o.budget ? p.o.budget : 64);
+        for (launch::Block &b : sblocks[k]) {
+            b.beg += uint32_t(units_before);
+            b.end += uint32_t(units_before);
+        }
+        units_before += uint64_t(m->lenAX / 2) / K;
+    }
+    // dispatch list: workgroup g takes blocks 4i .. 4i + 3 of slice (g % 8) % S,
+    // i = (g / 8) * (8 / S) + (g % 8) / S; missing blocks are empty
+    size_t most = 0;
+    for (const auto &b : sblocks) most = std::max(most, b.size());
+    const uint32_t per = 8 / S;
+    const uint64_t groups = ceil_div(ceil_div(most, 4), per) * 8;
+    std::vector<launch::Block> list(groups * 4, launch::Block{0, 0});
+    for (uint64_t g = 0; g < groups; ++g) {
+        const uint32_t xcd = uint32_t(g % 8), k = xcd % S;
+        const uint64_t i = (g / 8) * per + xcd / S;
+        for (uint32_t w = 0; w < 4; ++w)
+            if (4 * i + w < sblocks[k].size()) list[4 * g + w] = sblocks[k][4 * i + w];
+    }
+    if (list.size() >= (1ull << 32) / 64) return fail(AXSPMV_E_OVERFLOW, "AXT slices: too many wave blocks");
+    AX_TRY(upload_planes(p, vcat.data(), ccat.data(), vcat.size(), s));
+    p.d_sec = static_cast<uint32_t *>(p.dalloc(4ull * scat.size()));
+    if (!p.d_sec) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(p.d_sec, scat.data(), 4ull * scat.size(), s));
+    // Deterministic: slice partials stored in part[k * n + r] (0 for the rows
+    // without entries in slice k), summed in slice order by a merge kernel.
+    const bool det = p.o.flags & AXSPMV_PLAN_DETERMINISTIC;
+    std::vector<uint32_t> empty;
+    if (det) {
+        if (uint64_t(S) * n >= (1ull << 32)) return fail(AXSPMV_E_OVERFLOW, "AXT slices: S x nrows exceeds 32 bits");
+        for (uint32_t k = 0; k < S; ++k)
+            for (uint32_t r = 0; r < n; ++r)
+                if (!cnt[size_t(k) * n + r]) empty.push_back(k * n + r);
+        p.d_part = static_cast<double *>(p.dalloc(8ull * S * n));
+        if (!p.d_part) return AXSPMV_E_NOMEM;
+    }
+    AX_TRY(upload_blocks(p, list, empty, s));
+    p.thw = thw;
+    p.th = 1;
+    p.slices = S;
+    p.slots = vcat.size();
+    p.stream_bytes = (vs + 4) * p.slots + 4ull * scat.size() + 8ull * list.size();
+    p.launches = 2;
+    p.kind = Plan::AXT_UNC_H1_SLICED;
+    std::snprintf(p.name, sizeof p.name, "axt_unc_h1_thw%u_s%u%s", thw, S, det ? "_det" : "");
+    return AXSPMV_OK;
+}
+
 int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
     const uint32_t thw = p.o.thw, th = p.o.th;
     if (!is_pow2(thw) || thw > 64) return fail(AXSPMV_E_UNSUPPORTED, "AXT kernels need thw a power of two <= 64, got %u", thw);
     const bool axl = p.o.flags & AXSPMV_PLAN_AX_LAYOUT;
     if (axl && (th != 1 || p.o.mode != AXSPMV_AXT_UNC || p.o.vtype != AXSPMV_F64))
         return fail(AXSPMV_E_UNSUPPORTED, "AX layout: AXT UNC th = 1 with FP64 values only");
+    if (p.o.slices > 1) {
+        if (th != 1 || p.o.mode != AXSPMV_AXT_UNC || axl || (p.o.flags & AXSPMV_PLAN_STAGE_X))
+            return fail(AXSPMV_E_UNSUPPORTED, "AXT column slices: UNC th = 1 only, without the AX layout or x staging");
+        return plan_axt_sliced(p, a, s);
+    }
     if (p.o.mode == AXSPMV_AXT_COM) return plan_axt_com(p, a, s);
     axspmv_axt *m = nullptr;
     AX_TRY(axspmv_build_axt(a, nullptr, p.o.bs, thw, th, p.o.mode, AXSPMV_BUILD_PLANES | AXSPMV_BUILD_PAD_COL, &m,
@@ -869,6 +981,14 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
             return launch::csr_vector(p->nrows, p->d_val, p->o.vtype, p->d_col, p->d_row, p->lanes, p->spmm.long_slots,
                                       p->spmm.longrows, p->spmm.nlong, x, y, s);
         case Plan::AXT_UNC_H1: return launch::axt_unc_h1(seg_args(*p, p->d_sec), x, y, s);
+        case Plan::AXT_UNC_H1_SLICED: {
+            launch::SegArgs g = seg_args(*p, p->d_sec);
+            g.nrows_add = p->nrows;
+            g.mid = p->d_part ? nullptr : p->mid;
+            g.part = p->d_part;
+            g.S = p->slices;
+            return launch::axt_unc_h1(g, x, y, s);
+        }
         case Plan::AXT_UNC_H: return launch::axt_unc_h(seg_args(*p, p->d_sec), x, y, s);
         case Plan::AXT_COM: return launch::axt_com(p->com, x, y, s);
         case Plan::K1: return launch::k1(p->k1, x, y, s);
@@ -965,9 +1085,33 @@ int plan_add_transpose(Plan *p, const axspmv_csr *a, hipStream_t s) {
     o.flags &= ~AXSPMV_PLAN_TRANSPOSE;
     axspmv_plan *pt = nullptr;
     const int rc = plan_create(t, &o, s, true, &pt);
-    axspmv_csr_free(t);
-    if (rc) return rc;
+    if (rc) {
+        axspmv_csr_free(t);
+        return rc;
+    }
     p->T = pt;
+    if (pt->kind == Plan::RCT && (p->o.flags & AXSPMV_PLAN_DETERMINISTIC)) {
+        // x_bar = A^T y_bar without atomics: the AXT layout of A^T in column
+        // slices of at most 2 MB of y_bar (at most 8), slice partials summed
+        // in slice order.  Two slots per tile: on the column-uniform rows of
+        // a power-law matrix's transpose the least padding per slice.
+        axspmv_opts od{};
+        axspmv_opts_default(AXSPMV_FMT_AXT, &od);
+        od.thw = 2;
+        od.vtype = p->o.vtype;
+        od.flags = AXSPMV_PLAN_DETERMINISTIC;
+        uint32_t S = 1;
+        while (S < 8 && 8ull * t->ncols > (2ull << 20) * S) S *= 2;
+        od.slices = S;
+        axspmv_plan *pd = nullptr;
+        const int rd = plan_create(t, &od, s, false, &pd);
+        if (rd) {
+            axspmv_csr_free(t);
+            return rd;
+        }
+        p->Tdet = pd;
+    }
+    axspmv_csr_free(t);
     if (pt->kind == Plan::RCT) {
         // v_bar comes out in the transposed plan's entry order: the position
         // in A of each of its entries
@@ -981,7 +1125,7 @@ int plan_add_transpose(Plan *p, const axspmv_csr *a, hipStream_t s) {
 hipError_t plan_op(Plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s) {
     switch (op) {
         case AXSPMV_OP_SPMV: return plan_launch(p, o.x, o.y, s);
-        case AXSPMV_OP_ADJOINT: return plan_launch(p->T, o.ybar, o.xbar, s);
+        case AXSPMV_OP_ADJOINT: return plan_launch(p->Tdet ? p->Tdet : p->T, o.ybar, o.xbar, s);
         case AXSPMV_OP_VBAR: return launch_vbar(p, o.ybar, o.x, o.vbar, s);
         case AXSPMV_OP_FWD_ADJ: return launch_fwd_adj(p, o.x, o.b, o.y, o.r, o.xbar, o.vbar, true, s);
         case AXSPMV_OP_FWD_ADJ_SPLIT: return launch_fwd_adj(p, o.x, o.b, o.y, o.r, o.xbar, o.vbar, false, s);

@@ -11,7 +11,7 @@
 #include "common.hpp"
 
 struct axspmv_plan {
-    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC, RCT, AXT_UNC_H1_AX, CSR_VECTOR };
+    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC, RCT, AXT_UNC_H1_AX, CSR_VECTOR, AXT_UNC_H1_SLICED };
     Kind kind = NONE;
     int device = 0;
     axspmv_opts o{};
@@ -58,10 +58,18 @@ struct axspmv_plan {
     // of every value slot (0xffffffff for padding)
     double *d_ax = nullptr;
     uint32_t *d_axmap = nullptr;
-    // AXSPMV_TIME_KERNEL: event the path records after its first kernel (RCT)
+    // AXSPMV_TIME_KERNEL: event the path records after its first kernel (the
+    // y clear of RCT and AXT plans in column slices)
     hipEvent_t mid = nullptr;
     // A^T in the same format (AXSPMV_PLAN_TRANSPOSE)
     axspmv_plan *T = nullptr;
+    // RCT with AXSPMV_PLAN_DETERMINISTIC: A^T also as a deterministic AXT
+    // plan in column slices, for axspmv_adjoint
+    axspmv_plan *Tdet = nullptr;
+    // AXT in column slices: S, and with AXSPMV_PLAN_DETERMINISTIC the S x
+    // nrows slice partials
+    uint32_t slices = 1;
+    double *d_part = nullptr;
 
     std::vector<void *> bufs;
     void *dalloc(size_t bytes);

@@ -258,7 +258,15 @@ typedef struct axspmv_opts {
     uint32_t slices;    /* RCT: column slices 1, 2, 4 or 8 (0 = one slice if
                            the row blocks read column bands of <= 2 MB of x
                            -- banded matrices -- else the fewest slices with
-                           an x slice <= 2 MB) */
+                           an x slice <= 2 MB).
+                           AXT UNC th = 1: 2, 4 or 8 splits the matrix into
+                           that many column slices, each stored in its own
+                           AXT layout (the builders' arrays for that part of
+                           the matrix), each XCD gathering from one slice of
+                           x; y is cleared, then the slices' row sums are
+                           added with FP64 atomics (rounding order varies
+                           between runs).  SpMV and the adjoint only; 0 or 1
+                           = one layout; other AXT layouts reject S > 1. */
 } axspmv_opts;
 
 #define AXSPMV_PLAN_TRANSPOSE 1u /* also build A^T (needed by the adjoint) */
@@ -273,6 +281,20 @@ typedef struct axspmv_opts {
                                     there.  Off by default: on gfx950 the
                                     vector L1 already serves these gathers and
                                     the staging step measured slower. */
+#define AXSPMV_PLAN_DETERMINISTIC 16u /* results independent of the order in
+                                    which workgroups run: AXT plans in column
+                                    slices store each slice's row sums in a
+                                    partial vector (slices x nrows doubles,
+                                    held by the plan) and a second kernel adds
+                                    them in slice order, instead of FP64
+                                    atomics into y; RCT plans with
+                                    AXSPMV_PLAN_TRANSPOSE also build A^T as
+                                    such an AXT plan (two slots per tile,
+                                    slices of <= 2 MB of y_bar) and
+                                    axspmv_adjoint runs on it.  The RCT
+                                    forward SpMV, its value gradient and its
+                                    fused forward + adjoint keep their LDS and
+                                    slice atomics. */
 #define AXSPMV_PLAN_AX_LAYOUT 4u /* AXT UNC th == 1 or AXC, FP64 values: keep
                                     the reference's interleaved ax array on the
                                     device (tiles of thw values then thw
@@ -317,8 +339,17 @@ int axspmv_spmv(axspmv_plan *p, const double *d_x, double *d_y, void *stream);
 
 /* ------------------------------------------------- adjoint (gradients) */
 /* x_bar = A^T y_bar, run by the same kernel on a precomputed A^T in the same
- * format (plan created with AXSPMV_PLAN_TRANSPOSE): no atomics.
- * No reference counterpart (the reference computes forward products only). */
+ * format (plan created with AXSPMV_PLAN_TRANSPOSE).  CSR, K1, AXC, AXT COM
+ * and AXT UNC plans without column slices reduce every row inside one wave
+ * or a fixed fix-up order: no atomics, bit-identical results from call to
+ * call.  RCT plans run the RCT plan of A^T, whose tiles add into LDS with
+ * atomics and, with several column slices, into x_bar with FP64 atomics, and
+ * AXT plans in column slices add slice sums with FP64 atomics: the rounding
+ * of x_bar then varies between calls (within the FP64 tolerance).  With
+ * AXSPMV_PLAN_DETERMINISTIC both are atomic-free: RCT plans run a
+ * deterministic AXT plan of A^T in column slices, whose slice partials are
+ * summed in slice order.  No reference counterpart (the reference computes
+ * forward products only). */
 int axspmv_adjoint(axspmv_plan *p, const double *d_ybar, double *d_xbar, void *stream);
 /* Per-nonzero value gradient v_bar[k] = y_bar[row(k)] * x[col(k)].  Plans:
  * CSR naive and AXT UNC th == 1 write it in the CSR order of the matrix the
@@ -371,9 +402,10 @@ typedef struct axspmv_timing {
 #define AXSPMV_TIME_KERNEL 3u /* as REGION (SpMV only, iters <= 4096), with an
                                  event before every SpMV and one where its main
                                  kernel starts: main_ms is that kernel's mean
-                                 time.  Only RCT with slices has a kernel
-                                 before its main one (the clear of y); other
-                                 paths report main_ms = the SpMV's time.
+                                 time.  Only RCT and AXT plans with column
+                                 slices have a kernel before their main one
+                                 (the clear of y); other paths report
+                                 main_ms = the SpMV's time.
                                  median/min/max: of the intervals between the
                                  starts of consecutive SpMVs in the region. */
 /* Time `iters` SpMVs after `warmup` untimed ones.  AXSPMV_TIME_EVENTS is the

@@ -27,6 +27,8 @@ ADJ_KERNELS = [
     # taller than the transposed plan's tiles may be (x_bar sums and x values
     # of the fused gradient: 16 B of LDS per row): that plan is cut to fit
     ("rct_r16384_s4", ax.FMT_RCT, dict(tile_rows=16384, slices=4)),
+    # A^T also as a deterministic AXT plan in column slices for the adjoint
+    ("rct_r512_s4_det", ax.FMT_RCT, dict(tile_rows=512, slices=4, flags=ax.PLAN_DETERMINISTIC)),
 ]
 
 
@@ -282,3 +284,52 @@ def test_rct_vbar_at_8_byte_offset(fused):
     assert np.isnan(full[0]) and np.isnan(full[-1])
     assert np.array_equal(p.vbar_csr(full[1:-1]), orc.vbar(m, r, x))
     assert rel_err(orc.ncsr(m, x) - b, r) <= TOL
+
+
+@pytest.mark.parametrize("gen", [lambda: ax.powerlaw(1 << 18, 2.0, 2048, 20.0, seed=1),
+                                 lambda: ax.stencil27(48, permute=True, seed=2),
+                                 lambda: MATS["long_rows"]], ids=["powerlaw256k", "stencil48p", "long_rows"])
+def test_rct_adjoint_deterministic(gen):
+    """RCT plans with PLAN_DETERMINISTIC: x_bar = A^T y_bar runs on an AXT plan
+    of A^T in column slices whose slice partials are summed in slice order --
+    bit-identical over repeated calls (into dirty buffers), and within 1e-12
+    of the oracle; without the flag the same plan adds with atomics."""
+    m = gen()
+    w = ax.uniform(31, m.nrows)
+    ref = orc.ncsr_t(m, w, m.ncols)
+    p = ax.Plan(m, ax.FMT_RCT, flags=ax.PLAN_TRANSPOSE | ax.PLAN_DETERMINISTIC)
+    dw = ax.DeviceBuffer.from_array(w)
+    outs = []
+    for i in range(4):
+        dxb = ax.DeviceBuffer.from_array(np.full(m.ncols, np.nan if i % 2 else 7.0))
+        p.adjoint(dw, dxb)
+        outs.append(dxb.download(np.float64, m.ncols))
+    assert rel_err(ref, outs[0]) <= TOL
+    for o in outs[1:]:
+        assert o.tobytes() == outs[0].tobytes()
+    # the forward SpMV, v_bar and the fused pass of the same plan still work
+    x = ax.uniform(32, m.ncols)
+    assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
+
+
+@pytest.mark.parametrize("slices", [2, 4, 8])
+def test_axt_slices_deterministic(slices):
+    """AXT in column slices with PLAN_DETERMINISTIC: y bit-identical over
+    repeated SpMVs (slice partials summed in slice order), and the adjoint
+    through its sliced plan of A^T too."""
+    m = ax.powerlaw(1 << 17, 2.0, 2048, 20.0, seed=3)
+    p = ax.Plan(m, ax.FMT_AXT, thw=2, slices=slices, flags=ax.PLAN_DETERMINISTIC | ax.PLAN_TRANSPOSE)
+    assert p.info()["name"].endswith("_det")
+    x = ax.uniform(33, m.ncols)
+    w = ax.uniform(34, m.nrows)
+    ys = [ax.spmv_host(p, x) for _ in range(3)]
+    assert rel_err(orc.ncsr(m, x), ys[0]) <= TOL
+    assert all(y.tobytes() == ys[0].tobytes() for y in ys)
+    dw = ax.DeviceBuffer.from_array(w)
+    xs = []
+    for _ in range(3):
+        dxb = ax.DeviceBuffer.from_array(np.full(m.ncols, np.nan))
+        p.adjoint(dw, dxb)
+        xs.append(dxb.download(np.float64, m.ncols))
+    assert rel_err(orc.ncsr_t(m, w, m.ncols), xs[0]) <= TOL
+    assert all(v.tobytes() == xs[0].tobytes() for v in xs)

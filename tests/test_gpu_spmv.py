@@ -35,6 +35,12 @@ KERNELS = [
     ("unc_h12_thw32", ax.FMT_AXT, dict(thw=32, th=12)),
     ("unc_h5_thw64", ax.FMT_AXT, dict(thw=64, th=5)),
     ("unc_h1_thw4_small_budget", ax.FMT_AXT, dict(thw=4, th=1, budget=64)),
+    ("unc_h1_thw4_s4", ax.FMT_AXT, dict(thw=4, slices=4)),
+    ("unc_h1_thw2_s2", ax.FMT_AXT, dict(thw=2, slices=2)),
+    ("unc_h1_thw1_s8", ax.FMT_AXT, dict(thw=1, slices=8)),
+    ("unc_h1_thw8_s4_b256", ax.FMT_AXT, dict(thw=8, slices=4, budget=256)),
+    ("unc_h1_thw2_s4_det", ax.FMT_AXT, dict(thw=2, slices=4, flags=ax.PLAN_DETERMINISTIC)),
+    ("unc_h1_thw4_s8_det", ax.FMT_AXT, dict(thw=4, slices=8, flags=ax.PLAN_DETERMINISTIC)),
     ("k1_c64", ax.FMT_K1, dict(chunk=64)),
     ("k1_c64_natural", ax.FMT_K1, dict(chunk=64, reorder=0)),
     ("k1_c32", ax.FMT_K1, dict(chunk=32)),
@@ -112,13 +118,15 @@ def test_spmv_f64(mname, kname, fmt, kw):
     dy = ax.DeviceBuffer.from_array(np.full(max(m.nrows, 1), np.nan))
     plan.spmv(dx, dy)
     y2 = dy.download(np.float64, m.nrows)
-    if fmt == ax.FMT_RCT:  # LDS atomics: the order of additions varies between runs
+    atomic = fmt == ax.FMT_RCT or (kw.get("slices", 0) > 1 and not kw.get("flags", 0) & ax.PLAN_DETERMINISTIC)
+    if atomic:  # the order of additions varies between runs
         assert rel_err(ref, y2) <= TOL
     else:
         assert np.array_equal(y, y2)
 
 
 F32_KERNELS = [k for k in KERNELS if k[0] in ("csr_naive", "csr_vector_auto", "unc_h1_thw4", "unc_h4_thw64", "k1_c64", "axc_h8",
+                                                "unc_h1_thw4_s4",
                                                 "com_h1_thw64_bs256", "com_h8_thw64", "rct_r1024_s4",
                                                 "rct_r2048_s4_split")]
 
@@ -140,6 +148,7 @@ def test_powerlaw_config2_full_size():
     x = ax.uniform(1, m.nrows)
     ref = orc.ncsr(m, x, nthreads=int(os.environ.get("OMP_NUM_THREADS", "8")))
     for fmt, kw in ((ax.FMT_AXT, dict(thw=4, th=1)), (ax.FMT_AXT, dict(thw=64, th=4)),
+                    (ax.FMT_AXT, dict(thw=2, slices=4)), (ax.FMT_AXT, dict(thw=4, slices=4)),
                     (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=256)),
                     (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)), (ax.FMT_K1, dict(chunk=64)),
                     (ax.FMT_AXC, dict(hbs=4)), (ax.FMT_RCT, dict())):
@@ -272,7 +281,8 @@ EDGE_PLANS = [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_AXT, dict(thw=64, th=4)), (ax.F
               (ax.FMT_K1, dict(chunk=64)), (ax.FMT_AXC, dict(hbs=4)), (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1)),
               (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)), (ax.FMT_RCT, dict()),
               (ax.FMT_RCT, dict(slices=4, tile_rows=64)), (ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_AX_LAYOUT)),
-              (ax.FMT_AXT, dict(thw=4, vtype=ax.F32)), (ax.FMT_RCT, dict(vtype=ax.F32, slices=2))]
+              (ax.FMT_AXT, dict(thw=4, vtype=ax.F32)), (ax.FMT_RCT, dict(vtype=ax.F32, slices=2)),
+              (ax.FMT_AXT, dict(thw=2, slices=4)), (ax.FMT_AXT, dict(thw=4, slices=8, vtype=ax.F32))]
 
 
 @pytest.mark.parametrize("fmt,kw", EDGE_PLANS, ids=[f"{f}-{k}" for f, k in EDGE_PLANS])
@@ -341,3 +351,34 @@ def test_nonfinite_x_stays_in_its_rows(mname, kname, fmt, kw):
     np.testing.assert_array_equal(np.isfinite(y), fin)
     np.testing.assert_array_equal(y[~fin], ref[~fin])  # the same inf / nan
     assert rel_err(ref[fin], y[fin]) <= TOL
+
+
+@pytest.mark.parametrize("slices", [2, 4, 8])
+def test_axt_slices_layout_and_dispatch(slices):
+    """AXT in column slices: each slice is the builders' AXT layout of its
+    columns (the plan's slot count is the sum of the slices' layouts), the
+    clear + block kernel pair is one SpMV, and the adjoint through the
+    sliced plan of A^T matches the oracle."""
+    m = MATS["powerlaw64k"]
+    p = ax.Plan(m, ax.FMT_AXT, thw=4, slices=slices, flags=ax.PLAN_TRANSPOSE)
+    info = p.info()
+    assert info["name"] == f"axt_unc_h1_thw4_s{slices}" and info["launches"] == 2
+    C = -(-m.ncols // slices)
+    slots = 0
+    for k in range(slices):
+        keep = (m.col // C) == k
+        rows = np.repeat(np.arange(m.nrows), np.diff(m.row))[keep]
+        row = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=m.nrows))]).astype(np.uint32)
+        sub = ax.CsrMatrix(row, m.col[keep], m.val[keep], ncols=m.ncols)
+        slots += ax.build_axt(sub, None, 256, 4, 1, ax.AXT_UNC, ax.BUILD_PLANES)["lenAX"] // 2
+    assert info["slots"] == slots
+    x = ax.uniform(20, m.ncols)
+    assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
+    w = ax.uniform(21, m.nrows)
+    dw, dxb = ax.DeviceBuffer.from_array(w), ax.DeviceBuffer(8 * m.ncols)
+    p.adjoint(dw, dxb)
+    assert rel_err(orc.ncsr_t(m, w, m.ncols), dxb.download(np.float64, m.ncols)) <= TOL
+    with pytest.raises(ax.AxspmvError):
+        p.vbar(dw, ax.DeviceBuffer.from_array(x), ax.DeviceBuffer(8 * m.nnz))
+    with pytest.raises(ax.AxspmvError, match="column slices"):
+        ax.Plan(m, ax.FMT_AXT, thw=64, th=4, slices=slices)
