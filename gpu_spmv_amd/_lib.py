@@ -149,6 +149,7 @@ SIGNATURES = {
     "axspmv_dev_build_axt": (C.c_int, [P(Dcsr), vp, u32, u32, u32, u32, vp, P(P(Dfmt))]),
     "axspmv_dev_build_axc": (C.c_int, [P(Dcsr), vp, u32, vp, P(P(Dfmt))]),
     "axspmv_dfmt_free": (None, [P(Dfmt)]),
+    "axspmv_dev_build_axt_noalloc": (C.c_int, [P(Dcsr), vp, u32, u32, u32, u32, vp, P(Dfmt)]),
     "axspmv_opts_default": (C.c_int, [u32, P(Opts)]),
     "axspmv_plan_create": (C.c_int, [P(Csr), P(Opts), vp, P(vp)]),
     "axspmv_plan_destroy": (C.c_int, [vp]),

@@ -106,8 +106,6 @@ namespace axspmv {
 namespace launch {
 
 hipError_t build_check(const DevCsr &a, uint32_t *flag, hipStream_t s) {
-    const hipError_t e = hipMemsetAsync(flag, 0, 4, s);
-    if (e != hipSuccess) return e;
     const uint64_t threads = std::max<uint64_t>(1, std::max<uint64_t>(a.nrows, a.nnz));
     check_kernel<<<unsigned((threads + 255) / 256), 256, 0, s>>>(a, flag);
     return hipGetLastError();

@@ -36,6 +36,11 @@ __global__ void __launch_bounds__(256) clear_kernel(double *__restrict__ y, size
     }
 }
 
+__global__ void __launch_bounds__(256) fill_u32_kernel(uint32_t *__restrict__ dst, size_t n, uint32_t v) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
+}
+
 __global__ void __launch_bounds__(256) fill_rows_kernel(const uint32_t *__restrict__ rows, uint32_t n, double *__restrict__ y) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[rows[i]] = 0.0;
@@ -148,6 +153,11 @@ hipError_t clear_f64(double *y, size_t n, hipStream_t s) {
         clear_kernel<true><<<unsigned((n / 2 + 256) / 256), 256, 0, s>>>(y, n);
     else
         clear_kernel<false><<<unsigned((n + 255) / 256), 256, 0, s>>>(y, n);
+    return hipGetLastError();
+}
+
+hipError_t fill_u32(uint32_t *dst, size_t n, uint32_t v, hipStream_t s) {
+    if (n) fill_u32_kernel<<<grid_for(n, 256), 256, 0, s>>>(dst, n, v);
     return hipGetLastError();
 }
 

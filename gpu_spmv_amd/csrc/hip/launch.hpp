@@ -202,7 +202,8 @@ struct BuildOut {
     uint32_t *con;     // AXT con, AXC brp
 };
 // cnt[r] = ceil(rl_r / div) for r < nrows
-// *flag = 0, then the offsets / column checks of check_kernel (bits 1, 2, 4, 8)
+// the offsets / column checks of check_kernel OR-ed into *flag (bits 1, 2,
+// 4, 8; the caller clears it first)
 hipError_t build_check(const DevCsr &a, uint32_t *flag, hipStream_t s);
 hipError_t build_count(const DevCsr &a, uint32_t div, uint64_t *cnt, hipStream_t s);
 // rocPRIM exclusive scan; tmp == nullptr queries tmp_bytes
@@ -214,6 +215,7 @@ hipError_t build_fill(const DevCsr &a, int kind, const uint64_t *start, const Bu
 
 hipError_t copy_f64(const double *src, double *dst, size_t n, hipStream_t s);
 hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s);
+hipError_t fill_u32(uint32_t *dst, size_t n, uint32_t v, hipStream_t s);
 // y[0 .. n) = 0 in one pass (16-byte stores where y is 16-byte aligned)
 hipError_t clear_f64(double *y, size_t n, hipStream_t s);
 // reads p[0 .. n) (n even, p 16-byte aligned): evicts caches without dirtying them

@@ -482,6 +482,27 @@ int axspmv_dev_build_axc(const axspmv_dcsr *a, const double *d_x, uint32_t hbs, 
                          axspmv_dfmt **out);
 void axspmv_dfmt_free(axspmv_dfmt *f);
 
+/* axspmv_dev_build_axt without any device allocation.  No reference
+ * counterpart: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  The caller passes the
+ * output arrays in *f (ax, sec, con, plane; device memory it owns) with the
+ * layout's lengths in f->lenAX, lenSEC, lenCON (= nrows) and lenMAPX (=
+ * lenAX / 2) -- as an earlier axspmv_dev_build_axt of a matrix with the same
+ * row lengths reported them (or axspmv_build_axt's lenAX / lenSEC); other
+ * lengths are AXSPMV_E_INVALID, before anything is written.  The call fills
+ * the arrays (bit for bit those of axspmv_dev_build_axt) and the other fields
+ * of *f.  Its scratch (check flag, row counts, their scan, the scan's
+ * temporary storage) lives in a static device array of
+ * AXSPMV_DEV_BUILD_NOALLOC_SCRATCH bytes per GPU, held until the process
+ * ends: UNC layouts need about 16 * (nrows + 1) bytes of it plus the scan's
+ * storage (some 3 million rows at most), COM layouts only the flag; a matrix
+ * that needs more is AXSPMV_E_UNSUPPORTED.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (threads or streams) race.  The call
+ * synchronizes `stream` (it reads the layout's length back). */
+#define AXSPMV_DEV_BUILD_NOALLOC_SCRATCH (48u << 20)
+int axspmv_dev_build_axt_noalloc(const axspmv_dcsr *a, const double *d_x, uint32_t bs, uint32_t thw, uint32_t th,
+                                 uint32_t mode, void *stream, axspmv_dfmt *f);
+
 /* ------------------------------------------------------------ batches */
 /* Many matrices of different sizes, SpMV in one launch.  x and y are the
  * concatenations of the matrices' x (ncols each) and y (nrows each) in batch
