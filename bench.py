@@ -25,11 +25,11 @@ its structure, and the 256-matrix banded batch in one launch.
   python bench.py [--steps K] [--warmup W] [--formats flagship|all]
                   [--suite all|headline]
   python bench.py --calibrate      copy bandwidth and the x-footprint sweep
-  python bench.py --pmc-probe [stencil]
+  python bench.py --pmc-probe [stencil|spmm]
                                    a 256 MB copy then 5 SpMVs of each flagship
                                    kernel on the power-law matrix (or on the
-                                   permuted 27-point stencil), for rocprofv3
-                                   --pmc runs
+                                   permuted 27-point stencil), or 3 SpMMs at
+                                   k = 8, 32, 64, for rocprofv3 --pmc runs
 """
 from __future__ import annotations
 
@@ -52,8 +52,12 @@ from gpu_spmv_amd.core import TIME_COLD, TIME_GRAPH, TIME_KERNEL, TIME_REGION  #
 PEAK_HBM_GBS = 8000.0
 FLAGSHIP = ("rct_auto", ax.FMT_RCT, dict())
 FLAGSHIP_AXT = ("axt_unc_h1_thw4", ax.FMT_AXT, dict(thw=4, th=1))
+# the reference's AXT UNC th = 1 layout, one per column slice of 2 MB of x
+FLAGSHIP_AXT_SLICED = ("axt_unc_h1_thw2_s4", ax.FMT_AXT, dict(thw=2, slices=4))
 OTHERS = [
     FLAGSHIP_AXT,
+    FLAGSHIP_AXT_SLICED,
+    ("axt_unc_h1_thw2_s4_det", ax.FMT_AXT, dict(thw=2, slices=4, flags=ax.PLAN_DETERMINISTIC)),
     ("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
     ("csr_vector", ax.FMT_CSR, dict(kernel=ax.KER_STREAM)),
     ("axt_unc_h1_thw8", ax.FMT_AXT, dict(thw=8, th=1)),
@@ -439,21 +443,34 @@ def calibrate():
 
 
 def pmc_probe(matrix: str = "powerlaw"):
-    """A 256 MB copy (known bytes: the counter scale) then 5 SpMVs of each of
-    the two flagship kernels, on the 2^20-row power-law matrix or on the
-    permuted 27-point stencil on 128^3."""
+    """A 256 MB copy (known bytes: the counter scale), then, on the 2^20-row
+    power-law matrix (or the permuted 27-point stencil on 128^3), 5 SpMVs of
+    each flagship plan (AXT UNC th = 1 in one layout and in 4 column slices,
+    and RCT); with "spmm", 3 SpMMs at k = 8, 32 and 64 of the plans the
+    autotuner picks on the power-law matrix."""
     n = 32 << 20
     src, dst = ax.DeviceBuffer(8 * n), ax.DeviceBuffer(8 * n)
     L.call("axspmv_copy_f64", src.ptr, dst.ptr, n, None)
     L.call("axspmv_device_sync")
+    del src, dst
     if matrix == "stencil":
-        m = ax.stencil27(128, permute=True, seed=2)
-        x = ax.uniform(2, m.ncols)
+        m, x = W.config3()
     else:
-        m = ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
-        x = ax.uniform(1, m.ncols)
+        m, x = W.config2()
     dx, dy = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer(8 * m.nrows)
-    for _, fmt, kw in (FLAGSHIP_AXT, FLAGSHIP):
+    if matrix == "spmm":
+        for k, kw in ((8, dict(tile_rows=1024, slices=4, ktile=8)), (32, dict(tile_rows=256, slices=1, ktile=16)),
+                      (64, dict(tile_rows=256, slices=1, ktile=16))):
+            p = ax.Plan(m, ax.FMT_RCT, **kw)
+            dX = ax.DeviceBuffer.from_array(W.config4a_rhs(m.ncols, k))
+            dY = ax.DeviceBuffer(8 * m.nrows * k)
+            for _ in range(3):
+                p.spmm(k, dX, dY)
+            L.call("axspmv_device_sync")
+            log(f"pmc-probe: spmm k={k} {p.info()['name']} ktile={kw['ktile']}")
+            del p, dX, dY
+        return
+    for _, fmt, kw in (FLAGSHIP_AXT, FLAGSHIP_AXT_SLICED, FLAGSHIP):
         p = ax.Plan(m, fmt, **kw)
         for _ in range(5):
             p.spmv(dx, dy)
@@ -620,7 +637,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
-    ap.add_argument("--pmc-probe", nargs="?", const="powerlaw", choices=["powerlaw", "stencil"])
+    ap.add_argument("--pmc-probe", nargs="?", const="powerlaw", choices=["powerlaw", "stencil", "spmm"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: SpMV here runs on a single MI355X; "
