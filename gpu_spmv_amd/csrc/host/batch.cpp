@@ -164,6 +164,8 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
         if (o.format != AXSPMV_FMT_AXT || o.mode != AXSPMV_AXT_UNC)
             return fail(AXSPMV_E_UNSUPPORTED, "batch_create: matrix %u: batches use the AXT UNC layout", i);
         if (o.vtype != vtype) return fail(AXSPMV_E_INVALID, "batch_create: one value type per batch");
+        if (o.slices > 1)
+            return fail(AXSPMV_E_UNSUPPORTED, "batch_create: matrix %u: AXT column slices are for single plans", i);
         if (o.th == 0) o.th = 1;
         if (o.thw == 0) o.thw = o.th == 1 ? 4 : 64;
         if (!is_pow2(o.thw) || o.thw > 64) return fail(AXSPMV_E_UNSUPPORTED, "batch_create: thw must be a power of two <= 64");
@@ -310,8 +312,8 @@ extern "C" int axspmv_batch_time(axspmv_batch *b, const double *d_x, double *d_y
 namespace {
 
 // Built-in candidates.  Batches take AXT UNC per shape group and RCT (the
-// block-diagonal plan) for the whole batch; single plans also try K1 and
-// RCT.  SpMM: AXT
+// block-diagonal plan) for the whole batch; single plans also try AXT UNC
+// th = 1 in 2, 4 and 8 column slices, K1 and RCT.  SpMM: AXT
 // k-tiles, and RCT tiles with 64 KB of rows x k-tile sums (one column slice
 // and the default slicing) or 32 KB (one slice).
 std::vector<axspmv_opts> default_candidates(uint32_t op, bool for_batch = false) {
@@ -362,6 +364,14 @@ std::vector<axspmv_opts> default_candidates(uint32_t op, bool for_batch = false)
         axspmv_opts_default(AXSPMV_FMT_RCT, &o);
         c.push_back(o);
         return c;
+    }
+    // the AXT UNC th = 1 layout per column slice (two slots per tile)
+    o.budget = 64;
+    o.th = 1;
+    o.thw = 2;
+    for (uint32_t sl : {2u, 4u, 8u}) {
+        o.slices = sl;
+        c.push_back(o);
     }
     axspmv_opts_default(AXSPMV_FMT_K1, &o);
     c.push_back(o);

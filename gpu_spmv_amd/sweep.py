@@ -49,6 +49,8 @@ CANDIDATES = [
     ("axt_unc_h1_thw16", L.FMT_AXT, dict(thw=16, th=1), False),
     ("axt_unc_h4_thw64", L.FMT_AXT, dict(thw=64, th=4), False),
     ("axt_unc_h8_thw64", L.FMT_AXT, dict(thw=64, th=8), False),
+    ("axt_unc_h1_thw2_s4", L.FMT_AXT, dict(thw=2, slices=4), False),
+    ("axt_unc_h1_thw2_s8", L.FMT_AXT, dict(thw=2, slices=8), False),
     ("axt_com_h1_thw64_bs256", L.FMT_AXT, dict(mode=L.AXT_COM, thw=64, th=1, bs=256), False),
     ("axt_com_h4_thw64", L.FMT_AXT, dict(mode=L.AXT_COM, thw=64, th=4), False),
     ("rct_auto", L.FMT_RCT, dict(), True),
