@@ -307,8 +307,12 @@ int dev_build_axt_into(const axspmv_dcsr *a, const double *d_x, uint32_t bs, uin
                     "(axspmv_dev_build_axt has no such limit)",
                     a->nrows, need, cap);
     w.flag = reinterpret_cast<uint32_t *>(scratch);
-    set_axt_header(f, name, bs, thw, th, mode, log);
-    return run_build(a, d_x, kind, launch::BuildGeo{thw, th, bs, log}, false, s, f, w, false);
+    // *f changes only when the build succeeds
+    axspmv_dfmt g = *f;
+    set_axt_header(&g, name, bs, thw, th, mode, log);
+    const int rc = run_build(a, d_x, kind, launch::BuildGeo{thw, th, bs, log}, false, s, &g, w, false);
+    if (rc == AXSPMV_OK) *f = g;
+    return rc;
 }
 
 }  // namespace axspmv

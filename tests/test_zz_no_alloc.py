@@ -88,6 +88,17 @@ def test_noalloc_rejects_wrong_lengths():
     with pytest.raises(L.AxspmvError, match="the layout needs") as e:
         noalloc(m, None, 256, 4, 1, ax.AXT_UNC, lengths=lens)
     assert e.value.status == L.E_INVALID
+    # the caller's descriptor is left as it was
+    f = L.Dfmt()
+    for k, v in lens.items():
+        setattr(f, k, v)
+    a = DevCsr(m)
+    bufs = [ax.DeviceBuffer(8 * (lens["lenAX"] + 1)) for _ in range(4)]
+    f.ax, f.sec, f.con, f.plane = (b.ptr for b in bufs)
+    before = bytes(f)
+    with pytest.raises(L.AxspmvError):
+        L.call("axspmv_dev_build_axt_noalloc", C.byref(a.s), None, 256, 4, 1, ax.AXT_UNC, None, C.byref(f))
+    assert bytes(f) == before
 
 
 def test_noalloc_rejects_scratch_beyond_the_static_array():
