@@ -34,15 +34,23 @@ namespace {
 using axspmv::launch::RctArgs;
 
 // Entry e of a tile: value val[e]; column and tile row either packed in one
-// word key[e] = (column - tile base) << rbits | row (PACKED; the base is the
-// tile's smallest column) or in col[] / lrow[].
+// word key[e] = offset << rbits | row (PACKED) or in col[] / lrow[].  The
+// packed column is (sbase[e >> 10] + offset) mod ncols: every run of 1024
+// entries has a base column (its first entry's), and its entries, sorted by
+// column tile by tile, lie a short way past it -- going round from the last
+// column to column 0 where a run passes from one row block to the next.
 template <class VT, bool PACKED>
 struct RctEntries {
     const VT *__restrict__ val;
     const uint32_t *__restrict__ key;
     const uint32_t *__restrict__ col;
     const uint16_t *__restrict__ lrow;
-    uint32_t rbits, cbase;
+    const uint32_t *__restrict__ sbase;
+    uint32_t rbits, ncols;
+    __device__ __forceinline__ uint32_t column(uint32_t base, uint32_t off) const {
+        const uint32_t room = ncols - base;  // columns from base to the end
+        return off >= room ? off - room : base + off;
+    }
     __device__ __forceinline__ void load(uint32_t e, uint32_t &c, uint32_t &r, double &v) const {
         v = double(val[e]);
         at(e, c, r);
@@ -61,10 +69,10 @@ struct RctEntries {
         }
         if constexpr (PACKED) {
             const uint2 k = reinterpret_cast<const uint2 *>(key)[q];
-            const uint32_t m = (1u << rbits) - 1;
-            c[0] = cbase + (k.x >> rbits);
+            const uint32_t m = (1u << rbits) - 1, b = sbase[q >> 9];  // entries 2q, 2q + 1: one segment
+            c[0] = column(b, k.x >> rbits);
             r[0] = k.x & m;
-            c[1] = cbase + (k.y >> rbits);
+            c[1] = column(b, k.y >> rbits);
             r[1] = k.y & m;
         } else {
             const uint2 cc = reinterpret_cast<const uint2 *>(col)[q];
@@ -79,7 +87,7 @@ struct RctEntries {
     __device__ __forceinline__ void at(uint32_t e, uint32_t &c, uint32_t &r) const {
         if constexpr (PACKED) {
             const uint32_t k = key[e];
-            c = cbase + (k >> rbits);
+            c = column(sbase[e >> 10], k >> rbits);
             r = k & ((1u << rbits) - 1);
         } else {
             c = col[e];
@@ -122,8 +130,7 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     __syncthreads();
     const uint32_t t = rb * a.S + s;  // tile index in storage order
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.rbits,
-                                     PACKED ? a.cbase[t] : 0u};
+    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.sbase, a.rbits, a.ncols};
     // Each lane takes entries in aligned pairs (2q, 2q + 1): a value load
     // moves 16 bytes and a key load 8, and the value gradient (VB) goes out
     // as one 16-byte store per pair.  Pairs pb .. pl cover the tile and each
@@ -257,7 +264,7 @@ __global__ void __launch_bounds__(1024) rct_vbar_kernel(RctArgs a, const double 
     __syncthreads();
     const uint32_t t = tile.rb * a.S + tile.s;
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const RctEntries<double, PACKED> ent{nullptr, a.key, a.col, a.lrow, a.rbits, PACKED ? a.cbase[t] : 0u};
+    const RctEntries<double, PACKED> ent{nullptr, a.key, a.col, a.lrow, a.sbase, a.rbits, a.ncols};
     constexpr int U = 4;
     uint32_t e = beg + tid;
     for (; e + (U - 1) * nt < end; e += U * nt) {
@@ -294,8 +301,7 @@ __global__ void __launch_bounds__(1024) rct_spmm_kernel(RctArgs a, uint32_t k, u
     __syncthreads();
     const uint32_t t = tile.rb * a.S + tile.s;
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.rbits,
-                                     PACKED ? a.cbase[t] : 0u};
+    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.sbase, a.rbits, a.ncols};
     const uint32_t j = tid % KT, nsub = nt / KT;  // nt is a multiple of 64
     const bool live = k0 + j < k;                 // the last k-tile may be partial
     const double *__restrict__ xj = X + k0 + j;

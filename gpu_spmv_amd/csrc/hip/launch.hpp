@@ -88,11 +88,12 @@ struct RctArgs {
     const void *val;
     const uint32_t *col;    // unpacked: column of each entry
     const uint16_t *lrow;   // unpacked: row within the tile
-    const uint32_t *key;    // packed: (column - tile base) << rbits | row
-    const uint32_t *cbase;  // packed: base column of every tile (its smallest)
+    const uint32_t *key;    // packed: offset << rbits | row, where entry e's
+                            // column is (sbase[e >> 10] + offset) mod ncols
+    const uint32_t *sbase;  // packed: base column of every 1024-entry segment
     const uint32_t *tile_ptr;
     const uint32_t *rb_start;  // nrb + 1 row-block starts
-    uint32_t nrows, R, S, nrb, vtype, block, rbits;
+    uint32_t nrows, R, S, nrb, vtype, block, rbits, ncols;
 };
 // S > 1: a fill kernel clears y, then the tile kernel adds into it; mid
 // (optional) is recorded on s between the two.  vbar != nullptr (a plan of

@@ -682,8 +682,11 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         return fail(AXSPMV_E_INVALID, "RCT: block %u is not a multiple of 64 up to 1024", p.o.block);
     hipDeviceProp_t prop;
     size_t lds_max = 65536;
-    if (hipGetDeviceProperties(&prop, p.device) == hipSuccess && prop.sharedMemPerBlock > 0)
-        lds_max = prop.sharedMemPerBlock;
+    uint64_t cus = 256;
+    if (hipGetDeviceProperties(&prop, p.device) == hipSuccess) {
+        if (prop.sharedMemPerBlock > 0) lds_max = prop.sharedMemPerBlock;
+        if (prop.multiProcessorCount > 0) cus = uint64_t(prop.multiProcessorCount);
+    }
     const uint32_t lds_rows = uint32_t(std::min<size_t>(65536, lds_max / p.lds_per_row));
     p.lds_max = lds_max;
     const uint32_t n = a->nrows;
@@ -693,8 +696,8 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         const uint64_t fill = ceil_div(ceil_div(uint64_t(n) * slices, 256), 64) * 64;
         return uint32_t(std::min<uint64_t>(std::min<uint32_t>(16384, lds_rows), std::max<uint64_t>(64, fill)));
     };
-    // Default slicing: x slices of at most 2 MB (one XCD's L2 share), unless
-    // the row blocks read narrow column bands anyway.
+    // Default slicing: one slice when x fits 2 MB (one XCD's L2 share) or
+    // the row blocks read narrow column bands anyway; else see below.
     constexpr uint64_t slice_cols = (2ull << 20) / 8;
     uint32_t S = p.o.slices;
     bool banded = false;
@@ -702,15 +705,36 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         S = 1;
         if (a->ncols > slice_cols) {
             banded = narrow_bands(a, p.o.tile_rows ? p.o.tile_rows : auto_rows(1), slice_cols);
-            if (!banded)
-                while (S < 8 && ceil_div(a->ncols, S) > slice_cols) S *= 2;
+            if (!banded) {
+                uint32_t s2 = 1;  // slices of at most 2 MB of x
+                while (s2 < 8 && ceil_div(a->ncols, s2) > slice_cols) s2 *= 2;
+                if (p.lds_per_row > 8) {
+                    // The plan of A^T behind a fused value gradient holds 16 B
+                    // per tile row; 8192-row tiles with half the slices measured
+                    // fastest for it (forward + adjoint on the 2^20-row
+                    // power-law matrix 241 us against 269 us at 10240 rows and 4
+                    // slices; 702 against 745 us on the permuted 27-point
+                    // stencil on 128^3), and for the adjoint alone.
+                    S = std::max(1u, s2 / 2);
+                } else {
+                    // Every slice adds each row's sum into y with an FP64
+                    // atomic, so take the fewest slices that still give every
+                    // CU a tile of full height (row blocks x slices >= CUs),
+                    // but no more than the 2 MB rule asks and no fewer than
+                    // slices of 8 MiB of x.  The permuted 27-point stencil on
+                    // 128^3 (128 full-height row blocks): 2 slices 229.6 us,
+                    // 4 slices 239.4 us, 8 slices 269.5 us; the 2^20-row
+                    // power-law matrix (64 row blocks) keeps 4 slices (2: 121 us,
+                    // half the CUs idle; 4: 74 us).
+                    uint32_t s8 = 1;
+                    while (s8 < 8 && ceil_div(a->ncols, s8) > (8ull << 20) / 8) s8 *= 2;
+                    const uint64_t nrb_full = ceil_div(n, p.o.tile_rows ? p.o.tile_rows : std::min<uint32_t>(16384, lds_rows));
+                    uint32_t sf = 1;
+                    while (sf < 8 && nrb_full * sf < cus) sf *= 2;
+                    S = std::max(s8, std::min(sf, s2));
+                }
+            }
         }
-        // The plan of A^T behind a fused value gradient holds 16 B per tile
-        // row; 8192-row tiles with half the slices measured fastest for it
-        // (forward + adjoint on the 2^20-row power-law matrix 241 us against
-        // 269 us at 10240 rows and 4 slices; 702 against 745 us on the
-        // permuted 27-point stencil on 128^3), and for the adjoint alone.
-        if (p.lds_per_row > 8 && S > 1) S /= 2;
     }
     if (S != 1 && S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_INVALID, "RCT: slices must be 1, 2, 4 or 8");
     // Narrow bands reuse x within a band whatever the tile height: 2048-row
@@ -771,21 +795,34 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     std::vector<uint32_t> tptr(ntile + 1);
     for (uint64_t t = 0; t <= ntile; ++t) tptr[t] = uint32_t(cnt[t]);
     const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
-    // Pack (column - tile base, row within the tile) into one word when both
-    // fit in 32 bits.  A tile's entries are sorted by column, so its base (its
-    // smallest column) is its first entry's and its span ends at its last.
+    // Pack (column offset, row within the tile) into one word when both fit
+    // in 32 bits.  The offset is taken from a base column per run of 1024
+    // entries, counting round from the last column to column 0: a run's
+    // columns rise tile by tile, so they cover a short stretch of columns,
+    // far shorter than a tile's.  The base is the run's smallest column, or,
+    // where the run passes from the last slice of a row block into the first
+    // of the next (its columns fall back towards 0), its first column.
     auto bits = [](uint64_t v) {  // bits to hold values 0 .. v-1
         uint32_t b = 1;
         while ((1ull << b) < v) ++b;
         return b;
     };
-    std::vector<uint32_t> cbase(ntile, 0);
+    constexpr uint32_t SEG_LOG = 10;  // entries per run: 1 << SEG_LOG
+    const uint64_t nseg = std::max<uint64_t>(1, ceil_div(nnz + (nnz & 1), uint64_t(1) << SEG_LOG));
+    std::vector<uint32_t> sbase(nseg, 0);
+    const uint64_t nc = std::max<uint64_t>(1, a->ncols);
     uint64_t span = 1;
-    for (uint64_t t = 0; t < ntile; ++t)
-        if (cnt[t + 1] > cnt[t]) {
-            cbase[t] = col[cnt[t]];
-            span = std::max<uint64_t>(span, uint64_t(col[cnt[t + 1] - 1]) - cbase[t] + 1);
-        }
+#pragma omp parallel for schedule(static) reduction(max : span)
+    for (uint64_t g = 0; g < nseg; ++g) {
+        const uint64_t q0 = g << SEG_LOG, q1 = std::min<uint64_t>(nnz, q0 + (uint64_t(1) << SEG_LOG));
+        if (q0 >= nnz) continue;
+        const auto mm = std::minmax_element(col.begin() + q0, col.begin() + q1);
+        const uint64_t lin = uint64_t(*mm.second) - *mm.first + 1;  // from the smallest column
+        uint64_t cyc = 1;                                            // round from the first column
+        for (uint64_t q = q0; q < q1; ++q) cyc = std::max<uint64_t>(cyc, (col[q] + nc - col[q0]) % nc + 1);
+        sbase[g] = lin <= cyc ? *mm.first : col[q0];
+        span = std::max(span, std::min(lin, cyc));
+    }
     const uint32_t rbits = bits(R), cbits = bits(span);
     const bool packed = rbits + cbits <= 32 && !(p.o.flags & AXSPMV_PLAN_RCT_SPLIT);
     // one spare entry when nnz is odd: a kernel may load entries in aligned pairs
@@ -810,14 +847,14 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     uint16_t *dl = nullptr;
     if (packed) {
         std::vector<uint32_t> key(nnz ? nnz : 1);
-#pragma omp parallel for schedule(dynamic, 1)
-        for (uint64_t t = 0; t < ntile; ++t)
-            for (uint64_t q = cnt[t]; q < cnt[t + 1]; ++q) key[q] = ((col[q] - cbase[t]) << rbits) | lrow[q];
+#pragma omp parallel for schedule(static)
+        for (uint64_t q = 0; q < nnz; ++q)
+            key[q] = uint32_t(((col[q] + nc - sbase[q >> SEG_LOG]) % nc) << rbits) | lrow[q];
         dk = static_cast<uint32_t *>(p.dalloc(4 * nnz_pad));
-        dcb = static_cast<uint32_t *>(p.dalloc(4 * ntile));
+        dcb = static_cast<uint32_t *>(p.dalloc(4 * nseg));
         if (!dk || !dcb) return AXSPMV_E_NOMEM;
         AX_TRY(p.upload(dk, key.data(), 4 * nnz, s));
-        AX_TRY(p.upload(dcb, cbase.data(), 4 * ntile, s));
+        AX_TRY(p.upload(dcb, sbase.data(), 4 * nseg, s));
     } else {
         dc = static_cast<uint32_t *>(p.dalloc(4 * nnz_pad));
         dl = static_cast<uint16_t *>(p.dalloc(2 * nnz_pad));
@@ -836,7 +873,7 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         block = per_tile < 2048 ? 128 : per_tile < 16384 ? 256 : 512;
     }
 
-    p.rct = launch::RctArgs{dv, dc, dl, dk, dcb, dt, drb, n, R, S, nrb, p.o.vtype, block, rbits};
+    p.rct = launch::RctArgs{dv, dc, dl, dk, dcb, dt, drb, n, R, S, nrb, p.o.vtype, block, rbits, a->ncols};
     p.slots = nnz;
     // matrix stream, and with S > 1 the clear of y and S atomic adds per row
     p.stream_bytes = (vs + (packed ? 4 : 6)) * nnz + 4 * tptr.size() + (S > 1 ? 8ull * (S + 1) * n : 0);

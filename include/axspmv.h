@@ -256,9 +256,14 @@ typedef struct axspmv_opts {
                            min(16384, LDS / 8)); blocks are cut to about
                            equal nonzeros */
     uint32_t slices;    /* RCT: column slices 1, 2, 4 or 8 (0 = one slice if
-                           the row blocks read column bands of <= 2 MB of x
-                           -- banded matrices -- else the fewest slices with
-                           an x slice <= 2 MB).
+                           x fits 2 MB or the row blocks read column bands
+                           of <= 2 MB of x -- banded matrices; else the
+                           fewest slices for which the row blocks of full
+                           height (16384 rows, or tile_rows) times the
+                           slices cover the CUs, at most as many as give x
+                           slices of <= 2 MB and at least as many as give
+                           x slices of <= 8 MiB; the plan of A^T: half the
+                           2 MB count).
                            AXT UNC th = 1: 2, 4 or 8 splits the matrix into
                            that many column slices, each stored in its own
                            AXT layout (the builders' arrays for that part of

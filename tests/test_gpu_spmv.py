@@ -163,11 +163,16 @@ def test_powerlaw_config2_full_size():
     (lambda: ax.stencil27(96, permute=True, seed=2), 4),      # 884,736 random columns
     (lambda: ax.powerlaw(1 << 19, 2.0, 2048, 20.0, seed=1), 2),
     (lambda: ax.random_sparse(200000, 1, 30, seed=4), 1),     # x of 1.6 MB fits one slice
-], ids=["laplace1024", "banded", "stencil96", "stencil96p", "powerlaw512k", "random200k"])
+    # 128 row blocks of 16,384 rows: 2 slices give the 256 CUs a tile each
+    # (x slices of 8 MiB; 2 MB slices would take 8)
+    (lambda: ax.stencil27(128, permute=True, seed=2), 2),
+], ids=["laplace1024", "banded", "stencil96", "stencil96p", "powerlaw512k", "random200k", "stencil128p"])
 def test_rct_auto_slices(gen, slices):
     """Default slicing: one slice when the row blocks read narrow column
-    bands or x fits 2 MB, else the fewest slices with an x slice <= 2 MB; the
-    product is the same either way."""
+    bands or x fits 2 MB; else the fewest slices for which the full-height
+    row blocks times the slices cover the CUs, but no more than needed for
+    x slices of 2 MB and no fewer than for x slices of 8 MiB.  The product
+    is the same either way."""
     m = gen()
     p = ax.Plan(m, ax.FMT_RCT)
     assert p.info()["name"].split("_")[2] == f"s{slices}", p.info()["name"]
@@ -197,21 +202,40 @@ def test_default_lanes_per_row(gen, lanes):
         assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
 
 
-@pytest.mark.parametrize("gen,split", [
-    (lambda: ax.banded(300000, 16, 0.5, seed=9), False),    # span of a tile ~ 16k + 33 columns: 14 + 15 bits
-    (lambda: ax.random_sparse(300000, 1, 12, seed=10), True),  # span ~ 300k columns: 14 + 19 bits
-], ids=["banded", "random"])
-def test_rct_key_packing(gen, split):
-    """The packed key holds the column relative to the tile's smallest column:
-    a banded matrix with 2^14-row tiles over 300,000 columns packs, a random
-    one keeps separate column and row arrays; both give the same product."""
+def wide_csr(nrows, ncols, seed):
+    """One entry per row at a random column of a wide matrix."""
+    rng = np.random.default_rng(seed)
+    return ax.CsrMatrix(np.arange(nrows + 1, dtype=np.uint32), rng.integers(0, ncols, nrows).astype(np.uint32),
+                        rng.uniform(-1, 1, nrows), ncols)
+
+
+@pytest.mark.parametrize("gen,slices,split", [
+    # runs of 1024 entries step back a few columns where they pass into the
+    # next row block: based on their smallest column, 14 + 6 bits
+    (lambda: ax.banded(300000, 16, 0.5, seed=9), 1, False),
+    # a run covers about 1024 x 300,000 / 106,000 columns: 14 + 12 bits
+    (lambda: ax.random_sparse(300000, 1, 12, seed=10), 1, False),
+    # runs passing from the last slice of a row block into the first slice
+    # of the next count round from their first column
+    (lambda: ax.random_sparse(300000, 1, 12, seed=10), 4, False),
+    # 16,384 entries per tile over 2^23 columns: a run covers ~2^19 columns,
+    # 14 + 19 bits do not fit one word
+    (lambda: wide_csr(32768, 1 << 23, seed=11), 1, True),
+], ids=["banded", "random", "random_s4", "wide"])
+def test_rct_key_packing(gen, slices, split):
+    """The packed key holds the column as an offset from a base column per
+    run of 1024 entries; where the offsets need more bits than the row
+    leaves, the plan keeps separate column and row arrays.  Both give the
+    product of the oracle."""
     m = gen()
-    p = ax.Plan(m, ax.FMT_RCT, tile_rows=16384, slices=1)
+    p = ax.Plan(m, ax.FMT_RCT, tile_rows=16384, slices=slices)
     assert p.info()["name"].endswith("_split") == split, p.info()["name"]
     x = ax.uniform(16, m.ncols)
     assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
+    if m.ncols > 1 << 20:
+        return  # X of 8 columns would take 512 MB
     X = ax.uniform(17, m.ncols * 8).reshape(m.ncols, 8)
-    q = ax.Plan(m, ax.FMT_RCT, tile_rows=2048, slices=1)
+    q = ax.Plan(m, ax.FMT_RCT, tile_rows=2048, slices=slices)
     dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * m.nrows * 8)
     q.spmm(8, dX, dY)
     assert rel_err(orc.spmm(m, X), dY.download(np.float64, m.nrows * 8).reshape(m.nrows, 8)) <= TOL
