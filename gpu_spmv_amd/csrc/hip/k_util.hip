@@ -22,17 +22,22 @@ __global__ void __launch_bounds__(256) fill_kernel(double *__restrict__ dst, siz
 
 // y[0 .. n) = 0, one lane per 16 bytes (VEC: y 16-byte aligned) or per
 // element: the clear before the atomic slice adds of the sliced plans (one
-// pass over y, no grid-stride loop).
+// pass over y, no grid-stride loop).  Non-temporal stores: the zeros go to
+// memory without being kept in the clearing XCD's L2, and the slice adds
+// that follow, from every XCD, run faster for it (RCT on the 2^20-row
+// power-law matrix: tile kernel 69.5 -> 64.4 us, clear + tile kernel
+// 71.8 -> 66.6 us; the clear itself takes the same 5 us).
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 template <bool VEC>
 __global__ void __launch_bounds__(256) clear_kernel(double *__restrict__ y, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if constexpr (VEC) {
         if (2 * i + 1 < n)
-            reinterpret_cast<double2 *>(y)[i] = double2{0.0, 0.0};
+            __builtin_nontemporal_store(f64x2{0.0, 0.0}, reinterpret_cast<f64x2 *>(y) + i);
         else if (2 * i < n)
-            y[2 * i] = 0.0;
+            __builtin_nontemporal_store(0.0, y + 2 * i);
     } else {
-        if (i < n) y[i] = 0.0;
+        if (i < n) __builtin_nontemporal_store(0.0, y + i);
     }
 }
 
