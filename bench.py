@@ -667,9 +667,10 @@ def main():
 
     # The timed region: K SpMVs back to back, device-synchronized on both
     # sides.  HIP events on the launch stream mark the start of every SpMV
-    # and, for a sliced RCT plan (a clear of y, then the tile kernel), the
-    # start of its tile kernel, so the whole SpMV's mean, its spread and the
-    # tile kernel's own mean come out of the same region (TIME_KERNEL).
+    # and, for a sliced RCT plan (the tile kernel, then the merge of its slice
+    # partials), the end of its tile kernel, so the whole SpMV's mean, its
+    # spread and the tile kernel's own mean come out of the same region
+    # (TIME_KERNEL).
     plan.time(dx, dy, iters=max(args.warmup, 1), warmup=0, mode=TIME_REGION)
     L.call("axspmv_device_sync")
     w0 = time.perf_counter()
@@ -718,9 +719,10 @@ def main():
         tj = json.load(open(tpath))
         entry = tj.get(info["name"], {})
         if "hbm_bytes_per_launch" in entry:
-            # every launch of one SpMV: the tile kernel and, with slices, the y clear
-            clear = entry.get("clear_kernel", {})
-            traffic = entry["hbm_bytes_per_launch"] + clear.get("fetch_bytes", 0) + clear.get("write_bytes", 0)
+            # every launch of one SpMV: the tile kernel and, with slices, the
+            # merge of the slice partials (or the y clear before atomic adds)
+            other = entry.get("merge_kernel", entry.get("clear_kernel", {}))
+            traffic = entry["hbm_bytes_per_launch"] + other.get("fetch_bytes", 0) + other.get("write_bytes", 0)
             traffic_kernel = entry.get("kernel")
             traffic_src = os.path.relpath(tpath, ROOT)
 
@@ -743,7 +745,8 @@ def main():
                    "alg_bytes_per_spmv": info["alg_bytes"], "parity_rel_err": flag["rel_err"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "scope": "whole SpMV (every launch of one SpMV: y clear + tile kernel for a sliced RCT plan)",
+                     "scope": "whole SpMV (every launch of one SpMV: tile kernel + merge of the slice partials "
+                              "for a sliced RCT plan)",
                      "spmv_us": round(path_s * 1e6, 2), "spmv_us_median": us(t.median_ms),
                      "spmv_us_min": us(t.min_ms), "spmv_us_max": us(t.max_ms),
                      "kernel": traffic_kernel or info["name"], "kernel_us": round(kern_s * 1e6, 2),

@@ -40,6 +40,7 @@ PLAN_RCT_SPLIT = 2
 PLAN_AX_LAYOUT = 4
 PLAN_STAGE_X = 8
 PLAN_DETERMINISTIC = 16
+PLAN_RCT_ATOMIC = 32
 
 
 class Csr(C.Structure):

@@ -15,11 +15,14 @@
 //    ever reads its own slice of x, which stays resident in its L2.
 //
 // With one slice a tile's sums are its rows' totals and are stored to y.
-// With S slices y is first cleared by a fill kernel, and every tile adds its
-// rows' sums to y with hardware FP64 atomics (global_atomic_add_f64, S adds
-// per row): no partial vectors in HBM and no merge pass.  The LDS adds
-// (ds_add_f64) and the slice adds make the order of the additions inside a
-// row nondeterministic; the result stays within the FP64 tolerance of the
+// With S slices either every tile stores its rows' sums in its slice's
+// partial vector (S x n doubles held by the plan) and a merge kernel adds the
+// S partials of each row in slice order, or y is cleared first and every
+// tile adds its sums into it with FP64 atomics (global_atomic_add_f64).
+// Every tile writes its rows when its loop ends and all tiles end together;
+// the host picks partials where that burst is heavy against the loop.  The
+// LDS adds (ds_add_f64) make the order of the additions inside a row
+// nondeterministic; the result stays within the FP64 tolerance of the
 // left-to-right CPU sum.
 #include "launch.hpp"
 #include "wave64.hpp"
@@ -244,7 +247,13 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     __syncthreads();
     double *out = y + row0;
     if (a.S == 1) {
-        for (uint32_t k = tid; k < rows; k += nt) out[k] = yl[k];
+        // non-temporal: y kept in the L2s would take room from the next
+        // SpMV's x (the 1024^2 Laplacian: 14.6 -> 13.9 us)
+        for (uint32_t k = tid; k < rows; k += nt) __builtin_nontemporal_store(yl[k], out + k);
+    } else if (a.part && !VB) {
+        // the slice's partial vector, read once by the merge: non-temporal
+        double *ps = a.part + size_t(s) * a.nrows + row0;
+        for (uint32_t k = tid; k < rows; k += nt) __builtin_nontemporal_store(yl[k], ps + k);
     } else {
         for (uint32_t k = tid; k < rows; k += nt) unsafeAtomicAdd(out + k, yl[k]);
     }
@@ -392,7 +401,8 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
     const unsigned grid = groups * 8;
     const bool vb = vbar != nullptr;
     const size_t lds = size_t(a.R) * sizeof(double) * (vb ? 2 : 1);
-    if (a.S > 1) {
+    const bool partials = a.S > 1 && a.part && !vb;
+    if (a.S > 1 && !partials) {
         const hipError_t e = clear_f64(y, a.nrows, s);
         if (e != hipSuccess) return e;
         if (mid) {
@@ -420,7 +430,12 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
             AX_RCT_LAUNCH(double, false);
     }
 #undef AX_RCT_LAUNCH
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (partials) {
+        if (e == hipSuccess && mid) e = hipEventRecord(mid, s);
+        if (e == hipSuccess) e = part_merge(a.part, a.S, a.nrows, y, s);
+    }
+    return e;
 }
 
 }  // namespace launch

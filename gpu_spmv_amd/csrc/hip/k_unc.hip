@@ -84,15 +84,6 @@ __global__ void __launch_bounds__(256) rowblock_partial_kernel(Src src, const Bl
     rowblock_process<Src, XGlobal, false>(src, blocks[wave], XGlobal{x}, ys);
 }
 
-__global__ void __launch_bounds__(256) part_merge(const double *__restrict__ part, uint32_t S, uint32_t nrows,
-                                                  double *__restrict__ y) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows) return;
-    double v = part[r];
-    for (uint32_t k = 1; k < S; ++k) v += part[size_t(k) * nrows + r];
-    y[r] = v;
-}
-
 // ------------------------------------------------------------------ adjoint
 // Value gradient of the AXT UNC th == 1 layout: plane slot q of row r is CSR
 // position rowptr[r] + q - first[r] (first[r] = first plane slot of row r);
@@ -194,8 +185,10 @@ hipError_t run(const Src &src, const axspmv::launch::SegArgs &a, const double *x
     if (a.part) {  // column slices, deterministic: partials, then their sum in slice order
         rowblock_partial_kernel<Src><<<grid, block, 0, s>>>(src, a.blocks, a.nblocks, a.empty, a.nempty, a.S,
                                                             a.nrows_add, x, a.part);
-        if (a.nrows_add) part_merge<<<(a.nrows_add + 255) / 256, 256, 0, s>>>(a.part, a.S, a.nrows_add, y);
-        return hipGetLastError();
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && a.mid) e = hipEventRecord(a.mid, s);
+        if (e == hipSuccess) e = axspmv::launch::part_merge(a.part, a.S, a.nrows_add, y, s);
+        return e;
     }
     if (a.nrows_add) {  // column slices: clear y, then add every slice's row sums
         const hipError_t c = axspmv::launch::clear_f64(y, a.nrows_add, s);

@@ -94,11 +94,15 @@ struct RctArgs {
     const uint32_t *tile_ptr;
     const uint32_t *rb_start;  // nrb + 1 row-block starts
     uint32_t nrows, R, S, nrb, vtype, block, rbits, ncols;
+    double *part;  // S > 1: S x nrows slice partials (nullptr: atomic adds into y)
 };
-// S > 1: a fill kernel clears y, then the tile kernel adds into it; mid
-// (optional) is recorded on s between the two.  vbar != nullptr (a plan of
-// A^T, 16 B of LDS per tile row): also vbar[e] = xr[row(e)] * x[col(e)] for
-// every entry, in entry order.
+// S > 1 with a.part: the tile kernel stores every slice's row sums in its
+// partial vector, then part_merge adds them in slice order into y; mid
+// (optional) is recorded on s between the two.  S > 1 without a.part: a fill
+// kernel clears y, then the tile kernel adds into it; mid is recorded
+// between those.  vbar != nullptr (a plan of A^T, 16 B of LDS per tile row,
+// no partials): also vbar[e] = xr[row(e)] * x[col(e)] for every entry, in
+// entry order.
 hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid = nullptr,
                const double *xr = nullptr, double *vbar = nullptr);
 // Y = A X (k interleaved right-hand sides) on a one-slice plan, kt (8, 16,
@@ -219,6 +223,8 @@ hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s);
 hipError_t fill_u32(uint32_t *dst, size_t n, uint32_t v, hipStream_t s);
 // y[0 .. n) = 0 in one pass (16-byte stores where y is 16-byte aligned)
 hipError_t clear_f64(double *y, size_t n, hipStream_t s);
+// y[r] = part[r] + part[nrows + r] + ... + part[(S - 1) nrows + r], in that order
+hipError_t part_merge(const double *part, uint32_t S, uint32_t nrows, double *y, hipStream_t s);
 // reads p[0 .. n) (n even, p 16-byte aligned): evicts caches without dirtying them
 hipError_t touch(const double *p, size_t n, double *sink, hipStream_t s);
 // the wave64 primitive op (enum axspmv_wave_op) lane by lane; n a multiple of 64

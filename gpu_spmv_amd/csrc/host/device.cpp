@@ -210,8 +210,9 @@ int axspmv::time_launches(const std::function<hipError_t(hipStream_t)> &fn, uint
 namespace {
 
 // AXSPMV_TIME_KERNEL: back-to-back SpMVs; event k[i] before SpMV i, m[i]
-// where its main kernel starts (if the path marks it: RCT with slices marks
-// the end of its y clear), one more after the last.  main = m[i] .. k[i+1].
+// between its two launches (if the path has two: plans with column slices),
+// one more after the last.  main = m[i] .. k[i+1] after a clear of y, k[i] ..
+// m[i] before a merge of slice partials.
 int time_kernel(axspmv_plan *p, const axspmv_operands &o, uint32_t iters, uint32_t warmup, hipStream_t s,
                 axspmv_timing *out) {
     if (iters > 4096) return fail(AXSPMV_E_INVALID, "timing: AXSPMV_TIME_KERNEL takes at most 4096 iterations");
@@ -240,7 +241,11 @@ int time_kernel(axspmv_plan *p, const axspmv_operands &o, uint32_t iters, uint32
     }
     if (e == hipSuccess) e = hipEventCreate(&k[iters]);
     if (e != hipSuccess) return hip_fail(e, "timing: event creation");
-    const bool marks = (p->kind == Plan::RCT && p->rct.S > 1) || (p->kind == Plan::AXT_UNC_H1_SLICED && !p->d_part);
+    // Sliced plans record a marker between their two launches: after the
+    // clear of y (main kernel = what follows it) or, with slice partials,
+    // before the merge (main kernel = what precedes it).
+    const bool marks = (p->kind == Plan::RCT && p->rct.S > 1) || p->kind == Plan::AXT_UNC_H1_SLICED;
+    const bool before_merge = (p->kind == Plan::RCT && p->rct.part) || (p->kind == Plan::AXT_UNC_H1_SLICED && p->d_part);
     for (uint32_t i = 0; i < warmup && e == hipSuccess; ++i) e = plan_launch(p, o.x, o.y, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) {
@@ -257,7 +262,8 @@ int time_kernel(axspmv_plan *p, const axspmv_operands &o, uint32_t iters, uint32
     std::vector<double> per(iters);  // SpMV i: k[i] .. k[i+1]
     for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) {
         float t = 0.f;
-        e = hipEventElapsedTime(&t, marks ? m[i] : k[i], k[i + 1]);
+        e = marks && before_merge ? hipEventElapsedTime(&t, k[i], m[i])
+                                  : hipEventElapsedTime(&t, marks ? m[i] : k[i], k[i + 1]);
         main_sum += t;
         if (e == hipSuccess) e = hipEventElapsedTime(&t, k[i], k[i + 1]);
         per[i] = t;

@@ -873,14 +873,34 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         block = per_tile < 2048 ? 128 : per_tile < 16384 ? 256 : 512;
     }
 
-    p.rct = launch::RctArgs{dv, dc, dl, dk, dcb, dt, drb, n, R, S, nrb, p.o.vtype, block, rbits, a->ncols};
+    // S > 1: every tile writes its rows' sums when its loop ends, and all
+    // tiles end together.  Where that write-out is heavy against the loop
+    // (S n row sums >= nnz / 6), storing them in slice partials and merging
+    // those in a second kernel beats adding them into a cleared y with FP64
+    // atomics; where it is light, the merge's extra pass over S + 1 vectors
+    // costs more than the adds.  Measured on one box, whole SpMV: the 2^20-row
+    // power-law matrix (nnz 21 M) with 4 slices 60 against 66 us, 8 slices
+    // 74 against 88 us, 2 slices 87 against 83 us; the permuted 27-point
+    // stencil on 128^3 (nnz 56 M) with 2 slices 231 against 224 us, 4 slices
+    // 242 against 240 us, 8 slices 267 against 274 us (partials against
+    // atomics).  The plan of A^T, which adds into x_bar from its fused
+    // value-gradient sweep, keeps the atomic adds.
+    double *dpart = nullptr;
+    if (S > 1 && p.lds_per_row == 8 && !(p.o.flags & AXSPMV_PLAN_RCT_ATOMIC) && 6ull * S * n >= nnz) {
+        dpart = static_cast<double *>(p.dalloc(8ull * S * n));
+        if (!dpart) return AXSPMV_E_NOMEM;
+    }
+    p.rct = launch::RctArgs{dv, dc, dl, dk, dcb, dt, drb, n, R, S, nrb, p.o.vtype, block, rbits, a->ncols, dpart};
     p.slots = nnz;
-    // matrix stream, and with S > 1 the clear of y and S atomic adds per row
-    p.stream_bytes = (vs + (packed ? 4 : 6)) * nnz + 4 * tptr.size() + (S > 1 ? 8ull * (S + 1) * n : 0);
+    // matrix stream; with S > 1 the partials written and read back and y
+    // written by the merge, or the clear of y and S atomic adds per row
+    p.stream_bytes = (vs + (packed ? 4 : 6)) * nnz + 4 * tptr.size() +
+                     (S == 1 ? 0 : dpart ? 8ull * (2 * S + 1) * n : 8ull * (S + 1) * n);
     p.launches = S > 1 ? 2 : 1;
     p.nblocks = uint32_t(ntile);
     p.kind = Plan::RCT;
-    std::snprintf(p.name, sizeof p.name, "rct_r%u_s%u%s", Rt, S, packed ? "" : "_split");
+    std::snprintf(p.name, sizeof p.name, "rct_r%u_s%u%s%s", Rt, S, packed ? "" : "_split",
+                  S > 1 && !dpart && p.lds_per_row == 8 ? "_atomic" : "");
     return AXSPMV_OK;
 }
 
@@ -1021,7 +1041,7 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
         case Plan::AXT_UNC_H1_SLICED: {
             launch::SegArgs g = seg_args(*p, p->d_sec);
             g.nrows_add = p->nrows;
-            g.mid = p->d_part ? nullptr : p->mid;
+            g.mid = p->mid;
             g.part = p->d_part;
             g.S = p->slices;
             return launch::axt_unc_h1(g, x, y, s);

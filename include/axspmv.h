@@ -297,9 +297,21 @@ typedef struct axspmv_opts {
                                     such an AXT plan (two slots per tile,
                                     slices of <= 2 MB of y_bar) and
                                     axspmv_adjoint runs on it.  The RCT
-                                    forward SpMV, its value gradient and its
-                                    fused forward + adjoint keep their LDS and
-                                    slice atomics. */
+                                    forward SpMV already adds its slices in
+                                    slice order (partials and a merge), but
+                                    its tiles sum their rows in LDS in no
+                                    fixed order; its fused forward + adjoint
+                                    keeps those LDS adds and FP64 slice adds
+                                    into x_bar. */
+#define AXSPMV_PLAN_RCT_ATOMIC 32u /* RCT with column slices: always clear y
+                                    and add each slice's row sums into it
+                                    with FP64 atomics.  By default a plan
+                                    whose slices write many row sums
+                                    against its nonzeros (slices x nrows >=
+                                    nnz / 6) stores them in slice partials
+                                    (slices x nrows doubles held by the
+                                    plan) and a second kernel adds those in
+                                    slice order; other plans add. */
 #define AXSPMV_PLAN_AX_LAYOUT 4u /* AXT UNC th == 1 or AXC, FP64 values: keep
                                     the reference's interleaved ax array on the
                                     device (tiles of thw values then thw
@@ -405,12 +417,13 @@ typedef struct axspmv_timing {
                                  preceded by a 512 MB read that evicts the
                                  Infinity Cache and L2: times from HBM */
 #define AXSPMV_TIME_KERNEL 3u /* as REGION (SpMV only, iters <= 4096), with an
-                                 event before every SpMV and one where its main
-                                 kernel starts: main_ms is that kernel's mean
-                                 time.  Only RCT and AXT plans with column
-                                 slices have a kernel before their main one
-                                 (the clear of y); other paths report
-                                 main_ms = the SpMV's time.
+                                 event before every SpMV and one between its
+                                 launches: main_ms is the mean time of its
+                                 main kernel.  Only RCT and AXT plans with
+                                 column slices launch a second kernel: the
+                                 merge of slice partials after the main one
+                                 or the clear of y before it; other paths
+                                 report main_ms = the SpMV's time.
                                  median/min/max: of the intervals between the
                                  starts of consecutive SpMVs in the region. */
 /* Time `iters` SpMVs after `warmup` untimed ones.  AXSPMV_TIME_EVENTS is the

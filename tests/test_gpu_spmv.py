@@ -60,6 +60,8 @@ KERNELS = [
     ("rct_auto", ax.FMT_RCT, dict()),
     ("rct_r1024_s4", ax.FMT_RCT, dict(tile_rows=1024, slices=4)),
     ("rct_r4096_s8", ax.FMT_RCT, dict(tile_rows=4096, slices=8)),
+    ("rct_r1024_s4_atomic", ax.FMT_RCT, dict(tile_rows=1024, slices=4, flags=ax.PLAN_RCT_ATOMIC)),
+    ("rct_r2048_s2_atomic_split", ax.FMT_RCT, dict(tile_rows=2048, slices=2, flags=ax.PLAN_RCT_ATOMIC | ax.PLAN_RCT_SPLIT)),
     ("rct_r8192_s2_b256", ax.FMT_RCT, dict(tile_rows=8192, slices=2, block=256)),
     ("rct_r512_s1", ax.FMT_RCT, dict(tile_rows=512, slices=1)),
     ("rct_r2048_s4_split", ax.FMT_RCT, dict(tile_rows=2048, slices=4, flags=ax.PLAN_RCT_SPLIT)),
@@ -128,7 +130,7 @@ def test_spmv_f64(mname, kname, fmt, kw):
 F32_KERNELS = [k for k in KERNELS if k[0] in ("csr_naive", "csr_vector_auto", "unc_h1_thw4", "unc_h4_thw64", "k1_c64", "axc_h8",
                                                 "unc_h1_thw4_s4",
                                                 "com_h1_thw64_bs256", "com_h8_thw64", "rct_r1024_s4",
-                                                "rct_r2048_s4_split")]
+                                                "rct_r1024_s4_atomic", "rct_r2048_s4_split")]
 
 
 @pytest.mark.parametrize("mname", ["rand200", "powerlaw64k", "long_rows"])
@@ -177,6 +179,20 @@ def test_rct_auto_slices(gen, slices):
     p = ax.Plan(m, ax.FMT_RCT)
     assert p.info()["name"].split("_")[2] == f"s{slices}", p.info()["name"]
     x = ax.uniform(15, m.ncols)
+    assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
+
+
+@pytest.mark.parametrize("slices,flags,atomic", [(4, 0, False), (2, 0, True), (4, ax.PLAN_RCT_ATOMIC, True)])
+def test_rct_slice_write_form(slices, flags, atomic):
+    """Slices write their row sums as partials merged in slice order when
+    slices x nrows >= nnz / 6 (here about 20 nonzeros per row: 4 slices
+    yes, 2 no), else as FP64 adds into a cleared y; AXSPMV_PLAN_RCT_ATOMIC
+    always adds.  Both give the oracle's product."""
+    m = ax.powerlaw(1 << 16, 2.0, 2048, 20.0, seed=1)
+    assert 6 * 4 * m.nrows >= m.nnz > 6 * 2 * m.nrows
+    p = ax.Plan(m, ax.FMT_RCT, tile_rows=2048, slices=slices, flags=flags)
+    assert p.info()["name"].endswith("_atomic") == atomic, p.info()["name"]
+    x = ax.uniform(19, m.ncols)
     assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
 
 
@@ -338,13 +354,19 @@ def test_timing_modes():
         t = p.time(dx, dy, iters=20, warmup=2, mode=mode, batch=10)
         assert t.mean_ms > 0 and t.min_ms <= t.median_ms <= t.max_ms and t.mode == mode
     assert rel_err(orc.ncsr(m, x), dy.download(np.float64, m.nrows)) <= TOL
-    # the main-kernel split: RCT with slices marks its tile kernel
+    # the main-kernel split: plans with slices mark their main kernel, before
+    # the merge of slice partials or after the clear of y
     t = p.time(dx, dy, iters=20, warmup=2, mode=ax.core.TIME_KERNEL)
     assert t.mode == ax.core.TIME_KERNEL and t.main_ms == pytest.approx(t.mean_ms)
+    for fmt, kw in ((ax.FMT_RCT, dict(tile_rows=1024, slices=4)),
+                    (ax.FMT_RCT, dict(tile_rows=1024, slices=4, flags=ax.PLAN_RCT_ATOMIC)),
+                    (ax.FMT_AXT, dict(thw=2, slices=4)),
+                    (ax.FMT_AXT, dict(thw=2, slices=4, flags=ax.PLAN_DETERMINISTIC))):
+        r = ax.Plan(m, fmt, **kw)
+        t = r.time(dx, dy, iters=20, warmup=2, mode=ax.core.TIME_KERNEL)
+        assert 0 < t.main_ms < t.mean_ms, kw
+        assert rel_err(orc.ncsr(m, x), dy.download(np.float64, m.nrows)) <= TOL
     r = ax.Plan(m, ax.FMT_RCT, tile_rows=1024, slices=4)
-    t = r.time(dx, dy, iters=20, warmup=2, mode=ax.core.TIME_KERNEL)
-    assert 0 < t.main_ms < t.mean_ms
-    assert rel_err(orc.ncsr(m, x), dy.download(np.float64, m.nrows)) <= TOL
     with pytest.raises(ax.AxspmvError):
         r.time(dx, dy, iters=5000, warmup=0, mode=ax.core.TIME_KERNEL)
 
