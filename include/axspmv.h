@@ -351,7 +351,13 @@ typedef struct axspmv_plan_info {
 } axspmv_plan_info;
 int axspmv_plan_get_info(const axspmv_plan *p, axspmv_plan_info *info);
 
-/* y = A x.  y is fully overwritten (no zeroing needed by the caller). */
+/* y = A x.  y is fully overwritten (no zeroing needed by the caller).
+ * Some plans keep intermediate sums in device buffers they own and use in
+ * every call: RCT and AXSPMV_PLAN_DETERMINISTIC AXT plans with column
+ * slices (slice partials), K1 and AXT COM (rows split between waves), the
+ * AX layout (its refreshed x half).  Calls on one plan that overlap in time
+ * (several streams or host threads) race; serialize them, or give each
+ * stream its own plan. */
 int axspmv_spmv(axspmv_plan *p, const double *d_x, double *d_y, void *stream);
 
 /* ------------------------------------------------- adjoint (gradients) */
