@@ -78,6 +78,19 @@ OTHERS = [
     ("rct_auto_f32val", ax.FMT_RCT, dict(vtype=ax.F32)),
     ("axt_unc_h1_thw4_f32val", ax.FMT_AXT, dict(thw=4, vtype=ax.F32)),
 ]
+# C3's rows with AXSPMV_PLAN_REORDER (the plan of B = Q A Q^T, Q the reverse
+# Cuthill-McKee order; x and y permuted inside the timed SpMV)
+RCM = ax.PLAN_REORDER
+REORDERED = [
+    ("rct_auto_rcm", ax.FMT_RCT, dict(flags=RCM)),
+    ("rct_r8192_s1_rcm", ax.FMT_RCT, dict(tile_rows=8192, slices=1, flags=RCM)),
+    ("axt_unc_h1_thw4_rcm", ax.FMT_AXT, dict(thw=4, flags=RCM)),
+    ("axt_unc_h4_thw64_rcm", ax.FMT_AXT, dict(thw=64, th=4, flags=RCM)),
+    ("axt_com_h4_thw64_rcm", ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4, flags=RCM)),
+    ("k1_c64_rcm", ax.FMT_K1, dict(chunk=64, flags=RCM)),
+    ("axc_hbs4_rcm", ax.FMT_AXC, dict(hbs=4, flags=RCM)),
+    ("csr_vector_rcm", ax.FMT_CSR, dict(kernel=ax.KER_STREAM, flags=RCM)),
+]
 
 
 def log(*a):
@@ -513,7 +526,7 @@ def suite_c3_c5(tables, extras):
     d3, e3 = ax.DeviceBuffer.from_array(x3), ax.DeviceBuffer(8 * m3.nrows)
     rows = []
     c3 = [o for o in OTHERS if not o[0].startswith("rct_r") and not o[0].endswith("f32val")]
-    for name, fmt, kw in [FLAGSHIP] + c3:
+    for name, fmt, kw in [FLAGSHIP] + c3 + REORDERED:
         _, _, r = measure(m3, name, fmt, kw, d3, e3, 20, 3, y3)
         rows.append(r)
         log(f"stencil128p: {r}")
@@ -528,13 +541,15 @@ def suite_c3_c5(tables, extras):
     y5 = orc.ncsr(m5, x5, nthreads=omp_threads())
     d5 = ax.DeviceBuffer.from_array(x5)
     e5 = ax.DeviceBuffer(8 * m5.nrows)
-    pl = ax.Plan(m5, ax.FMT_AXT, thw=4)
-    pl.time(d5, e5, iters=3, warmup=0, mode=TIME_REGION)
-    plain = pl.time(d5, e5, iters=20, warmup=0, mode=TIME_REGION).mean_ms * 1e3
-    del pl, e5
-    extras["stencil128p_fwd_adj"] = fwd_adj_only(m5, x5, d5, y5, plain, b5)
-    extras["stencil128p_fwd_adj_rct"] = fwd_adj_rct(m5, d5, ax.DeviceBuffer.from_array(b5), y5, b5)
-    log(f"stencil fwd_adj_rct: {extras['stencil128p_fwd_adj_rct']}")
+    for suffix, flags in (("", 0), ("_rcm", ax.PLAN_REORDER)):
+        pl = ax.Plan(m5, ax.FMT_AXT, thw=4, flags=flags)
+        pl.time(d5, e5, iters=3, warmup=0, mode=TIME_REGION)
+        plain = pl.time(d5, e5, iters=20, warmup=0, mode=TIME_REGION).mean_ms * 1e3
+        del pl
+        extras["stencil128p_fwd_adj" + suffix] = fwd_adj_only(m5, x5, d5, y5, plain, b5, flags=flags)
+        extras["stencil128p_fwd_adj_rct" + suffix] = fwd_adj_rct(m5, d5, ax.DeviceBuffer.from_array(b5), y5, b5,
+                                                                 flags=flags)
+        log(f"stencil fwd_adj_rct{suffix}: {extras['stencil128p_fwd_adj_rct' + suffix]}")
 
 
 def suite_c4b(extras):
@@ -580,13 +595,13 @@ def suite_c4b(extras):
     extras["batch256"] = out
 
 
-def fwd_adj_rct(m, dx, db, yref, b):
+def fwd_adj_rct(m, dx, db, yref, b, flags=0):
     """Forward + adjoint on RCT plans (the fused form: SpMV, residual, one
     sweep over the RCT plan of A^T adding x_bar and writing v_bar; the split
     form: SpMV, residual, adjoint, value gradient), against plain RCT."""
     from oracle import oracle as orc
     n, nc, nnz = m.nrows, m.ncols, m.nnz
-    p = ax.Plan(m, ax.FMT_RCT, flags=ax.PLAN_TRANSPOSE)
+    p = ax.Plan(m, ax.FMT_RCT, flags=ax.PLAN_TRANSPOSE | flags)
     dy = ax.DeviceBuffer(8 * n)
     p.time(dx, dy, iters=3, warmup=0, mode=TIME_REGION)
     plain = p.time(dx, dy, iters=20, warmup=0, mode=TIME_REGION).mean_ms * 1e3
@@ -602,14 +617,14 @@ def fwd_adj_rct(m, dx, db, yref, b):
     return out
 
 
-def fwd_adj_only(m, x, dx, yref, plain_us, b=None):
+def fwd_adj_only(m, x, dx, yref, plain_us, b=None, flags=0):
     """Forward + adjoint on the AXT UNC th = 1 plan: fused (y, r and v_bar in
     one sweep, then x_bar = A^T r) and split (four kernels)."""
     from oracle import oracle as orc
     n, nc, nnz = m.nrows, m.ncols, m.nnz
     if b is None:
         b = ax.uniform(4, n, stream=ax.VEC_STREAM + 1)
-    p = ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_TRANSPOSE)
+    p = ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_TRANSPOSE | flags)
     db = ax.DeviceBuffer.from_array(b)
     bufs = dict(y=ax.DeviceBuffer(8 * n), r=ax.DeviceBuffer(8 * n), xbar=ax.DeviceBuffer(8 * nc),
                 vbar=ax.DeviceBuffer(8 * nnz))

@@ -41,6 +41,7 @@ PLAN_AX_LAYOUT = 4
 PLAN_STAGE_X = 8
 PLAN_DETERMINISTIC = 16
 PLAN_RCT_ATOMIC = 32
+PLAN_REORDER = 64
 
 
 class Csr(C.Structure):
@@ -132,6 +133,8 @@ SIGNATURES = {
     "axspmv_csr_read": (C.c_int, [C.c_char_p, P(P(Csr))]),
     "axspmv_csr_write": (C.c_int, [P(Csr), C.c_char_p]),
     "axspmv_csr_transpose": (C.c_int, [P(Csr), P(P(Csr))]),
+    "axspmv_csr_rcm": (C.c_int, [P(Csr), P(C.c_uint32)]),
+    "axspmv_csr_permute": (C.c_int, [P(Csr), P(C.c_uint32), P(P(Csr))]),
     "axspmv_rng_u64": (C.c_int, [u64, u64, P(u64), C.c_size_t]),
     "axspmv_rng_uniform": (C.c_int, [u64, u64, f64, f64, P(f64), C.c_size_t]),
     "axspmv_rng_normal": (C.c_int, [u64, u64, f64, f64, P(f64), C.c_size_t]),

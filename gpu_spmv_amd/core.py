@@ -80,6 +80,19 @@ class CsrMatrix:
         L.call("axspmv_csr_transpose", self.ref, C.byref(out))
         return CsrMatrix._adopt(out)
 
+    def rcm(self) -> np.ndarray:
+        """Reverse Cuthill-McKee order: perm[k] = row/column numbered k."""
+        perm = np.zeros(self.nrows, dtype=np.uint32)
+        L.call("axspmv_csr_rcm", self.ref, _ptr(perm, C.c_uint32))
+        return perm
+
+    def permute(self, perm: np.ndarray) -> "CsrMatrix":
+        """P A P^T: row k is row perm[k], column perm[k] becomes k."""
+        perm = np.ascontiguousarray(perm, dtype=np.uint32)
+        out = C.POINTER(L.Csr)()
+        L.call("axspmv_csr_permute", self.ref, _ptr(perm, C.c_uint32), C.byref(out))
+        return CsrMatrix._adopt(out)
+
     def write(self, path: str) -> None:
         L.call("axspmv_csr_write", self.ref, path.encode())
 
@@ -317,15 +330,15 @@ class Plan:
         L.call("axspmv_spmm", self.ptr, k, _dptr(dX), _dptr(dY), stream)
 
     def vbar_positions(self) -> np.ndarray:
-        """CSR position of every value-gradient entry (identity except RCT)."""
+        """CSR position of every value-gradient entry (identity except RCT
+        and reordered plans)."""
         pos = np.empty(max(self.nnz, 1), np.uint32)
         L.call("axspmv_plan_vbar_positions", self.ptr, _ptr(pos, L.u32), self.nnz)
         return pos[:self.nnz]
 
     def vbar_csr(self, v: np.ndarray) -> np.ndarray:
-        """A value gradient in CSR order (undoes RCT's entry order)."""
-        if self.opts.format != L.FMT_RCT:
-            return v
+        """A value gradient in CSR order (undoes the entry order of RCT and
+        reordered plans)."""
         out = np.empty_like(v)
         out[self.vbar_positions()] = v
         return out

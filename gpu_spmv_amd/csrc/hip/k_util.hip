@@ -70,6 +70,20 @@ __global__ void __launch_bounds__(256) fill_u32_kernel(uint32_t *__restrict__ ds
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
 }
 
+// dst[e] = src[idx[e]] for e in segment b % 8 of workgroup b, [lo[j],
+// lo[j + 1]): workgroups go round-robin to the 8 XCDs, so one XCD serves one
+// segment, and when a segment's idx values stay within one eighth of src
+// (2 MB for 2M doubles) its L2 keeps that part of src: each src line comes
+// from HBM once instead of once per entry.  dst is written in order.
+__global__ void __launch_bounds__(256) gather_xcd_kernel(const uint32_t *__restrict__ idx,
+                                                         const uint32_t *__restrict__ lo,
+                                                         const double *__restrict__ src, double *__restrict__ dst) {
+    const uint32_t j = blockIdx.x % 8;
+    const size_t stride = size_t(gridDim.x / 8) * blockDim.x, e = lo[j + 1];
+    for (size_t k = lo[j] + size_t(blockIdx.x / 8) * blockDim.x + threadIdx.x; k < e; k += stride)
+        dst[k] = src[idx[k]];
+}
+
 __global__ void __launch_bounds__(256) fill_rows_kernel(const uint32_t *__restrict__ rows, uint32_t n, double *__restrict__ y) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[rows[i]] = 0.0;
@@ -196,6 +210,12 @@ hipError_t part_merge(const double *part, uint32_t S, uint32_t nrows, double *y,
 
 hipError_t fill_u32(uint32_t *dst, size_t n, uint32_t v, hipStream_t s) {
     if (n) fill_u32_kernel<<<grid_for(n, 256), 256, 0, s>>>(dst, n, v);
+    return hipGetLastError();
+}
+
+hipError_t gather_xcd(const uint32_t *idx, const uint32_t *lo, uint32_t per_segment, const double *src, double *dst,
+                      hipStream_t s) {
+    if (per_segment) gather_xcd_kernel<<<8 * ((grid_for(per_segment, 256) + 7) / 8), 256, 0, s>>>(idx, lo, src, dst);
     return hipGetLastError();
 }
 

@@ -137,6 +137,9 @@ extern "C" int axspmv_batch_create(const axspmv_csr *const *mats, uint32_t count
                                    int per_matrix, void *stream, axspmv_batch **out) {
     if (!mats || !opts || !out || count == 0) return fail(AXSPMV_E_INVALID, "batch_create: bad argument");
     *out = nullptr;
+    for (uint32_t i = 0; i < (per_matrix ? count : 1u); ++i)
+        if (opts[i].flags & AXSPMV_PLAN_REORDER)
+            return fail(AXSPMV_E_UNSUPPORTED, "batch_create: AXSPMV_PLAN_REORDER is for single plans");
     if (opts[0].format == AXSPMV_FMT_RCT) return batch_create_rct(mats, count, opts, per_matrix, stream, out);
     hipStream_t s = static_cast<hipStream_t>(stream);
     auto b = std::make_unique<axspmv_batch>();

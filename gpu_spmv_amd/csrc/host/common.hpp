@@ -45,6 +45,12 @@ int check_csr_bounds(const axspmv_csr *a, const char *who);
 // position in A its entry came from (perm may be NULL).
 int csr_transpose_perm(const axspmv_csr *a, axspmv_csr **out, std::vector<uint32_t> *perm);
 
+// Reverse Cuthill-McKee order of a square matrix (q[k] = row of A numbered
+// k) and B = Q A Q^T with, for every entry of B, its position in A (pos may
+// be NULL).
+int rcm_order(const axspmv_csr *a, std::vector<uint32_t> &q);
+int csr_permute_sym(const axspmv_csr *a, const std::vector<uint32_t> &q, axspmv_csr **out, std::vector<uint32_t> *pos);
+
 }  // namespace axspmv
 
 #define AX_TRY(expr)                        \

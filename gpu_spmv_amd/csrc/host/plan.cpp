@@ -45,6 +45,7 @@ int axspmv_plan::upload(void *dst, const void *src, size_t bytes, hipStream_t s)
 axspmv_plan::~axspmv_plan() {
     delete T;
     delete Tdet;
+    delete inner;
     for (void *p : bufs) (void)hipFree(p);
 }
 
@@ -944,6 +945,119 @@ extern "C" int axspmv_opts_default(uint32_t format, axspmv_opts *o) {
 
 namespace {
 
+int plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream, bool transposed, axspmv_plan **out);
+
+// Mean |row - column| over the entries.
+double mean_diagonal_distance(const axspmv_csr *a) {
+    double sum = 0.0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : sum)
+    for (uint32_t i = 0; i < a->nrows; ++i)
+        for (uint32_t j = a->row[i]; j < a->row[i + 1]; ++j) sum += a->col[j] > i ? a->col[j] - i : i - a->col[j];
+    return a->nnz ? sum / a->nnz : 0.0;
+}
+
+// dst[k] = src[perm[k]] in two XCD-segmented gathers through a staging
+// vector t.  Pass 1: t[e] = src[idx1[e]], the entries k grouped by the eighth
+// of src that perm[k] lies in (segment j = group j, served by XCD j, whose L2
+// holds that eighth), in order of k within a group.  Pass 2: dst[k] =
+// t[idx2[k]], segment d = the d-th eighth of dst; its entries sit in one run
+// of each group, 2 MB of t together for 2M doubles.  Both passes write in
+// order.  (One pass reads a whole line of src per entry: 34 us for the 2M
+// entries of the permuted 128^3 stencil against about 2 x 8.)
+int upload_perm(Plan &p, const std::vector<uint32_t> &perm, Plan::Perm &out, hipStream_t s) {
+    const uint32_t n = uint32_t(perm.size());
+    const uint32_t w = uint32_t(std::max<uint64_t>(1, ceil_div(n, 8)));  // entries per eighth
+    std::vector<uint32_t> lo1(9, 0), lo2(9, 0);
+    for (uint32_t k = 0; k < n; ++k) lo1[perm[k] / w + 1]++;
+    for (int j = 0; j < 8; ++j) {
+        out.per1 = std::max(out.per1, lo1[j + 1]);
+        lo1[j + 1] += lo1[j];
+        lo2[j + 1] = std::min<uint64_t>(n, uint64_t(j + 1) * w);
+    }
+    out.per2 = std::min(w, n);
+    std::vector<uint32_t> idx1(n), idx2(n), next(lo1.begin(), lo1.end() - 1);
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t e = next[perm[k] / w]++;
+        idx1[e] = perm[k];
+        idx2[k] = e;
+    }
+    out.idx1 = static_cast<uint32_t *>(p.dalloc(4ull * n));
+    out.idx2 = static_cast<uint32_t *>(p.dalloc(4ull * n));
+    out.lo1 = static_cast<uint32_t *>(p.dalloc(4 * 9));
+    out.lo2 = static_cast<uint32_t *>(p.dalloc(4 * 9));
+    if (!out.idx1 || !out.idx2 || !out.lo1 || !out.lo2) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(out.idx1, idx1.data(), 4ull * n, s));
+    AX_TRY(p.upload(out.idx2, idx2.data(), 4ull * n, s));
+    AX_TRY(p.upload(out.lo1, lo1.data(), 4 * 9, s));
+    return p.upload(out.lo2, lo2.data(), 4 * 9, s);
+}
+
+// AXSPMV_PLAN_REORDER: Q = the reverse Cuthill-McKee order of A, the plan of
+// B = Q A Q^T (every other option as given) in `inner`, the permutation and
+// its inverse on the device, and the vectors the gathers go through.
+int plan_reordered(const axspmv_csr *a, const axspmv_opts *o, void *stream, axspmv_plan **out) {
+    if (a->nrows != a->ncols)
+        return fail(AXSPMV_E_INVALID, "AXSPMV_PLAN_REORDER: the %u x %u matrix is not square", a->nrows, a->ncols);
+    std::vector<uint32_t> q, pos;
+    AX_TRY(rcm_order(a, q));
+    axspmv_csr *b = nullptr;
+    AX_TRY(csr_permute_sym(a, q, &b, &pos));
+    axspmv_opts oi = *o;
+    oi.flags &= ~AXSPMV_PLAN_REORDER;
+    // Worth it only where the new order brings the entries at least twice
+    // as close to the diagonal on average (a random matrix gains nothing, a
+    // banded one is already close): else the plan of A itself.
+    if (2.0 * mean_diagonal_distance(b) > mean_diagonal_distance(a)) {
+        axspmv_csr_free(b);
+        return plan_create(a, &oi, stream, false, out);
+    }
+    axspmv_plan *pi = nullptr;
+    const int rc = plan_create(b, &oi, stream, false, &pi);
+    axspmv_csr_free(b);
+    if (rc) return rc;
+    auto p = std::make_unique<Plan>();
+    p->inner = pi;
+    p->kind = Plan::REORDERED;
+    p->o = pi->o;
+    p->o.flags |= AXSPMV_PLAN_REORDER;
+    p->device = pi->device;
+    p->nrows = a->nrows;
+    p->ncols = a->ncols;
+    p->nnz = a->nnz;
+    p->thw = pi->thw;
+    p->th = pi->th;
+    const uint32_t n = a->nrows;
+    std::vector<uint32_t> qi(n);
+    for (uint32_t k = 0; k < n; ++k) qi[q[k]] = k;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    AX_TRY(upload_perm(*p, q, p->to_b, s));
+    AX_TRY(upload_perm(*p, qi, p->from_b, s));
+    p->nscratch = pi->T ? 6 : 3;
+    for (uint32_t k = 0; k < p->nscratch; ++k) {
+        p->d_scr[k] = static_cast<double *>(p->dalloc(8ull * n));
+        if (!p->d_scr[k]) return AXSPMV_E_NOMEM;
+    }
+    p->bpos = std::move(pos);
+    std::snprintf(p->name, sizeof p->name, "%.59s_rcm", pi->name);
+    p->slots = pi->slots;
+    p->beta = pi->beta;
+    p->device_bytes += pi->device_bytes;
+    p->alg_bytes = pi->alg_bytes;
+    // the inner plan's bytes, then per vector permuted (x into B's order
+    // before the SpMV, y back after it) two passes of 4 B of index, 8 B read
+    // and 8 B written.  (RCT tiles storing their rows straight into A's
+    // numbering, y[q[k]], measured slower on the permuted 128^3 stencil: the
+    // tile kernel took 38 us more with ordinary scattered stores, 56 us more
+    // with non-temporal ones, against 30 us for the two passes.)
+    p->stream_bytes = pi->stream_bytes + 2ull * 40 * n;
+    p->launches = pi->launches + 4;
+    p->nblocks = pi->nblocks;
+    p->staged_blocks = pi->staged_blocks;
+    p->rowwise = pi->rowwise;
+    *out = p.release();
+    return AXSPMV_OK;
+}
+
 // axspmv_plan_create; `transposed` marks the plan of A^T that a plan with
 // AXSPMV_PLAN_TRANSPOSE builds (RCT: entry positions kept, 16 B of LDS per
 // tile row for the fused value gradient).
@@ -963,6 +1077,7 @@ int plan_create(const axspmv_csr *a, const axspmv_opts *o, void *stream, bool tr
             return fail(AXSPMV_E_OVERFLOW, "plan_create: %u rows x %llu lanes per row exceed a 32-bit grid", a->nrows,
                         (unsigned long long)g);
     }
+    if (o->flags & AXSPMV_PLAN_REORDER) return plan_reordered(a, o, stream, out);
     auto p = std::make_unique<Plan>();
     p->o = *o;
     if (transposed) {
@@ -1030,6 +1145,14 @@ extern "C" int axspmv_plan_get_info(const axspmv_plan *p, axspmv_plan_info *info
 
 namespace axspmv {
 
+// dst = src permuted by q, staged in the plan's last scratch vector
+static hipError_t permute(const Plan *p, const Plan::Perm &q, const double *src, double *dst, hipStream_t s) {
+    double *t = p->d_scr[p->nscratch - 1];
+    hipError_t e = launch::gather_xcd(q.idx1, q.lo1, q.per1, src, t, s);
+    if (e == hipSuccess) e = launch::gather_xcd(q.idx2, q.lo2, q.per2, t, dst, s);
+    return e;
+}
+
 hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s) {
     switch (p->kind) {
         case Plan::CSR_NAIVE:
@@ -1051,6 +1174,12 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
         case Plan::K1: return launch::k1(p->k1, x, y, s);
         case Plan::AXC: return launch::axc(p->axc, x, y, s);
         case Plan::RCT: return launch::rct(p->rct, x, y, s, p->mid);
+        case Plan::REORDERED: {
+            hipError_t e = permute(p, p->to_b, x, p->d_scr[0], s);
+            if (e == hipSuccess) e = plan_launch(p->inner, p->d_scr[0], p->d_scr[1], s);
+            if (e == hipSuccess) e = permute(p, p->from_b, p->d_scr[1], y, s);
+            return e;
+        }
         case Plan::AXT_UNC_H1_AX: {
             const hipError_t e = launch::x_refresh(p->d_axmap, p->slots, ilog2(p->thw), p->ncols, x, p->d_ax, s);
             if (e != hipSuccess) return e;
@@ -1179,7 +1308,54 @@ int plan_add_transpose(Plan *p, const axspmv_csr *a, hipStream_t s) {
     return AXSPMV_OK;
 }
 
+// A reordered plan: inputs gathered into B's numbering (x~ = Q x: x~[k] =
+// x[perm[k]]), the operation on the plan of B, results gathered back (y =
+// Q^T y~: y[i] = y~[iperm[i]]).  With B = Q A Q^T, A^T = Q^T B^T Q, so the
+// adjoint takes the same two gathers; v_bar stays in the order of B's plan.
+static hipError_t reordered_op(Plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s) {
+    double *const *w = p->d_scr;
+    auto in = [&](const double *v, double *d) { return permute(p, p->to_b, v, d, s); };
+    auto back = [&](const double *v, double *d) { return permute(p, p->from_b, v, d, s); };
+    axspmv_operands io{};
+    hipError_t e = hipSuccess;
+    switch (op) {
+        case AXSPMV_OP_SPMV: return plan_launch(p, o.x, o.y, s);
+        case AXSPMV_OP_ADJOINT:
+            io.ybar = w[0];
+            io.xbar = w[1];
+            e = in(o.ybar, w[0]);
+            if (e == hipSuccess) e = plan_op(p->inner, op, io, s);
+            if (e == hipSuccess) e = back(w[1], o.xbar);
+            return e;
+        case AXSPMV_OP_VBAR:
+            io.ybar = w[0];
+            io.x = w[1];
+            io.vbar = o.vbar;
+            e = in(o.ybar, w[0]);
+            if (e == hipSuccess) e = in(o.x, w[1]);
+            if (e == hipSuccess) e = plan_op(p->inner, op, io, s);
+            return e;
+        case AXSPMV_OP_FWD_ADJ:
+        case AXSPMV_OP_FWD_ADJ_SPLIT:
+            io.x = w[0];
+            io.b = w[1];
+            io.y = w[2];
+            io.r = w[3];
+            io.xbar = w[4];
+            io.vbar = o.vbar;
+            e = in(o.x, w[0]);
+            if (e == hipSuccess) e = in(o.b, w[1]);
+            if (e == hipSuccess) e = plan_op(p->inner, op, io, s);
+            if (e == hipSuccess) e = back(w[2], o.y);
+            if (e == hipSuccess) e = back(w[3], o.r);
+            if (e == hipSuccess) e = back(w[4], o.xbar);
+            return e;
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t plan_op(Plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s) {
+    if (p->kind == Plan::REORDERED) return reordered_op(p, op, o, s);
     switch (op) {
         case AXSPMV_OP_SPMV: return plan_launch(p, o.x, o.y, s);
         case AXSPMV_OP_ADJOINT: return plan_launch(p->Tdet ? p->Tdet : p->T, o.ybar, o.xbar, s);
@@ -1203,6 +1379,15 @@ hipError_t plan_op(Plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s
 
 int plan_op_check(const Plan *p, uint32_t op, const axspmv_operands &o) {
     if (!p) return fail(AXSPMV_E_INVALID, "NULL plan");
+    if (p->kind == Plan::REORDERED) {
+        if (op == AXSPMV_OP_SPMM || op == AXSPMV_OP_XREFRESH)
+            return fail(AXSPMV_E_UNSUPPORTED, "%s: plan %s was created with AXSPMV_PLAN_REORDER",
+                        op == AXSPMV_OP_SPMM ? "SpMM" : "x refresh", p->name);
+        if (op > AXSPMV_OP_XREFRESH) return fail(AXSPMV_E_INVALID, "unknown operation %u", op);
+        if (op == AXSPMV_OP_SPMV && ((!o.x && p->ncols) || (!o.y && p->nrows)))
+            return fail(AXSPMV_E_INVALID, "spmv: NULL argument");
+        return plan_op_check(p->inner, op, o);  // the operation's own conditions, on the plan of B
+    }
     const bool needT = op == AXSPMV_OP_ADJOINT || op == AXSPMV_OP_FWD_ADJ || op == AXSPMV_OP_FWD_ADJ_SPLIT ||
                        (op == AXSPMV_OP_VBAR && p->kind == Plan::RCT);
     if (needT && !p->T)
@@ -1284,6 +1469,11 @@ extern "C" int axspmv_spmm(axspmv_plan *p, uint32_t k, const double *d_X, double
 extern "C" int axspmv_plan_vbar_positions(const axspmv_plan *p, uint32_t *h_pos, uint64_t count) {
     if (!p || (count && !h_pos)) return fail(AXSPMV_E_INVALID, "vbar_positions: NULL argument");
     if (count != p->nnz) return fail(AXSPMV_E_INVALID, "vbar_positions: count %llu != nnz %u", (unsigned long long)count, p->nnz);
+    if (p->kind == Plan::REORDERED) {  // positions in B, then in A
+        AX_TRY(axspmv_plan_vbar_positions(p->inner, h_pos, count));
+        for (uint64_t k = 0; k < count; ++k) h_pos[k] = p->bpos[h_pos[k]];
+        return AXSPMV_OK;
+    }
     if (p->kind == Plan::RCT) {
         if (!p->T)
             return fail(AXSPMV_E_UNSUPPORTED, "vbar_positions: RCT plan created without AXSPMV_PLAN_TRANSPOSE");

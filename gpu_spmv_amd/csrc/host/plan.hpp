@@ -11,7 +11,7 @@
 #include "common.hpp"
 
 struct axspmv_plan {
-    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC, RCT, AXT_UNC_H1_AX, CSR_VECTOR, AXT_UNC_H1_SLICED };
+    enum Kind { NONE, CSR_NAIVE, AXT_UNC_H1, AXT_UNC_H, AXT_COM, K1, AXC, RCT, AXT_UNC_H1_AX, CSR_VECTOR, AXT_UNC_H1_SLICED, REORDERED };
     Kind kind = NONE;
     int device = 0;
     axspmv_opts o{};
@@ -70,6 +70,21 @@ struct axspmv_plan {
     // nrows slice partials
     uint32_t slices = 1;
     double *d_part = nullptr;
+    // AXSPMV_PLAN_REORDER (kind REORDERED): the plan of B = Q A Q^T in
+    // `inner`; the permutation into B's order (v~[k] = v[perm[k]]) and back
+    // (v[i] = v~[iperm[i]]), each as two XCD-segmented gathers through a
+    // staging vector (pass 1 reads the source by eighths, pass 2 writes the
+    // destination by eighths); nscratch vectors of nrows doubles (3, or 6
+    // with a transposed plan; the last one stages); bpos[e] = position in A
+    // of entry e of B's CSR
+    axspmv_plan *inner = nullptr;
+    struct Perm {
+        uint32_t *idx1 = nullptr, *lo1 = nullptr, *idx2 = nullptr, *lo2 = nullptr;
+        uint32_t per1 = 0, per2 = 0;
+    } to_b, from_b;
+    double *d_scr[6] = {};
+    uint32_t nscratch = 0;
+    std::vector<uint32_t> bpos;
 
     std::vector<void *> bufs;
     void *dalloc(size_t bytes);

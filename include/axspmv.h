@@ -94,6 +94,21 @@ int axspmv_csr_write(const axspmv_csr *a, const char *path);
 /* A^T as a CSR (CSR -> CSC), columns sorted; used for the adjoint. */
 int axspmv_csr_transpose(const axspmv_csr *a, axspmv_csr **out);
 
+/* Reverse Cuthill-McKee order of a square matrix, on the pattern of
+ * A + A^T: perm[k] (nrows entries) = the row/column of A numbered k.  Each
+ * connected part starts from a pseudo-peripheral vertex (George-Liu
+ * search from a vertex of least degree); neighbours are numbered by
+ * increasing degree, ties by index; the order is then reversed.  The
+ * ordering behind AXSPMV_PLAN_REORDER.  No reference counterpart (the
+ * reference runs every matrix in the order read by matrixReading,
+ * cudaSpmv.cu:265-328). */
+int axspmv_csr_rcm(const axspmv_csr *a, uint32_t *perm);
+/* B = P A P^T for a square A: row k of B is row perm[k] of A, column c of A
+ * becomes the k with perm[k] = c, entries of a row sorted by the new
+ * column.  perm must be a permutation of 0 .. nrows-1 (else
+ * AXSPMV_E_INVALID).  Library-owned result (axspmv_csr_free). */
+int axspmv_csr_permute(const axspmv_csr *a, const uint32_t *perm, axspmv_csr **out);
+
 /* ------------------------------------------------------------ RNG + data */
 /* xoshiro256** seeded through splitmix64(seed ^ stream * 0x9E3779B97F4A7C15).
  * u64 stream, U[lo,hi) doubles from (next>>11)*2^-53, and N(mu,sigma) by
@@ -312,6 +327,32 @@ typedef struct axspmv_opts {
                                     (slices x nrows doubles held by the
                                     plan) and a second kernel adds those in
                                     slice order; other plans add. */
+#define AXSPMV_PLAN_REORDER 64u /* square matrices: build the plan for
+                                    B = Q A Q^T, Q the reverse Cuthill-McKee
+                                    order of axspmv_csr_rcm, and run every
+                                    operation in the caller's numbering:
+                                    x (and y_bar, b) gathered into Q's
+                                    order, the format's kernels on B, the
+                                    results gathered back.  x and y stay in
+                                    A's numbering; v_bar comes out in the
+                                    order of B's plan, and
+                                    axspmv_plan_vbar_positions gives A's
+                                    CSR positions.  For matrices whose
+                                    numbering scatters a banded structure
+                                    (a permuted stencil): B reads x through
+                                    a narrow window.  Where B's entries lie
+                                    on average not even twice as close to
+                                    the diagonal as A's (random or already
+                                    banded matrices) the plan is built for A
+                                    as without the flag (its name then has
+                                    no "_rcm").  The plan holds B and
+                                    two (with AXSPMV_PLAN_TRANSPOSE five)
+                                    vectors of nrows doubles.  SpMV, the
+                                    adjoint, v_bar and forward + adjoint;
+                                    SpMM and the x refresh return
+                                    AXSPMV_E_UNSUPPORTED.  Building Q costs
+                                    a few breadth-first passes over A
+                                    (host, one thread). */
 #define AXSPMV_PLAN_AX_LAYOUT 4u /* AXT UNC th == 1 or AXC, FP64 values: keep
                                     the reference's interleaved ax array on the
                                     device (tiles of thw values then thw

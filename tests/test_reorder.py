@@ -1,0 +1,138 @@
+"""Reverse Cuthill-McKee order and B = P A P^T (host code, no GPU).
+
+Pins: the order is a permutation; on paths, grids and stencils its bandwidth
+equals what scipy's independent reverse_cuthill_mckee reaches (a path gives 1,
+a g x g grid g), elsewhere it stays within 25 % of scipy's; P A P^T equals
+scipy's A[p][:, p] array for array; the product of the permuted matrix with
+the permuted x is y permuted (oracle ncsr on both sides).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import reverse_cuthill_mckee
+
+import gpu_spmv_amd as ax
+from conftest import rel_err
+from oracle import oracle as orc
+
+
+def bandwidth(m):
+    if m.nnz == 0:
+        return 0
+    return int(np.max(np.abs(m.row_ids().astype(np.int64) - m.col.astype(np.int64))))
+
+
+def scipy_of(m):
+    return sp.csr_matrix((m.val, m.col, m.row), shape=(m.nrows, m.ncols))
+
+
+def scipy_rcm_bandwidth(m):
+    p = reverse_cuthill_mckee(scipy_of(m), symmetric_mode=False)
+    return bandwidth(m.permute(p.astype(np.uint32)))
+
+
+def shuffled(m, seed):
+    p = np.random.default_rng(seed).permutation(m.nrows).astype(np.uint32)
+    return m.permute(p)
+
+
+def path(n, seed):
+    """Tridiagonal n x n matrix (a path graph), rows and columns shuffled."""
+    i = np.arange(n)
+    rows = np.concatenate([i, i[:-1], i[1:]])
+    cols = np.concatenate([i, i[1:], i[:-1]])
+    A = sp.csr_matrix((np.ones(rows.size), (rows, cols)), shape=(n, n))
+    A.sort_indices()
+    m = ax.CsrMatrix(A.indptr.astype(np.uint32), A.indices.astype(np.uint32), A.data, ncols=n)
+    return shuffled(m, seed)
+
+
+def two_paths_and_isolated():
+    a, b = path(300, 1), path(200, 2)
+    rows = [a.row, b.row[1:] + a.nnz]
+    row = np.concatenate(rows + [np.full(50, a.nnz + b.nnz)]).astype(np.uint32)
+    col = np.concatenate([a.col, b.col + a.nrows]).astype(np.uint32)
+    val = np.concatenate([a.val, b.val])
+    return ax.CsrMatrix(row, col, val, ncols=550)
+
+
+MATS = {
+    "stencil12p": lambda: ax.stencil27(12, permute=True, seed=2),
+    "stencil16p": lambda: ax.stencil27(16, permute=True, seed=5),
+    "laplace40_shuffled": lambda: shuffled(ax.laplace2d(40), 3),
+    "path1000": lambda: path(1000, 4),
+    "two_paths_isolated": two_paths_and_isolated,
+    "random_nonsym": lambda: ax.random_sparse(2000, 1, 6, seed=3),
+    "banded_empty_rows": lambda: ax.banded(3000, 4, 0.3, seed=6),
+    "powerlaw4k": lambda: ax.powerlaw(4096, 2.0, 256, 20.0, seed=1),
+    "one_row": lambda: ax.CsrMatrix(np.array([0, 1], np.uint32), np.array([0], np.uint32), np.array([2.0])),
+}
+
+
+@pytest.mark.parametrize("name", list(MATS))
+def test_rcm_is_a_permutation(name):
+    m = MATS[name]()
+    p = m.rcm()
+    assert np.array_equal(np.sort(p), np.arange(m.nrows, dtype=np.uint32))
+    assert np.array_equal(m.rcm(), p)  # deterministic (the search runs on several threads)
+
+
+@pytest.mark.parametrize("name,exact", [("stencil12p", True), ("stencil16p", True), ("laplace40_shuffled", True),
+                                        ("path1000", True), ("two_paths_isolated", True),
+                                        ("random_nonsym", False), ("banded_empty_rows", False),
+                                        ("powerlaw4k", False)])
+def test_rcm_bandwidth_against_scipy(name, exact):
+    m = MATS[name]()
+    ours = bandwidth(m.permute(m.rcm()))
+    theirs = scipy_rcm_bandwidth(m)
+    if exact:
+        assert ours == theirs, (ours, theirs)
+    else:
+        assert ours <= 1.25 * theirs + 2, (ours, theirs)
+    if name != "banded_empty_rows":  # already banded: RCM widens its band of 4 (scipy's to 11)
+        assert ours <= bandwidth(m)
+
+
+def test_rcm_closed_forms():
+    assert bandwidth(MATS["path1000"]().permute(MATS["path1000"]().rcm())) == 1
+    m = MATS["two_paths_isolated"]()
+    assert bandwidth(m.permute(m.rcm())) == 1
+    g = 40
+    m = MATS["laplace40_shuffled"]()
+    assert bandwidth(m.permute(m.rcm())) == g
+
+
+@pytest.mark.parametrize("name", ["stencil12p", "random_nonsym", "banded_empty_rows", "two_paths_isolated"])
+def test_permute_matches_scipy(name):
+    m = MATS[name]()
+    p = m.rcm()
+    b = m.permute(p)
+    ref = scipy_of(m)[p][:, p].tocsr()
+    ref.sort_indices()
+    assert np.array_equal(b.row, ref.indptr)
+    assert np.array_equal(b.col, ref.indices)
+    assert np.array_equal(b.val, ref.data)
+
+
+@pytest.mark.parametrize("name", ["stencil16p", "powerlaw4k", "banded_empty_rows"])
+def test_permuted_product(name):
+    m = MATS[name]()
+    p = m.rcm()
+    b = m.permute(p)
+    x = ax.uniform(7, m.ncols)
+    y = orc.ncsr(m, x)
+    assert rel_err(y[p], orc.ncsr(b, x[p])) <= 1e-14
+
+
+def test_rejects():
+    rect = ax.CsrMatrix(np.array([0, 1, 2], np.uint32), np.array([0, 2], np.uint32), np.ones(2), ncols=3)
+    with pytest.raises(ax.AxspmvError):
+        rect.rcm()
+    m = MATS["path1000"]()
+    bad = np.arange(m.nrows, dtype=np.uint32)
+    bad[5] = 6  # 6 twice, 5 missing
+    with pytest.raises(ax.AxspmvError):
+        m.permute(bad)
+    bad[5] = m.nrows  # out of range
+    with pytest.raises(ax.AxspmvError):
+        m.permute(bad)
