@@ -460,13 +460,15 @@ def pmc_probe(matrix: str = "powerlaw"):
     power-law matrix (or the permuted 27-point stencil on 128^3), 5 SpMVs of
     each flagship plan (AXT UNC th = 1 in one layout and in 4 column slices,
     and RCT); with "spmm", 3 SpMMs at k = 8, 32 and 64 of the plans the
-    autotuner picks on the power-law matrix."""
+    autotuner picks on the power-law matrix; with "stencil_rcm", 5 SpMVs of
+    the reordered RCT (r8192, one slice) and AXT UNC th = 1 plans on the
+    permuted stencil."""
     n = 32 << 20
     src, dst = ax.DeviceBuffer(8 * n), ax.DeviceBuffer(8 * n)
     L.call("axspmv_copy_f64", src.ptr, dst.ptr, n, None)
     L.call("axspmv_device_sync")
     del src, dst
-    if matrix == "stencil":
+    if matrix in ("stencil", "stencil_rcm"):
         m, x = W.config3()
     else:
         m, x = W.config2()
@@ -483,7 +485,10 @@ def pmc_probe(matrix: str = "powerlaw"):
             log(f"pmc-probe: spmm k={k} {p.info()['name']} ktile={kw['ktile']}")
             del p, dX, dY
         return
-    for _, fmt, kw in (FLAGSHIP_AXT, FLAGSHIP_AXT_SLICED, FLAGSHIP):
+    plans = (FLAGSHIP_AXT, FLAGSHIP_AXT_SLICED, FLAGSHIP)
+    if matrix == "stencil_rcm":
+        plans = [r for r in REORDERED if r[0] in ("rct_r8192_s1_rcm", "axt_unc_h1_thw4_rcm")]
+    for _, fmt, kw in plans:
         p = ax.Plan(m, fmt, **kw)
         for _ in range(5):
             p.spmv(dx, dy)
@@ -652,7 +657,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
-    ap.add_argument("--pmc-probe", nargs="?", const="powerlaw", choices=["powerlaw", "stencil", "spmm"])
+    ap.add_argument("--pmc-probe", nargs="?", const="powerlaw", choices=["powerlaw", "stencil", "spmm", "stencil_rcm"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: SpMV here runs on a single MI355X; "

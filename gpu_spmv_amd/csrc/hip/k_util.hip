@@ -70,18 +70,51 @@ __global__ void __launch_bounds__(256) fill_u32_kernel(uint32_t *__restrict__ ds
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
 }
 
-// dst[e] = src[idx[e]] for e in segment b % 8 of workgroup b, [lo[j],
-// lo[j + 1]): workgroups go round-robin to the 8 XCDs, so one XCD serves one
-// segment, and when a segment's idx values stay within one eighth of src
-// (2 MB for 2M doubles) its L2 keeps that part of src: each src line comes
-// from HBM once instead of once per entry.  dst is written in order.
-__global__ void __launch_bounds__(256) gather_xcd_kernel(const uint32_t *__restrict__ idx,
-                                                         const uint32_t *__restrict__ lo,
-                                                         const double *__restrict__ src, double *__restrict__ dst) {
-    const uint32_t j = blockIdx.x % 8;
-    const size_t stride = size_t(gridDim.x / 8) * blockDim.x, e = lo[j + 1];
-    for (size_t k = lo[j] + size_t(blockIdx.x / 8) * blockDim.x + threadIdx.x; k < e; k += stride)
-        dst[k] = src[idx[k]];
+// One chunk of a permutation per workgroup, through LDS: chunk c covers
+// positions [c L, c L + len) of src; lds[i] = src[c L + i] (read whole), then
+// dst[w ? w[c L + i] : c L + i] = lds[b[c L + i]].  A permutation runs as two
+// such launches through a staging vector (the host arranges w and b): the
+// first reads a block of the source and writes its values in runs, one run
+// per destination chunk; the second reads a destination chunk's values, now
+// side by side, and writes the chunk in order.  No lane reads memory at
+// random; each lane issues U loads before it waits.
+template <bool W>
+__global__ void __launch_bounds__(1024) permute_chunk_kernel(const uint32_t *__restrict__ w,
+                                                             const uint16_t *__restrict__ b, uint32_t n, uint32_t L,
+                                                             const double *__restrict__ src,
+                                                             double *__restrict__ dst) {
+    constexpr int U = 8;
+    extern __shared__ double lds[];
+    const uint64_t base = uint64_t(blockIdx.x) * L;
+    const uint32_t len = n - base < L ? uint32_t(n - base) : L, nt = blockDim.x;
+    for (uint32_t i0 = threadIdx.x; i0 < len; i0 += U * nt) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * nt;
+            if (i < len) v[u] = src[base + i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * nt;
+            if (i < len) lds[i] = v[u];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i0 = threadIdx.x; i0 < len; i0 += U * nt) {
+        uint32_t j[U], to[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * nt;
+            j[u] = i < len ? b[base + i] : 0;
+            if constexpr (W) to[u] = i < len ? w[base + i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * nt;
+            if (i < len) dst[W ? to[u] : base + i] = lds[j[u]];
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) fill_rows_kernel(const uint32_t *__restrict__ rows, uint32_t n, double *__restrict__ y) {
@@ -213,9 +246,13 @@ hipError_t fill_u32(uint32_t *dst, size_t n, uint32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t gather_xcd(const uint32_t *idx, const uint32_t *lo, uint32_t per_segment, const double *src, double *dst,
-                      hipStream_t s) {
-    if (per_segment) gather_xcd_kernel<<<8 * ((grid_for(per_segment, 256) + 7) / 8), 256, 0, s>>>(idx, lo, src, dst);
+hipError_t permute_chunks(const uint32_t *w, const uint16_t *b, uint32_t n, uint32_t L, const double *src,
+                          double *dst, hipStream_t s) {
+    const unsigned grid = unsigned((uint64_t(n) + L - 1) / L);
+    if (n && w)
+        permute_chunk_kernel<true><<<grid, 1024, size_t(L) * 8, s>>>(w, b, n, L, src, dst);
+    else if (n)
+        permute_chunk_kernel<false><<<grid, 1024, size_t(L) * 8, s>>>(w, b, n, L, src, dst);
     return hipGetLastError();
 }
 

@@ -230,11 +230,12 @@ hipError_t touch(const double *p, size_t n, double *sink, hipStream_t s);
 // the wave64 primitive op (enum axspmv_wave_op) lane by lane; n a multiple of 64
 hipError_t wave_selftest(uint32_t op, const double *v, const uint32_t *h, double *ov, uint32_t *oh, size_t n,
                          hipStream_t s);
-// dst[e] = src[idx[e]] for e < lo[8], in 8 segments [lo[j], lo[j + 1]) of at
-// most per_segment entries, segment j served by XCD j (every idx[e] < the
-// length of src)
-hipError_t gather_xcd(const uint32_t *idx, const uint32_t *lo, uint32_t per_segment, const double *src, double *dst,
-                      hipStream_t s);
+// For every chunk c of L positions of src (n in all; the last chunk may be
+// short; L <= 65536 and L * 8 bytes of LDS): lds[i] = src[c L + i], then
+// dst[w ? w[c L + i] : c L + i] = lds[b[c L + i]] (every w value < the length
+// of dst, every b value < the chunk's length)
+hipError_t permute_chunks(const uint32_t *w, const uint16_t *b, uint32_t n, uint32_t L, const double *src,
+                          double *dst, hipStream_t s);
 // y[rows[i]] = 0 for i < n
 hipError_t fill_rows(const uint32_t *rows, uint32_t n, double *y, hipStream_t s);
 

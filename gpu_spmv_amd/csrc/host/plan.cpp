@@ -956,40 +956,57 @@ double mean_diagonal_distance(const axspmv_csr *a) {
     return a->nnz ? sum / a->nnz : 0.0;
 }
 
-// dst[k] = src[perm[k]] in two XCD-segmented gathers through a staging
-// vector t.  Pass 1: t[e] = src[idx1[e]], the entries k grouped by the eighth
-// of src that perm[k] lies in (segment j = group j, served by XCD j, whose L2
-// holds that eighth), in order of k within a group.  Pass 2: dst[k] =
-// t[idx2[k]], segment d = the d-th eighth of dst; its entries sit in one run
-// of each group, 2 MB of t together for 2M doubles.  Both passes write in
-// order.  (One pass reads a whole line of src per entry: 34 us for the 2M
-// entries of the permuted 128^3 stencil against about 2 x 8.)
+// dst[k] = src[perm[k]] in two passes of launch::permute_chunks through a
+// staging vector t, in chunks of L = 8192 positions (64 KB of LDS).  t holds
+// the values chunk of destinations by chunk (t[c L ..] feeds dst[c L ..]),
+// inside a chunk by source block, then by destination.  Pass 1, workgroup s
+// = source block s (src[s L ..]): reads the block, writes each value to its
+// place in t (w1), the block's values for one destination chunk side by
+// side (b1 = the value's offset in the block, in the order of w1).  Pass 2,
+// workgroup c: reads t[c L ..] whole, dst[k] = that chunk's value b2[k].
+// Measured on the permuted 128^3 stencil (2M entries, about 32 values per
+// run; kernel trace): 11.3 + 8.3 us per vector, against 34 us for one plain
+// gather (each 8-byte read pulls a whole line from the Infinity Cache), 29
+// us for two XCD-segmented gathers (each XCD reading an L2-resident eighth
+// of the source at random), 20.5 us with the runs on the reading side and
+// 23.8 us with chunks of 4096.
 int upload_perm(Plan &p, const std::vector<uint32_t> &perm, Plan::Perm &out, hipStream_t s) {
     const uint32_t n = uint32_t(perm.size());
-    const uint32_t w = uint32_t(std::max<uint64_t>(1, ceil_div(n, 8)));  // entries per eighth
-    std::vector<uint32_t> lo1(9, 0), lo2(9, 0);
-    for (uint32_t k = 0; k < n; ++k) lo1[perm[k] / w + 1]++;
-    for (int j = 0; j < 8; ++j) {
-        out.per1 = std::max(out.per1, lo1[j + 1]);
-        lo1[j + 1] += lo1[j];
-        lo2[j + 1] = std::min<uint64_t>(n, uint64_t(j + 1) * w);
+    const uint32_t L = std::max(1u, std::min<uint32_t>(8192, n));
+    const uint64_t nch = ceil_div(n, L);
+    // t position of every destination: chunk by chunk, sorted by (source block, k)
+    std::vector<uint32_t> tp(n ? n : 1);
+    std::vector<uint16_t> b1(n ? n : 1), b2(n ? n : 1);
+#pragma omp parallel for schedule(static)
+    for (uint64_t c = 0; c < nch; ++c) {
+        const uint32_t k0 = uint32_t(c * L), k1 = uint32_t(std::min<uint64_t>(n, k0 + uint64_t(L)));
+        std::vector<std::pair<uint32_t, uint32_t>> v;  // (source block, destination)
+        v.reserve(k1 - k0);
+        for (uint32_t k = k0; k < k1; ++k) v.emplace_back(perm[k] / L, k);
+        std::sort(v.begin(), v.end());
+        for (uint32_t i = 0; i < k1 - k0; ++i) {
+            tp[v[i].second] = k0 + i;
+            b2[v[i].second] = uint16_t(i);
+        }
     }
-    out.per2 = std::min(w, n);
-    std::vector<uint32_t> idx1(n), idx2(n), next(lo1.begin(), lo1.end() - 1);
-    for (uint32_t k = 0; k < n; ++k) {
-        const uint32_t e = next[perm[k] / w]++;
-        idx1[e] = perm[k];
-        idx2[k] = e;
+    // pass 1: each source block's values in order of their t positions (a
+    // counting pass over t: destination k sits at t position tp[k])
+    std::vector<uint32_t> kat(n ? n : 1), w1(n ? n : 1), next(nch);
+    for (uint32_t k = 0; k < n; ++k) kat[tp[k]] = k;
+    for (uint64_t c = 0; c < nch; ++c) next[c] = uint32_t(c * L);
+    for (uint32_t t = 0; t < n; ++t) {
+        const uint32_t k = kat[t], blk = perm[k] / L, j = next[blk]++;
+        w1[j] = t;
+        b1[j] = uint16_t(perm[k] - blk * L);
     }
-    out.idx1 = static_cast<uint32_t *>(p.dalloc(4ull * n));
-    out.idx2 = static_cast<uint32_t *>(p.dalloc(4ull * n));
-    out.lo1 = static_cast<uint32_t *>(p.dalloc(4 * 9));
-    out.lo2 = static_cast<uint32_t *>(p.dalloc(4 * 9));
-    if (!out.idx1 || !out.idx2 || !out.lo1 || !out.lo2) return AXSPMV_E_NOMEM;
-    AX_TRY(p.upload(out.idx1, idx1.data(), 4ull * n, s));
-    AX_TRY(p.upload(out.idx2, idx2.data(), 4ull * n, s));
-    AX_TRY(p.upload(out.lo1, lo1.data(), 4 * 9, s));
-    return p.upload(out.lo2, lo2.data(), 4 * 9, s);
+    out.L = L;
+    out.b1 = static_cast<uint16_t *>(p.dalloc(2ull * n));
+    out.b2 = static_cast<uint16_t *>(p.dalloc(2ull * n));
+    out.w1 = static_cast<uint32_t *>(p.dalloc(4ull * n));
+    if (!out.b1 || !out.b2 || !out.w1) return AXSPMV_E_NOMEM;
+    AX_TRY(p.upload(out.b1, b1.data(), 2ull * n, s));
+    AX_TRY(p.upload(out.b2, b2.data(), 2ull * n, s));
+    return p.upload(out.w1, w1.data(), 4ull * n, s);
 }
 
 // AXSPMV_PLAN_REORDER: Q = the reverse Cuthill-McKee order of A, the plan of
@@ -1044,12 +1061,12 @@ int plan_reordered(const axspmv_csr *a, const axspmv_opts *o, void *stream, axsp
     p->device_bytes += pi->device_bytes;
     p->alg_bytes = pi->alg_bytes;
     // the inner plan's bytes, then per vector permuted (x into B's order
-    // before the SpMV, y back after it) two passes of 4 B of index, 8 B read
-    // and 8 B written.  (RCT tiles storing their rows straight into A's
+    // before the SpMV, y back after it) two passes of 8 B read and 8 B
+    // written, with 2 + 6 B of indices.  (RCT tiles storing their rows straight into A's
     // numbering, y[q[k]], measured slower on the permuted 128^3 stencil: the
     // tile kernel took 38 us more with ordinary scattered stores, 56 us more
     // with non-temporal ones, against 30 us for the two passes.)
-    p->stream_bytes = pi->stream_bytes + 2ull * 40 * n;
+    p->stream_bytes = pi->stream_bytes + 2ull * 40 * n;  // 2 vectors x (16 + 16 + 8) B
     p->launches = pi->launches + 4;
     p->nblocks = pi->nblocks;
     p->staged_blocks = pi->staged_blocks;
@@ -1148,8 +1165,8 @@ namespace axspmv {
 // dst = src permuted by q, staged in the plan's last scratch vector
 static hipError_t permute(const Plan *p, const Plan::Perm &q, const double *src, double *dst, hipStream_t s) {
     double *t = p->d_scr[p->nscratch - 1];
-    hipError_t e = launch::gather_xcd(q.idx1, q.lo1, q.per1, src, t, s);
-    if (e == hipSuccess) e = launch::gather_xcd(q.idx2, q.lo2, q.per2, t, dst, s);
+    hipError_t e = launch::permute_chunks(q.w1, q.b1, p->nrows, q.L, src, t, s);
+    if (e == hipSuccess) e = launch::permute_chunks(nullptr, q.b2, p->nrows, q.L, t, dst, s);
     return e;
 }
 

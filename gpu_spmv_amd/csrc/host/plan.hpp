@@ -72,15 +72,15 @@ struct axspmv_plan {
     double *d_part = nullptr;
     // AXSPMV_PLAN_REORDER (kind REORDERED): the plan of B = Q A Q^T in
     // `inner`; the permutation into B's order (v~[k] = v[perm[k]]) and back
-    // (v[i] = v~[iperm[i]]), each as two XCD-segmented gathers through a
-    // staging vector (pass 1 reads the source by eighths, pass 2 writes the
-    // destination by eighths); nscratch vectors of nrows doubles (3, or 6
-    // with a transposed plan; the last one stages); bpos[e] = position in A
-    // of entry e of B's CSR
+    // (v[i] = v~[iperm[i]]), each as two launch::permute_chunks passes
+    // through a staging vector (pass 1: w1, b1; pass 2: b2; chunks of L);
+    // nscratch vectors of nrows doubles (3, or 6 with a transposed plan; the
+    // last one stages); bpos[e] = position in A of entry e of B's CSR
     axspmv_plan *inner = nullptr;
     struct Perm {
-        uint32_t *idx1 = nullptr, *lo1 = nullptr, *idx2 = nullptr, *lo2 = nullptr;
-        uint32_t per1 = 0, per2 = 0;
+        uint16_t *b1 = nullptr, *b2 = nullptr;
+        uint32_t *w1 = nullptr;
+        uint32_t L = 0;
     } to_b, from_b;
     double *d_scr[6] = {};
     uint32_t nscratch = 0;
