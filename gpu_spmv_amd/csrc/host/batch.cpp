@@ -383,6 +383,14 @@ std::vector<axspmv_opts> default_candidates(uint32_t op, bool for_batch = false)
         o.block = block;
         c.push_back(o);
     }
+    // renumbered by reverse Cuthill-McKee (square matrices only; where the
+    // renumbering brings nothing the plan is the plain one, timed again)
+    o.block = 0;
+    o.flags = AXSPMV_PLAN_REORDER;
+    c.push_back(o);
+    axspmv_opts_default(AXSPMV_FMT_AXT, &o);
+    o.flags = AXSPMV_PLAN_REORDER;
+    c.push_back(o);
     return c;
 }
 

@@ -54,6 +54,9 @@ CANDIDATES = [
     ("axt_com_h1_thw64_bs256", L.FMT_AXT, dict(mode=L.AXT_COM, thw=64, th=1, bs=256), False),
     ("axt_com_h4_thw64", L.FMT_AXT, dict(mode=L.AXT_COM, thw=64, th=4), False),
     ("rct_auto", L.FMT_RCT, dict(), True),
+    # renumbered by reverse Cuthill-McKee inside the plan (square matrices)
+    ("rct_auto_rcm", L.FMT_RCT, dict(flags=L.PLAN_REORDER), False),
+    ("axt_unc_h1_thw4_rcm", L.FMT_AXT, dict(thw=4, flags=L.PLAN_REORDER), False),
 ]
 
 GENERATORS = {

@@ -618,7 +618,9 @@ int axspmv_graph_destroy(axspmv_graph *g);
  * reference sweeps block sizes by relaunching (jobLauncher.sh:145-176). */
 /* The built-in candidates: for SpMV AXT UNC th = 1 with thw 2..16 and th =
  * 2..8 with thw 64 at two block budgets, AXT UNC th = 1 (thw 2) in 2, 4 and
- * 8 column slices, K1 (chunk 64) and RCT (default, 256 and 1024 threads); for batches the AXT part and the default RCT options;
+ * 8 column slices, K1 (chunk 64), RCT (default, 256 and 1024 threads), and
+ * RCT and AXT UNC th = 1 (thw 4) with AXSPMV_PLAN_REORDER (rejected, time
+ * -1, for non-square matrices); for batches the AXT part and the default RCT options;
  * for SpMM the four AXT k-tiles and eleven RCT plans
  * (ktile 8 with 1024 and 2048 rows, 16 with 512, 32 with 256, each with one
  * column slice and with the default slicing; and ktile 8 / 16 / 32 with 512
