@@ -539,6 +539,7 @@ def suite_c3_c5(tables, extras):
                                         "x ~ U[-1,1] seed 2", nrows=m3.nrows, nnz=m3.nnz,
                                  formats=formats_table(m3, x3, d3), kernels=rows)
     extras["stencil128p_rocsparse"] = rocsparse_rows(m3, d3, y3, iters=20)
+    extras["stencil128p_spmm_k8"] = spmm_reordered(m3)
     del e3
     # config 5: forward + adjoint on the stencil structure, values N(1, 0.1)
     m5, x5, b5 = W.config5(m3)
@@ -555,6 +556,32 @@ def suite_c3_c5(tables, extras):
         extras["stencil128p_fwd_adj_rct" + suffix] = fwd_adj_rct(m5, d5, ax.DeviceBuffer.from_array(b5), y5, b5,
                                                                  flags=flags)
         log(f"stencil fwd_adj_rct{suffix}: {extras['stencil128p_fwd_adj_rct' + suffix]}")
+
+
+def spmm_reordered(m, k=8):
+    """SpMM with k right-hand sides on the permuted stencil, plain and with
+    AXSPMV_PLAN_REORDER (X's rows moved into the renumbered order and Y's
+    back inside the call), against the FP64 oracle on sampled rows."""
+    from oracle import oracle as orc
+    n, nnz = m.nrows, m.nnz
+    X = ax.uniform(3, m.ncols * k).reshape(m.ncols, k)
+    dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * n * k)
+    rows = np.arange(0, n, max(1, n // 2000))
+    Yref = orc.spmm(sub_rows(m, rows), X, nthreads=omp_threads())
+    out = []
+    for name, fmt, kw in (("rct_r1024", ax.FMT_RCT, dict(tile_rows=1024, ktile=8)),
+                          ("axt_unc_h1_thw4", ax.FMT_AXT, dict(thw=4, ktile=8))):
+        for suffix, flags in (("", 0), ("_rcm", ax.PLAN_REORDER)):
+            p = ax.Plan(m, fmt, flags=flags, **kw)
+            t = p.time_op(L.OP_SPMM, iters=10, warmup=2, k=k, X=dX, Y=dY)
+            Y = dY.download(np.float64, n * k).reshape(n, k)
+            us = t.mean_ms * 1e3
+            out.append(dict(k=k, label=name + suffix, kernel=p.info()["name"], us=round(us, 1),
+                            gflops=round(2.0 * nnz * k / (us * 1e-6) / 1e9, 1),
+                            rel_err_vs_f64_oracle=rel_err(Yref, Y[rows]), rows_checked=int(rows.size)))
+            log(f"stencil spmm: {out[-1]}")
+            del p
+    return out
 
 
 def suite_c4b(extras):

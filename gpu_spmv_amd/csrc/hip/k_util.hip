@@ -117,6 +117,19 @@ __global__ void __launch_bounds__(1024) permute_chunk_kernel(const uint32_t *__r
     }
 }
 
+// dst[i k + j] = src[idx[i] k + j]: rows of k values moved whole (a
+// reordered plan's SpMM operands), one lane per value, a row's lanes side by
+// side.
+__global__ void __launch_bounds__(256) gather_rows_kernel(const uint32_t *__restrict__ idx, uint64_t total,
+                                                          uint32_t k, const double *__restrict__ src,
+                                                          double *__restrict__ dst) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const size_t i = t / k, j = t - i * k;
+        dst[t] = src[size_t(idx[i]) * k + j];
+    }
+}
+
 __global__ void __launch_bounds__(256) fill_rows_kernel(const uint32_t *__restrict__ rows, uint32_t n, double *__restrict__ y) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[rows[i]] = 0.0;
@@ -253,6 +266,12 @@ hipError_t permute_chunks(const uint32_t *w, const uint16_t *b, uint32_t n, uint
         permute_chunk_kernel<true><<<grid, 1024, size_t(L) * 8, s>>>(w, b, n, L, src, dst);
     else if (n)
         permute_chunk_kernel<false><<<grid, 1024, size_t(L) * 8, s>>>(w, b, n, L, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t gather_rows(const uint32_t *idx, uint32_t n, uint32_t k, const double *src, double *dst, hipStream_t s) {
+    const uint64_t total = uint64_t(n) * k;
+    if (total) gather_rows_kernel<<<grid_for(total, 256), 256, 0, s>>>(idx, total, k, src, dst);
     return hipGetLastError();
 }
 

@@ -236,6 +236,9 @@ hipError_t wave_selftest(uint32_t op, const double *v, const uint32_t *h, double
 // of dst, every b value < the chunk's length)
 hipError_t permute_chunks(const uint32_t *w, const uint16_t *b, uint32_t n, uint32_t L, const double *src,
                           double *dst, hipStream_t s);
+// dst[i k + j] = src[idx[i] k + j] for i < n, j < k (every idx[i] k + k <= the
+// length of src)
+hipError_t gather_rows(const uint32_t *idx, uint32_t n, uint32_t k, const double *src, double *dst, hipStream_t s);
 // y[rows[i]] = 0 for i < n
 hipError_t fill_rows(const uint32_t *rows, uint32_t n, double *y, hipStream_t s);
 

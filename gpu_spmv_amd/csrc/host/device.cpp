@@ -281,6 +281,7 @@ extern "C" int axspmv_time_op(axspmv_plan *p, uint32_t op, const axspmv_operands
                               uint32_t warmup, uint32_t mode, uint32_t graph_batch, void *stream, axspmv_timing *out) {
     if (!p || !ops) return fail(AXSPMV_E_INVALID, "time_op: NULL argument");
     AX_TRY(plan_op_check(p, op, *ops));
+    AX_TRY(plan_op_prepare(p, op, *ops));
     const axspmv_operands o = *ops;
     if (mode == AXSPMV_TIME_KERNEL) {
         if (op != AXSPMV_OP_SPMV) return fail(AXSPMV_E_INVALID, "timing: AXSPMV_TIME_KERNEL times SpMV only");
@@ -338,6 +339,7 @@ extern "C" int axspmv_plan_graph(axspmv_plan *p, uint32_t op, const axspmv_opera
                                  axspmv_graph **out) {
     if (!p || !ops) return fail(AXSPMV_E_INVALID, "plan_graph: NULL argument");
     AX_TRY(plan_op_check(p, op, *ops));
+    AX_TRY(plan_op_prepare(p, op, *ops));
     const axspmv_operands o = *ops;
     return capture_graph([p, op, o](hipStream_t st) { return plan_op(p, op, o, st); }, repeats, out);
 }

@@ -1049,6 +1049,11 @@ int plan_reordered(const axspmv_csr *a, const axspmv_opts *o, void *stream, axsp
     hipStream_t s = static_cast<hipStream_t>(stream);
     AX_TRY(upload_perm(*p, q, p->to_b, s));
     AX_TRY(upload_perm(*p, qi, p->from_b, s));
+    p->d_perm = static_cast<uint32_t *>(p->dalloc(4ull * n));
+    p->d_iperm = static_cast<uint32_t *>(p->dalloc(4ull * n));
+    if (!p->d_perm || !p->d_iperm) return AXSPMV_E_NOMEM;
+    AX_TRY(p->upload(p->d_perm, q.data(), 4ull * n, s));
+    AX_TRY(p->upload(p->d_iperm, qi.data(), 4ull * n, s));
     p->nscratch = pi->T ? 6 : 3;
     for (uint32_t k = 0; k < p->nscratch; ++k) {
         p->d_scr[k] = static_cast<double *>(p->dalloc(8ull * n));
@@ -1367,6 +1372,22 @@ static hipError_t reordered_op(Plan *p, uint32_t op, const axspmv_operands &o, h
             if (e == hipSuccess) e = back(w[3], o.r);
             if (e == hipSuccess) e = back(w[4], o.xbar);
             return e;
+        case AXSPMV_OP_SPMM: {  // rows of X into B's order, Y's rows back
+            const uint64_t nk = uint64_t(p->nrows) * o.k;
+            if (!p->d_mm || p->mm_cap < 2 * nk) return hipErrorInvalidValue;  // plan_op_prepare first
+            io.k = o.k;
+            io.X = p->d_mm;
+            io.Y = p->d_mm + nk;
+            e = launch::gather_rows(p->d_perm, p->nrows, o.k, o.X, p->d_mm, s);
+            if (e == hipSuccess) e = plan_op(p->inner, op, io, s);
+            if (e == hipSuccess) e = launch::gather_rows(p->d_iperm, p->nrows, o.k, p->d_mm + nk, o.Y, s);
+            return e;
+        }
+        case AXSPMV_OP_XREFRESH:
+            io.x = w[0];
+            e = in(o.x, w[0]);
+            if (e == hipSuccess) e = plan_op(p->inner, op, io, s);
+            return e;
         default: return hipErrorInvalidValue;
     }
 }
@@ -1397,10 +1418,9 @@ hipError_t plan_op(Plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s
 int plan_op_check(const Plan *p, uint32_t op, const axspmv_operands &o) {
     if (!p) return fail(AXSPMV_E_INVALID, "NULL plan");
     if (p->kind == Plan::REORDERED) {
-        if (op == AXSPMV_OP_SPMM || op == AXSPMV_OP_XREFRESH)
-            return fail(AXSPMV_E_UNSUPPORTED, "%s: plan %s was created with AXSPMV_PLAN_REORDER",
-                        op == AXSPMV_OP_SPMM ? "SpMM" : "x refresh", p->name);
         if (op > AXSPMV_OP_XREFRESH) return fail(AXSPMV_E_INVALID, "unknown operation %u", op);
+        if (op == AXSPMV_OP_SPMM && uint64_t(p->nrows) * o.k >= (1ull << 40))
+            return fail(AXSPMV_E_OVERFLOW, "SpMM: %u rows x k %u", p->nrows, o.k);
         if (op == AXSPMV_OP_SPMV && ((!o.x && p->ncols) || (!o.y && p->nrows)))
             return fail(AXSPMV_E_INVALID, "spmv: NULL argument");
         return plan_op_check(p->inner, op, o);  // the operation's own conditions, on the plan of B
@@ -1434,6 +1454,19 @@ int plan_op_check(const Plan *p, uint32_t op, const axspmv_operands &o) {
         if (!o.x && p->ncols) return fail(AXSPMV_E_INVALID, "x refresh: NULL x");
     }
     if (op > AXSPMV_OP_XREFRESH) return fail(AXSPMV_E_INVALID, "unknown operation %u", op);
+    return AXSPMV_OK;
+}
+
+int plan_op_prepare(Plan *p, uint32_t op, const axspmv_operands &o) {
+    if (p->kind != Plan::REORDERED || op != AXSPMV_OP_SPMM) return AXSPMV_OK;
+    const uint64_t need = 2ull * p->nrows * o.k;
+    if (p->d_mm && p->mm_cap >= need) return AXSPMV_OK;
+    // A smaller buffer stays allocated until the plan goes: a graph captured
+    // with it may still run.
+    void *m = p->dalloc(8 * std::max<uint64_t>(need, 1));
+    if (!m) return AXSPMV_E_NOMEM;
+    p->d_mm = static_cast<double *>(m);
+    p->mm_cap = need;
     return AXSPMV_OK;
 }
 
@@ -1479,6 +1512,7 @@ extern "C" int axspmv_spmm(axspmv_plan *p, uint32_t k, const double *d_X, double
     o.X = d_X;
     o.Y = d_Y;
     AX_TRY(plan_op_check(p, AXSPMV_OP_SPMM, o));
+    AX_TRY(plan_op_prepare(p, AXSPMV_OP_SPMM, o));
     const hipError_t e = plan_op(p, AXSPMV_OP_SPMM, o, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? AXSPMV_OK : hip_fail(e, "spmm launch");
 }

@@ -82,6 +82,11 @@ struct axspmv_plan {
         uint32_t *w1 = nullptr;
         uint32_t L = 0;
     } to_b, from_b;
+    uint32_t *d_perm = nullptr, *d_iperm = nullptr;  // SpMM: rows of X into B's order, rows of Y back
+    // SpMM operands in B's order (2 x nrows x k doubles, grown by
+    // plan_op_prepare before a call needs them)
+    double *d_mm = nullptr;
+    uint64_t mm_cap = 0;  // doubles
     double *d_scr[6] = {};
     uint32_t nscratch = 0;
     std::vector<uint32_t> bpos;
@@ -99,6 +104,10 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
 // One operation of enum axspmv_op; plan_op_check validates it first.
 hipError_t plan_op(axspmv_plan *p, uint32_t op, const axspmv_operands &o, hipStream_t s);
 int plan_op_check(const axspmv_plan *p, uint32_t op, const axspmv_operands &o);
+// Device memory an operation needs beyond what the plan holds (a reordered
+// plan's SpMM operands for this k); call after plan_op_check and before any
+// launch or capture of the operation.
+int plan_op_prepare(axspmv_plan *p, uint32_t op, const axspmv_operands &o);
 int plan_add_transpose(axspmv_plan *p, const axspmv_csr *a, hipStream_t s);
 // Times launches of `fn` (see axspmv_time_op for the modes).
 int time_launches(const std::function<hipError_t(hipStream_t)> &fn, uint32_t iters, uint32_t warmup, uint32_t mode,

@@ -346,13 +346,16 @@ typedef struct axspmv_opts {
                                     banded matrices) the plan is built for A
                                     as without the flag (its name then has
                                     no "_rcm").  The plan holds B and
-                                    two (with AXSPMV_PLAN_TRANSPOSE five)
-                                    vectors of nrows doubles.  SpMV, the
-                                    adjoint, v_bar and forward + adjoint;
-                                    SpMM and the x refresh return
-                                    AXSPMV_E_UNSUPPORTED.  Building Q costs
-                                    a few breadth-first passes over A
-                                    (host, one thread). */
+                                    three (with AXSPMV_PLAN_TRANSPOSE six)
+                                    vectors of nrows doubles; SpMM moves
+                                    X's and Y's rows through 2 x nrows x k
+                                    doubles the plan allocates at the first
+                                    call with that k or more (a smaller
+                                    earlier buffer is kept until the plan
+                                    is destroyed, for graphs that use it).
+                                    Every operation is available.  Building
+                                    Q costs a few breadth-first passes over
+                                    A on the host. */
 #define AXSPMV_PLAN_AX_LAYOUT 4u /* AXT UNC th == 1 or AXC, FP64 values: keep
                                     the reference's interleaved ax array on the
                                     device (tiles of thw values then thw

@@ -121,12 +121,53 @@ def test_reordered_graph_replays_and_timing():
     assert rel_err(y_ref, dy.download(np.float64, m.nrows)) <= TOL
 
 
+SPMM_KERNELS = [
+    ("csr_vector", ax.FMT_CSR, dict(kernel=ax.KER_STREAM)),
+    ("unc_h1_thw4", ax.FMT_AXT, dict(thw=4, th=1)),
+    ("rct_r1024_s1", ax.FMT_RCT, dict(tile_rows=1024, slices=1)),
+    ("rct_r512_s2", ax.FMT_RCT, dict(tile_rows=512, slices=2)),
+]
+
+
+@pytest.mark.parametrize("mname", ["stencil20p", "banded_empty_shuffled"])
+@pytest.mark.parametrize("kname,fmt,kw", SPMM_KERNELS, ids=[k[0] for k in SPMM_KERNELS])
+def test_reordered_spmm(mname, kname, fmt, kw):
+    m, _ = MATS[mname]
+    p = ax.Plan(m, fmt, flags=ax.PLAN_REORDER, **kw)
+    assert p.info()["name"].endswith("_rcm")
+    # k = 8 after k = 24: the larger operand buffer serves the smaller k
+    for k in (24, 8, 40):
+        X = ax.uniform(41 + k, m.ncols * k).reshape(m.ncols, k)
+        dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer.from_array(np.full(m.nrows * k, np.nan))
+        p.spmm(k, dX, dY)
+        Y = dY.download(np.float64, m.nrows * k).reshape(m.nrows, k)
+        assert rel_err(orc.spmm(m, X), Y) <= TOL, (mname, kname, k)
+
+
+def test_reordered_spmm_graph_and_x_refresh():
+    m, _ = MATS["stencil20p"]
+    p = ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_REORDER)
+    k = 16
+    X = ax.uniform(51, m.ncols * k).reshape(m.ncols, k)
+    dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * m.nrows * k)
+    g = ax.Graph.of_plan(p, ax._lib.OP_SPMM, repeats=2, k=k, X=dX, Y=dY)
+    ref = orc.spmm(m, X)
+    for _ in range(3):
+        dY.upload(np.full(m.nrows * k, np.nan))
+        g.launch()
+        assert rel_err(ref, dY.download(np.float64, m.nrows * k).reshape(m.nrows, k)) <= TOL
+        p.spmm(32, ax.DeviceBuffer(8 * m.ncols * 32), ax.DeviceBuffer(8 * m.nrows * 32))  # grows the buffer
+    # the AX layout's x refresh through a reordered plan, then its SpMV
+    pa = ax.Plan(m, ax.FMT_AXT, thw=4, flags=ax.PLAN_REORDER | ax.PLAN_AX_LAYOUT)
+    x = ax.uniform(52, m.ncols)
+    dx = ax.DeviceBuffer.from_array(x)
+    t = pa.time_op(ax._lib.OP_XREFRESH, iters=3, warmup=1, x=dx)
+    assert t.mean_ms > 0
+    assert rel_err(orc.ncsr(m, x), ax.spmv_host(pa, x)) <= TOL
+
+
 def test_reordered_rejects():
     m, _ = MATS["stencil20p"]
-    p = ax.Plan(m, ax.FMT_RCT, flags=ax.PLAN_REORDER)
-    dX, dY = ax.DeviceBuffer(8 * m.ncols * 8), ax.DeviceBuffer(8 * m.nrows * 8)
-    with pytest.raises(ax.AxspmvError):
-        p.spmm(8, dX, dY)
     rect = ax.CsrMatrix(np.array([0, 1, 2], np.uint32), np.array([0, 2], np.uint32), np.ones(2), ncols=3)
     with pytest.raises(ax.AxspmvError):
         ax.Plan(rect, ax.FMT_CSR, flags=ax.PLAN_REORDER)
