@@ -17,7 +17,7 @@
  * the reference computes them (get_errors compares |y| with |y_ref|,
  * cudaSpmv.cu:474-495) and signed (|y - y_ref|).
  *
- *   axspmv_main MATRIX [-b CBS] [-t THREADS] [-i NUM_ITE] [-x index|SEED] [-s]
+ *   axspmv_main MATRIX [-b CBS] [-t THREADS] [-i NUM_ITE] [-x index|SEED] [-s] [-r]
  *
  * MATRIX is a .bin or .csr file (the reference's two layouts), or a
  * generator: laplace2d:GRID, powerlaw:N, stencil27:GRID (permuted, seed 2),
@@ -26,7 +26,9 @@
  * -t the OpenMP threads of the CPU product, -i NUM_ITE (default 250, as the
  * reference), -x the vector: "index" = x_i = i (the reference's init_vec),
  * otherwise U[-1, 1] from that seed (default 1), -s writes MATRIX's .sta
- * statistics file (matrix files only).
+ * statistics file (matrix files only), -r adds rows for RCT and AXT UNC th
+ * = 1 (thw 4) plans renumbered by reverse Cuthill-McKee inside the plan
+ * (AXSPMV_PLAN_REORDER; square matrices; no reference counterpart).
  */
 #include <math.h>
 #include <omp.h>
@@ -123,12 +125,12 @@ typedef struct {
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        fprintf(stderr, "usage: %s MATRIX [-b CBS] [-t THREADS] [-i NUM_ITE] [-x index|SEED] [-s]\n", argv[0]);
+        fprintf(stderr, "usage: %s MATRIX [-b CBS] [-t THREADS] [-i NUM_ITE] [-x index|SEED] [-s] [-r]\n", argv[0]);
         return EXIT_FAILURE;
     }
     const char *spec = argv[1];
     uint32_t cbs = 256, iters = 250;
-    int threads = omp_get_max_threads(), xindex = 0, sta = 0;
+    int threads = omp_get_max_threads(), xindex = 0, sta = 0, reorder = 0;
     uint64_t xseed = 1;
     for (int i = 2; i < argc; ++i) {
         if (!strcmp(argv[i], "-b") && i + 1 < argc)
@@ -145,6 +147,8 @@ int main(int argc, char **argv) {
                 xseed = strtoull(argv[i], NULL, 10);
         } else if (!strcmp(argv[i], "-s"))
             sta = 1;
+        else if (!strcmp(argv[i], "-r"))
+            reorder = 1;
         else {
             fprintf(stderr, "axspmv_main: unknown argument %s\n", argv[i]);
             return EXIT_FAILURE;
@@ -286,9 +290,12 @@ int main(int argc, char **argv) {
         {"gaxtch8hw64", AXSPMV_FMT_AXT, 0, AXSPMV_AXT_COM, 64, 8, 0, 0, 0, 0, 0},
         {"gaxtch16hw64", AXSPMV_FMT_AXT, 0, AXSPMV_AXT_COM, 64, 16, 0, 0, 0, 0, 0},
         {"rct (default tiles)", AXSPMV_FMT_RCT, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+        {"rct, renumbered", AXSPMV_FMT_RCT, 0, 0, 0, 0, 0, 0, 0, 0, AXSPMV_PLAN_REORDER},
+        {"gaxtuh1hw04, renumbered", AXSPMV_FMT_AXT, 0, AXSPMV_AXT_UNC, 4, 1, 0, 0, 0, 0, AXSPMV_PLAN_REORDER},
     };
     for (size_t i = 0; i < sizeof ks / sizeof ks[0]; ++i) {
         const kern_t *k = &ks[i];
+        if ((k->flags & AXSPMV_PLAN_REORDER) && !reorder) continue;
         axspmv_opts o;
         CHECK(axspmv_opts_default(k->format, &o));
         o.kernel = k->kernel;

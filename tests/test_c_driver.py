@@ -68,23 +68,30 @@ def parse_tables(out):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("spec", ["laplace2d:64", "random:3001:0:40", "file"])
+@pytest.mark.parametrize("spec", ["laplace2d:64", "random:3001:0:40", "file", "stencil27:12 -r"])
 def test_main_driver_tables(tmp_path, spec):
     """axspmv_main on a generated matrix and on a .bin file: every format's
     row (mfp, beta, conversion times) and every kernel's row, each kernel
-    within 1e-12 (signed) of the CPU CSR product."""
+    within 1e-12 (signed) of the CPU CSR product; with -r also the
+    renumbered RCT and AXT plans."""
     import gpu_spmv_amd as ax
+    extra = 0
     if spec == "file":
         m = ax.powerlaw(5000, 2.0, 256, 12.0, seed=4)
         spec = str(tmp_path / "plaw.bin")
         m.write(spec)
         args = [DRIVER, spec, "-i", "20", "-x", "index", "-s"]
+    elif spec.endswith(" -r"):
+        args = [DRIVER, spec.split()[0], "-i", "20", "-r"]
+        extra = 1
     else:
         args = [DRIVER, spec, "-i", "20", "-b", "128"]
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout
     fm, km = parse_tables(r.stdout)
-    assert len(fm) == 16 and len(km) == 21, (fm.keys(), km.keys())
+    assert len(fm) == 16 + extra and len(km) == 21 + 2 * extra, (fm.keys(), km.keys())
+    if extra:
+        assert sum("_rcm]" in k for k in km) == 2, km.keys()
     assert any(k.startswith("faxtch1hw64bs") for k in fm)
     for name, row in km.items():
         et, gf, ot, ea, er, pos, ser = row
