@@ -44,6 +44,8 @@ def mats():
     yield "laplace64_shuffled", shuffled(ax.laplace2d(64), 5), True
     yield "banded_empty_shuffled", shuffled(ax.banded(5000, 3, 0.3, seed=6), 7), True
     yield "powerlaw16k", ax.powerlaw(1 << 14, 2.0, 1024, 20.0, seed=1), False
+    yield "stencil2p_tiny", ax.stencil27(2, permute=True, seed=1), False  # 8 rows, all adjacent: nothing to gain
+    yield "one_row", ax.CsrMatrix(np.array([0, 1], np.uint32), np.array([0], np.uint32), np.array([3.0])), True
 
 
 MATS = {name: (m, applies) for name, m, applies in mats()}
