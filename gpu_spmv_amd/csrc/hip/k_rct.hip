@@ -42,7 +42,11 @@ using axspmv::launch::RctArgs;
 // entries has a base column (its first entry's), and its entries, sorted by
 // column tile by tile, lie a short way past it -- going round from the last
 // column to column 0 where a run passes from one row block to the next.
-template <class VT, bool PACKED>
+// NT: the paired stream loads of load2 are non-temporal (the plan sets it
+// when the matrix stream is too large to stay in the Infinity Cache from one
+// SpMV to the next; there the L2 lines it would leave behind only crowd out
+// x).
+template <class VT, bool PACKED, bool NT = false>
 struct RctEntries {
     const VT *__restrict__ val;
     const uint32_t *__restrict__ key;
@@ -61,25 +65,33 @@ struct RctEntries {
     // entries 2q and 2q + 1 with one 16-byte (8 for FP32 values) and one
     // 8-byte load; the plan pads the arrays to an even number of entries
     __device__ __forceinline__ void load2(uint32_t q, uint32_t (&c)[2], uint32_t (&r)[2], double (&v)[2]) const {
+        typedef double f64x2 __attribute__((ext_vector_type(2)));
+        typedef float f32x2 __attribute__((ext_vector_type(2)));
+        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
         if constexpr (sizeof(VT) == 8) {
-            const double2 vv = reinterpret_cast<const double2 *>(val)[q];
+            const f64x2 *p = reinterpret_cast<const f64x2 *>(val) + q;
+            const f64x2 vv = NT ? __builtin_nontemporal_load(p) : *p;
             v[0] = vv.x;
             v[1] = vv.y;
         } else {
-            const float2 vv = reinterpret_cast<const float2 *>(val)[q];
+            const f32x2 *p = reinterpret_cast<const f32x2 *>(val) + q;
+            const f32x2 vv = NT ? __builtin_nontemporal_load(p) : *p;
             v[0] = double(vv.x);
             v[1] = double(vv.y);
         }
         if constexpr (PACKED) {
-            const uint2 k = reinterpret_cast<const uint2 *>(key)[q];
+            const u32x2 *pk = reinterpret_cast<const u32x2 *>(key) + q;
+            const u32x2 k = NT ? __builtin_nontemporal_load(pk) : *pk;
             const uint32_t m = (1u << rbits) - 1, b = sbase[q >> 9];  // entries 2q, 2q + 1: one segment
             c[0] = column(b, k.x >> rbits);
             r[0] = k.x & m;
             c[1] = column(b, k.y >> rbits);
             r[1] = k.y & m;
         } else {
-            const uint2 cc = reinterpret_cast<const uint2 *>(col)[q];
-            const uint32_t rr = reinterpret_cast<const uint32_t *>(lrow)[q];
+            const u32x2 *pc = reinterpret_cast<const u32x2 *>(col) + q;
+            const uint32_t *pr = reinterpret_cast<const uint32_t *>(lrow) + q;
+            const u32x2 cc = NT ? __builtin_nontemporal_load(pc) : *pc;
+            const uint32_t rr = NT ? __builtin_nontemporal_load(pr) : *pr;
             c[0] = cc.x;
             c[1] = cc.y;
             r[0] = rr & 0xffffu;
@@ -116,7 +128,7 @@ struct TileOf {
 // A: vb[e] = xr[row(e)] * x[col(e)] for every entry, in entry order (the
 // tile's xr values sit in LDS after its sums).  With x = r and xr = the
 // forward x, y = A^T r and vb holds r[i] * x[j] for every nonzero A(i, j).
-template <class VT, bool PACKED, bool VB>
+template <class VT, bool PACKED, bool VB, bool NT>
 __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__restrict__ x, double *__restrict__ y,
                                                    const double *__restrict__ xr, double *__restrict__ vb) {
     extern __shared__ double yl[];
@@ -133,7 +145,8 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     __syncthreads();
     const uint32_t t = rb * a.S + s;  // tile index in storage order
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.sbase, a.rbits, a.ncols};
+    const RctEntries<VT, PACKED, NT> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.sbase, a.rbits,
+                                         a.ncols};
     // Each lane takes entries in aligned pairs (2q, 2q + 1): a value load
     // moves 16 bytes and a key load 8, and the value gradient (VB) goes out
     // as one 16-byte store per pair.  Pairs pb .. pl cover the tile and each
@@ -411,12 +424,14 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
         }
     }
     const bool packed = a.key != nullptr;
-#define AX_RCT_LAUNCH(VT, P)                                                  \
-    do {                                                                      \
-        if (vb)                                                               \
-            rct_kernel<VT, P, true><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar);  \
-        else                                                                  \
-            rct_kernel<VT, P, false><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar); \
+#define AX_RCT_LAUNCH(VT, P)                                                             \
+    do {                                                                                 \
+        if (vb)                                                                          \
+            rct_kernel<VT, P, true, false><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar);  \
+        else if (a.nt_stream)                                                            \
+            rct_kernel<VT, P, false, true><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar);  \
+        else                                                                             \
+            rct_kernel<VT, P, false, false><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar); \
     } while (0)
     if (a.vtype == 1) {
         if (packed)

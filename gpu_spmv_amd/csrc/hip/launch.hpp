@@ -95,6 +95,7 @@ struct RctArgs {
     const uint32_t *rb_start;  // nrb + 1 row-block starts
     uint32_t nrows, R, S, nrb, vtype, block, rbits, ncols;
     double *part;  // S > 1: S x nrows slice partials (nullptr: atomic adds into y)
+    uint32_t nt_stream = 0;  // 1: the SpMV tile kernel streams the matrix with non-temporal loads
 };
 // S > 1 with a.part: the tile kernel stores every slice's row sums in its
 // partial vector, then part_merge adds them in slice order into y; mid

@@ -741,9 +741,23 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     // Narrow bands reuse x within a band whatever the tile height: 2048-row
     // tiles (16 KB of LDS) let several workgroups share a CU, which pays on
     // matrices mixing short and long rows (256 banded matrices as one
-    // block-diagonal plan: 1259 us against 1390 us at 16384 rows) and costs
-    // at most 2.5 % on uniform ones.
-    uint32_t Rt = p.o.tile_rows ? p.o.tile_rows : banded ? std::min<uint32_t>(2048, auto_rows(1)) : auto_rows(S);
+    // block-diagonal plan: 1259 us against 1390 us at 16384 rows, row
+    // lengths varying by 60 % of their mean).  Rows of one length (a mesh:
+    // the 128^3 stencil renumbered by reverse Cuthill-McKee, 7 %) take the
+    // tallest tiles that still cover the CUs: 168 against 180 us at 2048
+    // rows, with the non-temporal stream loads below.
+    uint32_t Rt = p.o.tile_rows ? p.o.tile_rows : auto_rows(S);
+    if (!p.o.tile_rows && banded) {
+        double sum = 0.0, sq = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : sum, sq)
+        for (uint32_t i = 0; i < n; ++i) {
+            const double l = double(a->row[i + 1] - a->row[i]);
+            sum += l;
+            sq += l * l;
+        }
+        const double mean = n ? sum / n : 0.0, var = n ? std::max(0.0, sq / n - mean * mean) : 0.0;
+        if (!(mean > 0.0 && std::sqrt(var) < 0.25 * mean)) Rt = std::min<uint32_t>(2048, auto_rows(1));
+    }
     if (p.lds_per_row > 8)  // a transposed plan's tiles are cut to fit (8192 rows when automatic)
         Rt = std::min(Rt, p.o.tile_rows || banded ? lds_rows : std::min<uint32_t>(lds_rows, 8192));
     if (Rt == 0 || Rt > lds_rows)
@@ -892,6 +906,16 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         if (!dpart) return AXSPMV_E_NOMEM;
     }
     p.rct = launch::RctArgs{dv, dc, dl, dk, dcb, dt, drb, n, R, S, nrb, p.o.vtype, block, rbits, a->ncols, dpart};
+    // A matrix stream larger than the 256 MB Infinity Cache comes from HBM
+    // on every SpMV whatever the loads do: there tall tiles (one workgroup
+    // per CU) stream it with non-temporal loads (the 128^3 stencil
+    // renumbered by reverse Cuthill-McKee, 8192-row tiles: 184 -> 168 us per
+    // SpMV; the permuted stencil's 16384-row tiles: unchanged).  A smaller
+    // stream stays cached between SpMVs, and non-temporal loads give that up
+    // (the 2^20-row power-law matrix, 252 MB: 58 -> 89 us); short tiles,
+    // several workgroups per CU, measured 1 % slower with them (256 banded
+    // matrices, 2048-row tiles: 1236 -> 1249 us).
+    p.rct.nt_stream = (vs + (packed ? 4 : 6)) * nnz > (256ull << 20) && R >= 4096 ? 1 : 0;
     p.slots = nnz;
     // matrix stream; with S > 1 the partials written and read back and y
     // written by the merge, or the clear of y and S atomic adds per row
