@@ -43,24 +43,26 @@ void build_graph(const axspmv_csr *a, Graph &g) {
     g.n = n;
     g.row = row;
     g.col = col;
-    // Structurally symmetric A (rows sorted, and every entry (i, j) has a
-    // partner (j, i), found by binary search in row j): its rows are the
-    // graph.  Otherwise the rows of A^T's pattern are added.
+    // Structurally symmetric A (every row's columns strictly increasing, and
+    // every entry (i, j) with a partner (j, i), found by binary search in
+    // row j): its rows are the graph.  Otherwise the rows of A^T's pattern
+    // are added.
     bool sym = true;
 #pragma omp parallel for schedule(dynamic, 4096) reduction(&& : sym)
-    for (uint32_t i = 0; i < n; ++i) {
-        if (!sym) continue;
-        for (uint32_t j = row[i]; j < row[i + 1]; ++j) {
-            if (j > row[i] && col[j] <= col[j - 1]) {
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = row[i] + 1; j < row[i + 1]; ++j)
+            if (col[j] <= col[j - 1]) {
                 sym = false;
                 break;
             }
-            const uint32_t c = col[j];
-            if (!std::binary_search(col + row[c], col + row[c + 1], i)) {
-                sym = false;
-                break;
-            }
-        }
+    if (sym) {
+#pragma omp parallel for schedule(dynamic, 4096) reduction(&& : sym)
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t j = row[i]; j < row[i + 1]; ++j)
+                if (!std::binary_search(col + row[col[j]], col + row[col[j] + 1], i)) {
+                    sym = false;
+                    break;
+                }
     }
     if (!sym) {
         // A^T's pattern, rows in no particular order (the numbering sorts

@@ -56,7 +56,16 @@ def two_paths_and_isolated():
     return ax.CsrMatrix(row, col, val, ncols=550)
 
 
+def unsorted_rows(m, seed):
+    """m with the entries of every row in a random order (the products sum
+    them in any order; the graph is then built through A^T's pattern)."""
+    rng = np.random.default_rng(seed)
+    idx = np.concatenate([m.row[i] + rng.permutation(m.row[i + 1] - m.row[i]) for i in range(m.nrows)])
+    return ax.CsrMatrix(m.row, m.col[idx], m.val[idx], ncols=m.ncols)
+
+
 MATS = {
+    "stencil12p_unsorted": lambda: unsorted_rows(ax.stencil27(12, permute=True, seed=2), 9),
     "stencil12p": lambda: ax.stencil27(12, permute=True, seed=2),
     "stencil16p": lambda: ax.stencil27(16, permute=True, seed=5),
     "laplace40_shuffled": lambda: shuffled(ax.laplace2d(40), 3),
@@ -80,6 +89,7 @@ def test_rcm_is_a_permutation(name):
 @pytest.mark.parametrize("name,exact", [("stencil12p", True), ("stencil16p", True), ("laplace40_shuffled", True),
                                         ("path1000", True), ("two_paths_isolated", True),
                                         ("random_nonsym", False), ("banded_empty_rows", False),
+                                        ("stencil12p_unsorted", False),
                                         ("powerlaw4k", False)])
 def test_rcm_bandwidth_against_scipy(name, exact):
     m = MATS[name]()
