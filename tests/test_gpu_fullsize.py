@@ -4,6 +4,8 @@ matrix of 2^20 rows is covered in test_gpu_spmv.py).
 * 27-point stencil on a 128^3 grid, symmetric random permutation (seed 2):
   n = 2,097,152, nnz = 55,742,968 (the nonzero count is a closed form of the
   grid, checked first); every format against the oracle's ncsr, 1e-12.
+* The same with AXSPMV_PLAN_REORDER (RCT, AXT UNC th=1, K1): x and y in the
+  matrix's own numbering, the format's kernels on the renumbered matrix.
 * Forward + adjoint on the same structure with values N(1, 0.1) (seed 4),
   b ~ U[-1, 1]: y, r = y - b, x_bar = A^T r against the oracle (1e-12 /
   1e-11), v_bar = r[row] * x[col] bit for bit against the oracle's product of
@@ -52,6 +54,10 @@ FORMATS = [
     ("axc_hbs4", ax.FMT_AXC, dict(hbs=4)),
     ("rct_auto", ax.FMT_RCT, dict()),
     ("rct_r8192_s1_split", ax.FMT_RCT, dict(tile_rows=8192, slices=1, flags=ax.PLAN_RCT_SPLIT)),
+    # renumbered by reverse Cuthill-McKee inside the plan
+    ("rct_auto_rcm", ax.FMT_RCT, dict(flags=ax.PLAN_REORDER)),
+    ("axt_unc_h1_thw4_rcm", ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_REORDER)),
+    ("k1_c64_rcm", ax.FMT_K1, dict(chunk=64, flags=ax.PLAN_REORDER)),
 ]
 
 
@@ -65,14 +71,18 @@ def test_stencil_config3_full_size(stencil, name, fmt, kw):
 
 
 @pytest.mark.timeout(300)
-@pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_RCT, dict())], ids=["axt", "rct"])
+@pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_RCT, dict()),
+                                    (ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_REORDER)),
+                                    (ax.FMT_RCT, dict(flags=ax.PLAN_REORDER))],
+                         ids=["axt", "rct", "axt_rcm", "rct_rcm"])
 def test_fwd_adj_config5_full_size(stencil, fmt, kw):
     s, x, _ = stencil
     m = ax.CsrMatrix(s.row, s.col, s.val.copy(), ncols=s.ncols)
     ax.fill_values_normal(m, 1.0, 0.1, 4)
     b = ax.uniform(4, m.nrows, stream=ax.VEC_STREAM + 1)
     n, nc, nnz = m.nrows, m.ncols, m.nnz
-    p = ax.Plan(m, fmt, flags=ax.PLAN_TRANSPOSE, **kw)
+    kw = dict(kw)
+    p = ax.Plan(m, fmt, flags=ax.PLAN_TRANSPOSE | kw.pop("flags", 0), **kw)
     dx, db = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer.from_array(b)
     outs = [ax.DeviceBuffer(8 * k) for k in (n, n, nc, nnz)]
     p.fwd_adj(dx, db, *outs, fused=True)
