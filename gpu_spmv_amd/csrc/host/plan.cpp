@@ -1383,18 +1383,43 @@ static hipError_t reordered_op(Plan *p, uint32_t op, const axspmv_operands &o, h
             return e;
         case AXSPMV_OP_FWD_ADJ:
         case AXSPMV_OP_FWD_ADJ_SPLIT:
-            io.x = w[0];
-            io.b = w[1];
-            io.y = w[2];
-            io.r = w[3];
-            io.xbar = w[4];
-            io.vbar = o.vbar;
+            if (op == AXSPMV_OP_FWD_ADJ && p->inner->kind != Plan::RCT) {
+                // the format's fused sweep takes b: x and b in, y, r and x_bar out
+                io.x = w[0];
+                io.b = w[1];
+                io.y = w[2];
+                io.r = w[3];
+                io.xbar = w[4];
+                io.vbar = o.vbar;
+                e = in(o.x, w[0]);
+                if (e == hipSuccess) e = in(o.b, w[1]);
+                if (e == hipSuccess) e = plan_op(p->inner, op, io, s);
+                if (e == hipSuccess) e = back(w[2], o.y);
+                if (e == hipSuccess) e = back(w[3], o.r);
+                if (e == hipSuccess) e = back(w[4], o.xbar);
+                return e;
+            }
+            // four permutations instead of five: y comes back, r = y - b in
+            // A's numbering, r goes in; then x_bar and v_bar from B's plan of
+            // B^T (RCT: one sweep; otherwise the adjoint and the value
+            // gradient), x_bar back
             e = in(o.x, w[0]);
-            if (e == hipSuccess) e = in(o.b, w[1]);
-            if (e == hipSuccess) e = plan_op(p->inner, op, io, s);
-            if (e == hipSuccess) e = back(w[2], o.y);
-            if (e == hipSuccess) e = back(w[3], o.r);
-            if (e == hipSuccess) e = back(w[4], o.xbar);
+            if (e == hipSuccess) e = plan_launch(p->inner, w[0], w[1], s);
+            if (e == hipSuccess) e = back(w[1], o.y);
+            if (e == hipSuccess) e = launch::residual(p->nrows, o.y, o.b, o.r, s);
+            if (e == hipSuccess) e = in(o.r, w[2]);
+            if (e != hipSuccess) return e;
+            if (op == AXSPMV_OP_FWD_ADJ) {  // RCT: x_bar = B^T r~ and v_bar in one sweep over the plan of B^T
+                e = launch::rct(p->inner->T->rct, w[2], w[3], s, nullptr, w[0], o.vbar);
+            } else {
+                io.ybar = w[2];
+                io.xbar = w[3];
+                e = plan_op(p->inner, AXSPMV_OP_ADJOINT, io, s);
+                io.x = w[0];
+                io.vbar = o.vbar;
+                if (e == hipSuccess) e = plan_op(p->inner, AXSPMV_OP_VBAR, io, s);
+            }
+            if (e == hipSuccess) e = back(w[3], o.xbar);
             return e;
         case AXSPMV_OP_SPMM: {  // rows of X into B's order, Y's rows back
             const uint64_t nk = uint64_t(p->nrows) * o.k;
