@@ -418,23 +418,26 @@ def conversion(m, x, dx):
     return out
 
 
-def calibrate():
-    """Copy bandwidth, and the flagship AXT kernel on the power-law structure
-    with columns folded into a smaller x (footprint sweep): the gather bound."""
-    res = {}
-    n = 32 << 20  # 256 MB of doubles
+def copy_bandwidth(reps: int = 20) -> dict:
+    """A 256 MiB device-to-device copy (copy_kernel), host-clocked over reps
+    back-to-back launches: the read + write bandwidth this box reaches."""
+    n = 32 << 20  # 256 MiB of doubles
     src, dst = ax.DeviceBuffer(8 * n), ax.DeviceBuffer(8 * n)
     for _ in range(3):
         L.call("axspmv_copy_f64", src.ptr, dst.ptr, n, None)
     L.call("axspmv_device_sync")
     t0 = time.perf_counter()
-    reps = 20
     for _ in range(reps):
         L.call("axspmv_copy_f64", src.ptr, dst.ptr, n, None)
     L.call("axspmv_device_sync")
     dt = (time.perf_counter() - t0) / reps
-    res["copy_256MB"] = dict(us=round(dt * 1e6, 1), gbs_read_plus_write=round(16.0 * n / dt / 1e9, 1))
-    del src, dst
+    return dict(us=round(dt * 1e6, 1), gbs_read_plus_write=round(16.0 * n / dt / 1e9, 1))
+
+
+def calibrate():
+    """Copy bandwidth, and the flagship AXT kernel on the power-law structure
+    with columns folded into a smaller x (footprint sweep): the gather bound."""
+    res = {"copy_256MB": copy_bandwidth()}
     m = ax.powerlaw(1 << 20, 2.0, 2048, 20.0, seed=1)
     sweep = []
     for fold in (1, 2, 4, 8, 64):
@@ -698,6 +701,8 @@ def main():
         return
 
     t_start = time.perf_counter()
+    copy = copy_bandwidth()
+    log(f"copy: {copy}")
     m, x = W.config2()
     log(f"matrix: n={m.nrows} nnz={m.nnz} ({time.perf_counter() - t_start:.1f} s)")
 
@@ -799,7 +804,8 @@ def main():
                      "kernel": traffic_kernel or info["name"], "kernel_us": round(kern_s * 1e6, 2),
                      "kernel_frac": round(info["alg_bytes"] / kern_s / 1e9 / PEAK_HBM_GBS, 4),
                      "alg_bytes_per_spmv": info["alg_bytes"], "launches_per_spmv": info["launches"],
-                     "cold_caches": cold, "traffic_source": traffic_src},
+                     "cold_caches": cold, "traffic_source": traffic_src,
+                     "copy_same_box": dict(copy, frac_of_copy=round(achieved / copy["gbs_read_plus_write"], 4))},
         "cpu_baseline": cpu,
         "formats": formats,
         "tables": tables,
