@@ -168,6 +168,35 @@ def test_reordered_spmm_graph_and_x_refresh():
     assert rel_err(orc.ncsr(m, x), ax.spmv_host(pa, x)) <= TOL
 
 
+COMBOS = [
+    ("rct_f32val", ax.FMT_RCT, dict(vtype=ax.F32)),
+    ("unc_h1_thw4_f32val", ax.FMT_AXT, dict(thw=4, vtype=ax.F32)),
+    ("k1_c64_f32val", ax.FMT_K1, dict(chunk=64, vtype=ax.F32)),
+    ("csr_vector_f32val", ax.FMT_CSR, dict(kernel=ax.KER_STREAM, vtype=ax.F32)),
+    ("unc_h1_thw4_stage_x", ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_STAGE_X)),
+    ("unc_h1_thw2_s4_det", ax.FMT_AXT, dict(thw=2, slices=4, flags=ax.PLAN_DETERMINISTIC)),
+    ("rct_r256_split", ax.FMT_RCT, dict(tile_rows=256, flags=ax.PLAN_RCT_SPLIT)),
+]
+
+
+@pytest.mark.parametrize("kname,fmt,kw", COMBOS, ids=[k[0] for k in COMBOS])
+def test_reordered_with_other_options(kname, fmt, kw):
+    """The flag combines with FP32 values (against the oracle on the
+    float-rounded values), x staging, deterministic slices (bit-identical
+    over repeated calls) and split RCT keys."""
+    m, _ = MATS["stencil20p"]
+    kw = dict(kw)
+    p = ax.Plan(m, fmt, flags=ax.PLAN_REORDER | kw.pop("flags", 0), **kw)
+    assert p.info()["name"].endswith("_rcm"), p.info()["name"]
+    x = ax.uniform(61, m.ncols)
+    ref = orc.ncsr_f32val(m, x) if kw.get("vtype") == ax.F32 else orc.ncsr(m, x)
+    ys = [ax.spmv_host(p, x) for _ in range(3)]
+    for y in ys:
+        assert rel_err(ref, y) <= TOL, kname
+    if "det" in kname:
+        assert all(np.array_equal(ys[0], y) for y in ys[1:])
+
+
 def test_reordered_rejects():
     m, _ = MATS["stencil20p"]
     rect = ax.CsrMatrix(np.array([0, 1, 2], np.uint32), np.array([0, 2], np.uint32), np.ones(2), ncols=3)
