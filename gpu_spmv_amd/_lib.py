@@ -42,6 +42,7 @@ PLAN_STAGE_X = 8
 PLAN_DETERMINISTIC = 16
 PLAN_RCT_ATOMIC = 32
 PLAN_REORDER = 64
+PLAN_RCT_WIDE_KEY = 128
 
 
 class Csr(C.Structure):

@@ -36,34 +36,54 @@
 namespace {
 using axspmv::launch::RctArgs;
 
-// Entry e of a tile: value val[e]; column and tile row either packed in one
-// word key[e] = offset << rbits | row (PACKED) or in col[] / lrow[].  The
-// packed column is (sbase[e >> 10] + offset) mod ncols: every run of 1024
-// entries has a base column (its first entry's), and its entries, sorted by
-// column tile by tile, lie a short way past it -- going round from the last
-// column to column 0 where a run passes from one row block to the next.
+// Entry e of a tile: value val[e]; its column and tile row stored one of
+// three ways (KEY):
+//  * KEY_SPLIT: col[e] and lrow[e];
+//  * KEY_PACKED: one word key[e] = offset << rbits | row, the column being
+//    (sbase[e >> 10] + offset) mod ncols: every run of 1024 entries has a
+//    base column (its first entry's or its smallest), and its entries,
+//    sorted by column tile by tile, lie a short way past it -- going round
+//    from the last column to column 0 where a run passes from one row block
+//    to the next;
+//  * KEY_DIAG (narrow bands): a 16-bit key16[e] = d << cbits | offset, the
+//    column being sbase[e >> 7] + offset (the smallest column of its run of
+//    128 entries) and the row (global) column + d + dmin, where d + dmin is
+//    the entry's row minus its column; the tile row is that row minus the
+//    tile's first row (rofs = dmin - row0, all modulo 2^32).
 // NT: the paired stream loads of load2 are non-temporal (the plan sets it
 // when the matrix stream is too large to stay in the Infinity Cache from one
 // SpMV to the next; there the L2 lines it would leave behind only crowd out
 // x).
-template <class VT, bool PACKED, bool NT = false>
+enum : int { KEY_SPLIT = 0, KEY_PACKED = 1, KEY_DIAG = 2 };
+
+template <class VT, int KEY, bool NT = false>
 struct RctEntries {
     const VT *__restrict__ val;
     const uint32_t *__restrict__ key;
     const uint32_t *__restrict__ col;
     const uint16_t *__restrict__ lrow;
+    const uint16_t *__restrict__ key16;
     const uint32_t *__restrict__ sbase;
-    uint32_t rbits, ncols;
+    uint32_t rbits, ncols, cbits, rofs;
+    // the tile whose rows start at row0
+    __device__ __forceinline__ RctEntries(const RctArgs &a, uint32_t row0)
+        : val(static_cast<const VT *>(a.val)), key(a.key), col(a.col), lrow(a.lrow), key16(a.key16),
+          sbase(a.sbase), rbits(a.rbits), ncols(a.ncols), cbits(a.cbits16), rofs(a.dmin - row0) {}
     __device__ __forceinline__ uint32_t column(uint32_t base, uint32_t off) const {
         const uint32_t room = ncols - base;  // columns from base to the end
         return off >= room ? off - room : base + off;
+    }
+    __device__ __forceinline__ void diag(uint32_t base, uint32_t k, uint32_t &c, uint32_t &r) const {
+        c = base + (k & ((1u << cbits) - 1));
+        r = c + (k >> cbits) + rofs;
     }
     __device__ __forceinline__ void load(uint32_t e, uint32_t &c, uint32_t &r, double &v) const {
         v = double(val[e]);
         at(e, c, r);
     }
     // entries 2q and 2q + 1 with one 16-byte (8 for FP32 values) and one
-    // 8-byte load; the plan pads the arrays to an even number of entries
+    // 8-byte (KEY_DIAG: 4-byte) load; the plan pads the arrays to an even
+    // number of entries
     __device__ __forceinline__ void load2(uint32_t q, uint32_t (&c)[2], uint32_t (&r)[2], double (&v)[2]) const {
         typedef double f64x2 __attribute__((ext_vector_type(2)));
         typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -79,7 +99,13 @@ struct RctEntries {
             v[0] = double(vv.x);
             v[1] = double(vv.y);
         }
-        if constexpr (PACKED) {
+        if constexpr (KEY == KEY_DIAG) {
+            const uint32_t *pk = reinterpret_cast<const uint32_t *>(key16) + q;
+            const uint32_t k = NT ? __builtin_nontemporal_load(pk) : *pk;
+            const uint32_t b = sbase[q >> 6];  // entries 2q, 2q + 1: one run of 128
+            diag(b, k & 0xffffu, c[0], r[0]);
+            diag(b, k >> 16, c[1], r[1]);
+        } else if constexpr (KEY == KEY_PACKED) {
             const u32x2 *pk = reinterpret_cast<const u32x2 *>(key) + q;
             const u32x2 k = NT ? __builtin_nontemporal_load(pk) : *pk;
             const uint32_t m = (1u << rbits) - 1, b = sbase[q >> 9];  // entries 2q, 2q + 1: one segment
@@ -100,7 +126,9 @@ struct RctEntries {
     }
     // column and tile row only
     __device__ __forceinline__ void at(uint32_t e, uint32_t &c, uint32_t &r) const {
-        if constexpr (PACKED) {
+        if constexpr (KEY == KEY_DIAG) {
+            diag(sbase[e >> 7], key16[e], c, r);
+        } else if constexpr (KEY == KEY_PACKED) {
             const uint32_t k = key[e];
             c = column(sbase[e >> 10], k >> rbits);
             r = k & ((1u << rbits) - 1);
@@ -110,6 +138,8 @@ struct RctEntries {
         }
     }
 };
+
+int key_mode(const RctArgs &a) { return a.key16 ? KEY_DIAG : a.key ? KEY_PACKED : KEY_SPLIT; }
 
 // Tile w of the grid: XCD w % 8 takes slice (w % 8) % S; false for the
 // workgroups past the last row block.
@@ -128,7 +158,7 @@ struct TileOf {
 // A: vb[e] = xr[row(e)] * x[col(e)] for every entry, in entry order (the
 // tile's xr values sit in LDS after its sums).  With x = r and xr = the
 // forward x, y = A^T r and vb holds r[i] * x[j] for every nonzero A(i, j).
-template <class VT, bool PACKED, bool VB, bool NT>
+template <class VT, int KEY, bool VB, bool NT>
 __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__restrict__ x, double *__restrict__ y,
                                                    const double *__restrict__ xr, double *__restrict__ vb) {
     extern __shared__ double yl[];
@@ -145,8 +175,7 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     __syncthreads();
     const uint32_t t = rb * a.S + s;  // tile index in storage order
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const RctEntries<VT, PACKED, NT> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.sbase, a.rbits,
-                                         a.ncols};
+    const RctEntries<VT, KEY, NT> ent(a, row0);
     // Each lane takes entries in aligned pairs (2q, 2q + 1): a value load
     // moves 16 bytes and a key load 8, and the value gradient (VB) goes out
     // as one 16-byte store per pair.  Pairs pb .. pl cover the tile and each
@@ -274,7 +303,7 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
 
 // Value gradient in entry order: vbar[e] = ybar[row(e)] * x[col(e)].  The
 // tile's ybar rows sit in LDS; x is gathered in the tile's column order.
-template <bool PACKED>
+template <int KEY>
 __global__ void __launch_bounds__(1024) rct_vbar_kernel(RctArgs a, const double *__restrict__ ybar,
                                                         const double *__restrict__ x, double *__restrict__ vbar) {
     extern __shared__ double yb[];
@@ -286,7 +315,7 @@ __global__ void __launch_bounds__(1024) rct_vbar_kernel(RctArgs a, const double 
     __syncthreads();
     const uint32_t t = tile.rb * a.S + tile.s;
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const RctEntries<double, PACKED> ent{nullptr, a.key, a.col, a.lrow, a.sbase, a.rbits, a.ncols};
+    const RctEntries<double, KEY> ent(a, row0);  // columns and rows only
     constexpr int U = 4;
     uint32_t e = beg + tid;
     for (; e + (U - 1) * nt < end; e += U * nt) {
@@ -311,7 +340,7 @@ __global__ void __launch_bounds__(1024) rct_vbar_kernel(RctArgs a, const double 
 // row); the tile's rows x KT sums stay in LDS (ds_add_f64) and go to Y at
 // the end: stored with one slice, added with FP64 atomics (Y cleared first)
 // with several.
-template <class VT, bool PACKED, int KT>
+template <class VT, int KEY, int KT>
 __global__ void __launch_bounds__(1024) rct_spmm_kernel(RctArgs a, uint32_t k, uint32_t k0,
                                                         const double *__restrict__ X, double *__restrict__ Y) {
     extern __shared__ double acc[];
@@ -323,7 +352,7 @@ __global__ void __launch_bounds__(1024) rct_spmm_kernel(RctArgs a, uint32_t k, u
     __syncthreads();
     const uint32_t t = tile.rb * a.S + tile.s;
     const uint32_t beg = a.tile_ptr[t], end = a.tile_ptr[t + 1];
-    const RctEntries<VT, PACKED> ent{static_cast<const VT *>(a.val), a.key, a.col, a.lrow, a.sbase, a.rbits, a.ncols};
+    const RctEntries<VT, KEY> ent(a, row0);
     const uint32_t j = tid % KT, nsub = nt / KT;  // nt is a multiple of 64
     const bool live = k0 + j < k;                 // the last k-tile may be partial
     const double *__restrict__ xj = X + k0 + j;
@@ -358,17 +387,17 @@ __global__ void __launch_bounds__(1024) rct_spmm_kernel(RctArgs a, uint32_t k, u
     }
 }
 
-template <class VT, bool PACKED>
+template <class VT, int KEY>
 hipError_t rct_spmm_vt(const RctArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s) {
     const size_t lds = size_t(a.R) * kt * sizeof(double);
     const uint32_t per = 8 / a.S;
     const unsigned grid = (a.nrb + per - 1) / per * 8;
     for (uint32_t k0 = 0; k0 < k; k0 += kt) {
         switch (kt) {
-            case 8: rct_spmm_kernel<VT, PACKED, 8><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
-            case 16: rct_spmm_kernel<VT, PACKED, 16><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
-            case 32: rct_spmm_kernel<VT, PACKED, 32><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
-            case 64: rct_spmm_kernel<VT, PACKED, 64><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
+            case 8: rct_spmm_kernel<VT, KEY, 8><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
+            case 16: rct_spmm_kernel<VT, KEY, 16><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
+            case 32: rct_spmm_kernel<VT, KEY, 32><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
+            case 64: rct_spmm_kernel<VT, KEY, 64><<<grid, a.block, lds, s>>>(a, k, k0, X, Y); break;
             default: return hipErrorInvalidValue;
         }
         const hipError_t e = hipGetLastError();
@@ -388,10 +417,14 @@ hipError_t rct_spmm(const RctArgs &a, uint32_t k, uint32_t kt, const double *X, 
         const hipError_t e = clear_f64(Y, size_t(a.nrows) * k, s);
         if (e != hipSuccess) return e;
     }
-    const bool packed = a.key != nullptr;
+    const int km = key_mode(a);
     if (a.vtype == 1)
-        return packed ? rct_spmm_vt<float, true>(a, k, kt, X, Y, s) : rct_spmm_vt<float, false>(a, k, kt, X, Y, s);
-    return packed ? rct_spmm_vt<double, true>(a, k, kt, X, Y, s) : rct_spmm_vt<double, false>(a, k, kt, X, Y, s);
+        return km == KEY_DIAG     ? rct_spmm_vt<float, KEY_DIAG>(a, k, kt, X, Y, s)
+               : km == KEY_PACKED ? rct_spmm_vt<float, KEY_PACKED>(a, k, kt, X, Y, s)
+                                  : rct_spmm_vt<float, KEY_SPLIT>(a, k, kt, X, Y, s);
+    return km == KEY_DIAG     ? rct_spmm_vt<double, KEY_DIAG>(a, k, kt, X, Y, s)
+           : km == KEY_PACKED ? rct_spmm_vt<double, KEY_PACKED>(a, k, kt, X, Y, s)
+                              : rct_spmm_vt<double, KEY_SPLIT>(a, k, kt, X, Y, s);
 }
 
 hipError_t rct_vbar(const RctArgs &a, const double *ybar, const double *x, double *vbar, hipStream_t s) {
@@ -399,10 +432,11 @@ hipError_t rct_vbar(const RctArgs &a, const double *ybar, const double *x, doubl
     const uint32_t per = 8 / a.S;
     const unsigned grid = (a.nrb + per - 1) / per * 8;
     const size_t lds = size_t(a.R) * sizeof(double);
-    if (a.key)
-        rct_vbar_kernel<true><<<grid, a.block, lds, s>>>(a, ybar, x, vbar);
-    else
-        rct_vbar_kernel<false><<<grid, a.block, lds, s>>>(a, ybar, x, vbar);
+    switch (key_mode(a)) {
+        case KEY_DIAG: rct_vbar_kernel<KEY_DIAG><<<grid, a.block, lds, s>>>(a, ybar, x, vbar); break;
+        case KEY_PACKED: rct_vbar_kernel<KEY_PACKED><<<grid, a.block, lds, s>>>(a, ybar, x, vbar); break;
+        default: rct_vbar_kernel<KEY_SPLIT><<<grid, a.block, lds, s>>>(a, ybar, x, vbar);
+    }
     return hipGetLastError();
 }
 
@@ -423,7 +457,7 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
             if (e != hipSuccess) return e;
         }
     }
-    const bool packed = a.key != nullptr;
+    const int km = key_mode(a);
 #define AX_RCT_LAUNCH(VT, P)                                                             \
     do {                                                                                 \
         if (vb)                                                                          \
@@ -434,15 +468,19 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
             rct_kernel<VT, P, false, false><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar); \
     } while (0)
     if (a.vtype == 1) {
-        if (packed)
-            AX_RCT_LAUNCH(float, true);
+        if (km == KEY_DIAG)
+            AX_RCT_LAUNCH(float, KEY_DIAG);
+        else if (km == KEY_PACKED)
+            AX_RCT_LAUNCH(float, KEY_PACKED);
         else
-            AX_RCT_LAUNCH(float, false);
+            AX_RCT_LAUNCH(float, KEY_SPLIT);
     } else {
-        if (packed)
-            AX_RCT_LAUNCH(double, true);
+        if (km == KEY_DIAG)
+            AX_RCT_LAUNCH(double, KEY_DIAG);
+        else if (km == KEY_PACKED)
+            AX_RCT_LAUNCH(double, KEY_PACKED);
         else
-            AX_RCT_LAUNCH(double, false);
+            AX_RCT_LAUNCH(double, KEY_SPLIT);
     }
 #undef AX_RCT_LAUNCH
     hipError_t e = hipGetLastError();

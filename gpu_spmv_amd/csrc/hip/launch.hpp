@@ -96,6 +96,12 @@ struct RctArgs {
     uint32_t nrows, R, S, nrb, vtype, block, rbits, ncols;
     double *part;  // S > 1: S x nrows slice partials (nullptr: atomic adds into y)
     uint32_t nt_stream = 0;  // 1: the SpMV tile kernel streams the matrix with non-temporal loads
+    // narrow bands: a 16-bit key per entry (key16 != nullptr; key, col and
+    // lrow unused) with the column as an offset of cbits16 bits from sbase
+    // (runs of 128 entries) and row - column - dmin above it (dmin modulo
+    // 2^32)
+    const uint16_t *key16 = nullptr;
+    uint32_t cbits16 = 0, dmin = 0;
 };
 // S > 1 with a.part: the tile kernel stores every slice's row sums in its
 // partial vector, then part_merge adds them in slice order into y; mid

@@ -738,14 +738,16 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
         }
     }
     if (S != 1 && S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_INVALID, "RCT: slices must be 1, 2, 4 or 8");
-    // Narrow bands reuse x within a band whatever the tile height: 2048-row
-    // tiles (16 KB of LDS) let several workgroups share a CU, which pays on
-    // matrices mixing short and long rows (256 banded matrices as one
-    // block-diagonal plan: 1259 us against 1390 us at 16384 rows, row
-    // lengths varying by 60 % of their mean).  Rows of one length (a mesh:
-    // the 128^3 stencil renumbered by reverse Cuthill-McKee, 7 %) take the
-    // tallest tiles that still cover the CUs: 168 against 180 us at 2048
-    // rows, with the non-temporal stream loads below.
+    // Narrow bands reuse x within a band whatever the tile height: 512-row
+    // tiles (4 KB of LDS) let many workgroups share a CU and even out the
+    // end of the launch, which pays on matrices mixing short and long rows
+    // (256 banded matrices as one block-diagonal plan, row lengths varying
+    // by 60 % of their mean, 16-bit keys: 985 us at 512 rows, 973 at 256,
+    // 999 at 1024, 1035 at 2048; 32-bit keys: 1170 us at 512, 1227 at 2048,
+    // 1390 at 16384).  Rows of one length (a mesh: the 128^3 stencil
+    // renumbered by reverse Cuthill-McKee, 7 %) take the tallest tiles that
+    // still cover the CUs: 168 against 180 us at 2048 rows, with the
+    // non-temporal stream loads below.
     uint32_t Rt = p.o.tile_rows ? p.o.tile_rows : auto_rows(S);
     if (!p.o.tile_rows && banded) {
         double sum = 0.0, sq = 0.0;
@@ -756,7 +758,7 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
             sq += l * l;
         }
         const double mean = n ? sum / n : 0.0, var = n ? std::max(0.0, sq / n - mean * mean) : 0.0;
-        if (!(mean > 0.0 && std::sqrt(var) < 0.25 * mean)) Rt = std::min<uint32_t>(2048, auto_rows(1));
+        if (!(mean > 0.0 && std::sqrt(var) < 0.25 * mean)) Rt = std::min<uint32_t>(512, auto_rows(1));
     }
     if (p.lds_per_row > 8)  // a transposed plan's tiles are cut to fit (8192 rows when automatic)
         Rt = std::min(Rt, p.o.tile_rows || banded ? lds_rows : std::min<uint32_t>(lds_rows, 8192));
@@ -840,6 +842,43 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     }
     const uint32_t rbits = bits(R), cbits = bits(span);
     const bool packed = rbits + cbits <= 32 && !(p.o.flags & AXSPMV_PLAN_RCT_SPLIT);
+    // Narrow bands: one 16-bit key per entry, the column as an offset from
+    // the smallest column of its run of 128 entries and, above it, the
+    // entry's row minus its column less the smallest such difference, where
+    // both fit 16 bits together -- 10 bytes per FP64 entry instead of 12.
+    // Entries of a band lie within a few columns of the diagonal, and a run
+    // of 128 of them covers few columns, also where it passes from one tile
+    // to the next (its columns step back by about the band's width).
+    constexpr uint32_t DSEG_LOG = 7;  // entries per run: 1 << DSEG_LOG
+    const uint64_t nseg16 = std::max<uint64_t>(1, ceil_div(nnz + (nnz & 1), uint64_t(1) << DSEG_LOG));
+    std::vector<uint32_t> sbase16;
+    int64_t dlo = 0;
+    uint32_t cbits16 = 0;
+    bool diag = packed && nnz > 0 && !(p.o.flags & AXSPMV_PLAN_RCT_WIDE_KEY);
+    if (diag) {
+        int64_t lo = INT64_MAX, hi = INT64_MIN;
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : lo) reduction(max : hi)
+        for (uint32_t rb = 0; rb < nrb; ++rb)
+            for (uint64_t q = cnt[uint64_t(rb) * S]; q < cnt[uint64_t(rb + 1) * S]; ++q) {
+                const int64_t d = int64_t(rbs[rb]) + lrow[q] - int64_t(col[q]);
+                lo = std::min(lo, d);
+                hi = std::max(hi, d);
+            }
+        sbase16.assign(nseg16, 0);
+        uint64_t span16 = 1;
+#pragma omp parallel for schedule(static) reduction(max : span16)
+        for (uint64_t g = 0; g < nseg16; ++g) {
+            const uint64_t q0 = g << DSEG_LOG, q1 = std::min<uint64_t>(nnz, q0 + (uint64_t(1) << DSEG_LOG));
+            if (q0 >= nnz) continue;
+            const auto mm = std::minmax_element(col.begin() + q0, col.begin() + q1);
+            sbase16[g] = *mm.first;
+            span16 = std::max<uint64_t>(span16, uint64_t(*mm.second) - *mm.first + 1);
+        }
+        dlo = lo;
+        cbits16 = bits(span16);
+        diag = cbits16 + bits(uint64_t(hi - lo) + 1) <= 16;
+    }
+    const uint64_t kbytes = diag ? 2 : packed ? 4 : 6;  // column and row per entry
     // one spare entry when nnz is odd: a kernel may load entries in aligned pairs
     const uint64_t nnz_pad = nnz + (nnz & 1);
     void *dv = p.dalloc(vs * nnz_pad);
@@ -859,8 +898,21 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     }
     AX_TRY(p.upload(dt, tptr.data(), 4 * tptr.size(), s));
     uint32_t *dc = nullptr, *dk = nullptr, *dcb = nullptr;
-    uint16_t *dl = nullptr;
-    if (packed) {
+    uint16_t *dl = nullptr, *dk16 = nullptr;
+    if (diag) {
+        std::vector<uint16_t> key(nnz);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (uint32_t rb = 0; rb < nrb; ++rb)
+            for (uint64_t q = cnt[uint64_t(rb) * S]; q < cnt[uint64_t(rb + 1) * S]; ++q) {
+                const uint64_t d = uint64_t(int64_t(rbs[rb]) + lrow[q] - int64_t(col[q]) - dlo);
+                key[q] = uint16_t(d << cbits16 | (col[q] - sbase16[q >> DSEG_LOG]));
+            }
+        dk16 = static_cast<uint16_t *>(p.dalloc(2 * nnz_pad));
+        dcb = static_cast<uint32_t *>(p.dalloc(4 * nseg16));
+        if (!dk16 || !dcb) return AXSPMV_E_NOMEM;
+        AX_TRY(p.upload(dk16, key.data(), 2 * nnz, s));
+        AX_TRY(p.upload(dcb, sbase16.data(), 4 * nseg16, s));
+    } else if (packed) {
         std::vector<uint32_t> key(nnz ? nnz : 1);
 #pragma omp parallel for schedule(static)
         for (uint64_t q = 0; q < nnz; ++q)
@@ -915,16 +967,19 @@ int plan_rct(Plan &p, const axspmv_csr *a, hipStream_t s) {
     // (the 2^20-row power-law matrix, 252 MB: 58 -> 89 us); short tiles,
     // several workgroups per CU, measured 1 % slower with them (256 banded
     // matrices, 2048-row tiles: 1236 -> 1249 us).
-    p.rct.nt_stream = (vs + (packed ? 4 : 6)) * nnz > (256ull << 20) && R >= 4096 ? 1 : 0;
+    p.rct.nt_stream = (vs + kbytes) * nnz > (256ull << 20) && R >= 4096 ? 1 : 0;
+    p.rct.key16 = dk16;
+    p.rct.cbits16 = cbits16;
+    p.rct.dmin = uint32_t(uint64_t(dlo));
     p.slots = nnz;
     // matrix stream; with S > 1 the partials written and read back and y
     // written by the merge, or the clear of y and S atomic adds per row
-    p.stream_bytes = (vs + (packed ? 4 : 6)) * nnz + 4 * tptr.size() +
+    p.stream_bytes = (vs + kbytes) * nnz + 4 * tptr.size() +
                      (S == 1 ? 0 : dpart ? 8ull * (2 * S + 1) * n : 8ull * (S + 1) * n);
     p.launches = S > 1 ? 2 : 1;
     p.nblocks = uint32_t(ntile);
     p.kind = Plan::RCT;
-    std::snprintf(p.name, sizeof p.name, "rct_r%u_s%u%s%s", Rt, S, packed ? "" : "_split",
+    std::snprintf(p.name, sizeof p.name, "rct_r%u_s%u%s%s", Rt, S, diag ? "_k16" : packed ? "" : "_split",
                   S > 1 && !dpart && p.lds_per_row == 8 ? "_atomic" : "");
     return AXSPMV_OK;
 }

@@ -356,6 +356,15 @@ typedef struct axspmv_opts {
                                     Every operation is available.  Building
                                     Q costs a few breadth-first passes over
                                     A on the host. */
+#define AXSPMV_PLAN_RCT_WIDE_KEY 128u /* RCT: keep the 32-bit key where a
+                                    16-bit one would fit.  By default a
+                                    matrix whose entries lie close to the
+                                    diagonal (a band of up to about 127 on
+                                    each side) stores each entry's column and
+                                    row in 16 bits -- the column's offset
+                                    from the smallest column of its run of
+                                    128 entries and the row minus the column
+                                    -- and its plan's name ends in "_k16". */
 #define AXSPMV_PLAN_AX_LAYOUT 4u /* AXT UNC th == 1 or AXC, FP64 values: keep
                                     the reference's interleaved ax array on the
                                     device (tiles of thw values then thw

@@ -225,36 +225,102 @@ def wide_csr(nrows, ncols, seed):
                         rng.uniform(-1, 1, nrows), ncols)
 
 
-@pytest.mark.parametrize("gen,slices,split", [
-    # runs of 1024 entries step back a few columns where they pass into the
-    # next row block: based on their smallest column, 14 + 6 bits
-    (lambda: ax.banded(300000, 16, 0.5, seed=9), 1, False),
+@pytest.mark.parametrize("gen,slices,flags,key", [
+    # a band of 16 each side: row - column spans 33 values (6 bits), a run of
+    # 128 entries at most 36 columns where it steps back into the next tile
+    # (6 bits): the 16-bit key
+    (lambda: ax.banded(300000, 16, 0.5, seed=9), 1, 0, "k16"),
+    (lambda: ax.banded(300000, 16, 0.5, seed=9), 4, 0, "k16"),
+    # ... unless asked for the 32-bit key: runs of 1024 entries step back a few
+    # columns where they pass into the next row block: based on their
+    # smallest column, 14 + 6 bits
+    (lambda: ax.banded(300000, 16, 0.5, seed=9), 1, ax.PLAN_RCT_WIDE_KEY, ""),
+    # the widest band of the 256-matrix batch: 129 values of row - column (8
+    # bits) and runs stepping back about 128 columns (8 bits)
+    (lambda: ax.banded(200000, 64, 0.3, seed=12), 1, 0, "k16"),
+    # a band of 200: 401 values (9 bits) and runs stepping back about 400
+    # columns (9 bits) do not fit 16 bits
+    (lambda: ax.banded(100000, 200, 0.3, seed=13), 1, 0, ""),
     # a run covers about 1024 x 300,000 / 106,000 columns: 14 + 12 bits
-    (lambda: ax.random_sparse(300000, 1, 12, seed=10), 1, False),
+    (lambda: ax.random_sparse(300000, 1, 12, seed=10), 1, 0, ""),
     # runs passing from the last slice of a row block into the first slice
     # of the next count round from their first column
-    (lambda: ax.random_sparse(300000, 1, 12, seed=10), 4, False),
+    (lambda: ax.random_sparse(300000, 1, 12, seed=10), 4, 0, ""),
     # 16,384 entries per tile over 2^23 columns: a run covers ~2^19 columns,
     # 14 + 19 bits do not fit one word
-    (lambda: wide_csr(32768, 1 << 23, seed=11), 1, True),
-], ids=["banded", "random", "random_s4", "wide"])
-def test_rct_key_packing(gen, slices, split):
-    """The packed key holds the column as an offset from a base column per
-    run of 1024 entries; where the offsets need more bits than the row
-    leaves, the plan keeps separate column and row arrays.  Both give the
-    product of the oracle."""
+    (lambda: wide_csr(32768, 1 << 23, seed=11), 1, 0, "split"),
+], ids=["banded", "banded_s4", "banded_wide_key", "band64", "band200", "random", "random_s4", "wide"])
+def test_rct_key_packing(gen, slices, flags, key):
+    """Column and tile row of an entry: 16 bits (the column's offset from the
+    smallest column of its run of 128 entries, and the row minus the column)
+    on narrow bands; else one 32-bit word (the column as an offset from a
+    base column per run of 1024 entries, and the row); where the offsets
+    need more bits than the row leaves, separate column and row arrays.  All
+    give the product of the oracle, and SpMM too."""
     m = gen()
-    p = ax.Plan(m, ax.FMT_RCT, tile_rows=16384, slices=slices)
-    assert p.info()["name"].endswith("_split") == split, p.info()["name"]
+    p = ax.Plan(m, ax.FMT_RCT, tile_rows=16384, slices=slices, flags=flags)
+    name = p.info()["name"]
+    assert ("_k16" in name) == (key == "k16") and ("_split" in name) == (key == "split"), name
     x = ax.uniform(16, m.ncols)
     assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
     if m.ncols > 1 << 20:
         return  # X of 8 columns would take 512 MB
     X = ax.uniform(17, m.ncols * 8).reshape(m.ncols, 8)
-    q = ax.Plan(m, ax.FMT_RCT, tile_rows=2048, slices=slices)
+    q = ax.Plan(m, ax.FMT_RCT, tile_rows=2048, slices=slices, flags=flags)
+    assert ("_k16" in q.info()["name"]) == (key == "k16"), q.info()["name"]
     dX, dY = ax.DeviceBuffer.from_array(X), ax.DeviceBuffer(8 * m.nrows * 8)
     q.spmm(8, dX, dY)
     assert rel_err(orc.spmm(m, X), dY.download(np.float64, m.nrows * 8).reshape(m.nrows, 8)) <= TOL
+
+
+def test_rct_banded_tile_rows():
+    """Bands (x wider than 2 MB) with rows of uneven length (two bands of 2
+    and 60 each side, one after the other) take 512-row tiles; a band of one
+    width, with rows of about one length, the tallest tiles that still give
+    every CU one (300,000 / 256 rows, rounded up to 64)."""
+    import scipy.sparse as sp
+    a, b = ax.banded(200000, 2, 0.3, seed=21), ax.banded(200000, 60, 0.3, seed=22)
+    bd = sp.block_diag([sp.csr_matrix((v.val, v.col, v.row), shape=(v.nrows, v.ncols)) for v in (a, b)]).tocsr()
+    bd.sort_indices()
+    m = ax.CsrMatrix(bd.indptr.astype(np.uint32), bd.indices.astype(np.uint32), bd.data, ncols=bd.shape[1])
+    p = ax.Plan(m, ax.FMT_RCT)
+    assert p.info()["name"] == "rct_r512_s1_k16", p.info()["name"]
+    x = ax.uniform(24, m.ncols)
+    assert rel_err(orc.ncsr(m, x), ax.spmv_host(p, x)) <= TOL
+    c = ax.banded(300000, 60, 0.3, seed=23)
+    assert ax.Plan(c, ax.FMT_RCT).info()["name"] == "rct_r1216_s1_k16"
+
+
+def small_banded(n, hbw, seed):
+    """Banded n x n matrix with about half its band filled, small-integer
+    values (exact sums)."""
+    m = ax.banded(n, hbw, 0.5, seed=seed)
+    rng = np.random.default_rng(seed)
+    return ax.CsrMatrix(m.row, m.col, rng.integers(-9, 10, m.nnz).astype(np.float64), ncols=n)
+
+
+@pytest.mark.parametrize("n,hbw", [(1, 0), (2, 1), (3, 1), (5, 2), (70, 1), (129, 3), (700, 9), (4000, 64)])
+@pytest.mark.parametrize("slices,block,vtype", [(1, 64, ax.F64), (2, 128, ax.F64), (8, 64, ax.F64), (4, 192, ax.F32)])
+def test_rct_diag_key_pairs(n, hbw, slices, block, vtype):
+    """The 16-bit key on bands: tiles starting or ending at odd entries, runs
+    of 128 entries crossing tiles and slices, the spare entry of an odd
+    count; the product equals the oracle bit for bit (integer values and x),
+    and the value gradient and adjoint of the transposed plan too."""
+    m = small_banded(n, hbw, seed=n + hbw)
+    x = np.random.default_rng(n).integers(-9, 10, m.ncols).astype(np.float64)
+    p = ax.Plan(m, ax.FMT_RCT, tile_rows=64, slices=slices, block=block, vtype=vtype,
+                flags=ax.PLAN_TRANSPOSE if vtype == ax.F64 else 0)
+    assert "_k16" in p.info()["name"], p.info()["name"]
+    np.testing.assert_array_equal(ax.spmv_host(p, x), orc.ncsr(m, x))
+    if vtype == ax.F32:
+        return
+    w = np.random.default_rng(n + 1).integers(-9, 10, m.nrows).astype(np.float64)
+    dw, dx = ax.DeviceBuffer.from_array(w), ax.DeviceBuffer.from_array(x)
+    dxb, dvb = ax.DeviceBuffer(8 * m.ncols), ax.DeviceBuffer(8 * max(m.nnz, 1))
+    p.adjoint(dw, dxb)
+    p.vbar(dw, dx, dvb)
+    np.testing.assert_array_equal(dxb.download(np.float64, m.ncols), orc.ncsr_t(m, w, m.ncols))
+    np.testing.assert_array_equal(p.vbar_csr(dvb.download(np.float64, m.nnz)), orc.vbar(m, w, x))
 
 
 def small_csr(nrows, ncols, nnz, seed):
