@@ -25,7 +25,7 @@ its structure, and the 256-matrix banded batch in one launch.
   python bench.py [--steps K] [--warmup W] [--formats flagship|all]
                   [--suite all|headline]
   python bench.py --calibrate      copy bandwidth and the x-footprint sweep
-  python bench.py --pmc-probe [stencil|spmm]
+  python bench.py --pmc-probe [stencil|spmm|stencil_rcm|batch256]
                                    a 256 MB copy then 5 SpMVs of each flagship
                                    kernel on the power-law matrix (or on the
                                    permuted 27-point stencil), or 3 SpMMs at
@@ -465,12 +465,26 @@ def pmc_probe(matrix: str = "powerlaw"):
     and RCT); with "spmm", 3 SpMMs at k = 8, 32 and 64 of the plans the
     autotuner picks on the power-law matrix; with "stencil_rcm", 5 SpMVs of
     the reordered RCT (r8192, one slice) and AXT UNC th = 1 plans on the
-    permuted stencil."""
+    permuted stencil; with "batch256", 5 SpMVs of the 256 banded matrices'
+    block-diagonal RCT plan with its default 16-bit keys, then 5 with 32-bit
+    keys."""
     n = 32 << 20
     src, dst = ax.DeviceBuffer(8 * n), ax.DeviceBuffer(8 * n)
     L.call("axspmv_copy_f64", src.ptr, dst.ptr, n, None)
     L.call("axspmv_device_sync")
     del src, dst
+    if matrix == "batch256":
+        mats = W.config4b()
+        for kw in ({}, dict(flags=ax.PLAN_RCT_WIDE_KEY)):
+            bt = ax.Batch(mats, fmt=ax.FMT_RCT, **kw)
+            dxb = ax.DeviceBuffer.from_array(ax.uniform(6, int(bt.xoff[-1])))
+            dyb = ax.DeviceBuffer(8 * int(bt.yoff[-1]))
+            for _ in range(5):
+                bt.spmv(dxb, dyb)
+            L.call("axspmv_device_sync")
+            log(f"pmc-probe: batch256 {kw} {bt.info()}")
+            del bt, dxb, dyb
+        return
     if matrix in ("stencil", "stencil_rcm"):
         m, x = W.config3()
     else:
@@ -687,7 +701,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
-    ap.add_argument("--pmc-probe", nargs="?", const="powerlaw", choices=["powerlaw", "stencil", "spmm", "stencil_rcm"])
+    ap.add_argument("--pmc-probe", nargs="?", const="powerlaw", choices=["powerlaw", "stencil", "spmm", "stencil_rcm",
+                                                                           "batch256"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: SpMV here runs on a single MI355X; "
