@@ -160,7 +160,8 @@ struct TileOf {
 // forward x, y = A^T r and vb holds r[i] * x[j] for every nonzero A(i, j).
 template <class VT, int KEY, bool VB, bool NT>
 __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__restrict__ x, double *__restrict__ y,
-                                                   const double *__restrict__ xr, double *__restrict__ vb) {
+                                                   const double *__restrict__ xr, double *__restrict__ vb,
+                                                   const double *__restrict__ bvec, double *__restrict__ res) {
     extern __shared__ double yl[];
     TileOf tile;
     if (!tile.from(a, blockIdx.x)) return;  // the whole workgroup leaves together
@@ -198,7 +199,9 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     // entries of the tile (an entry outside it, or a lane past the end,
     // stores its partner's own value again), as one 16-byte store when every
     // lane of the wave holds a whole pair and vb is 16-byte aligned, else as
-    // two 8-byte stores.
+    // two 8-byte stores; non-temporal, as nothing here reads vb again (the
+    // permuted 128^3 stencil's forward + adjoint, renumbered: 485.8 ->
+    // 478.8 us; the 2^20-row power-law matrix's: 233.0 -> 229.2 us).
     constexpr int UP = (RCT_UNROLL + 1) / 2;
     struct PSet {
         uint32_t c[UP][2], r[UP][2];
@@ -251,10 +254,11 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
                 if constexpr (VB) {
                     const double p0 = xl[j][0] * xv[j][0], p1 = xl[j][1] * xv[j][1];
                     if (__all(vb16 && S.at[j][1] == S.at[j][0] + 1)) {  // whole pairs (at[j][0] even)
-                        reinterpret_cast<double2 *>(vb)[S.at[j][0] >> 1] = double2{p0, p1};
+                        typedef double f64x2 __attribute__((ext_vector_type(2)));
+                        __builtin_nontemporal_store(f64x2{p0, p1}, reinterpret_cast<f64x2 *>(vb) + (S.at[j][0] >> 1));
                     } else {
-                        vb[S.at[j][0]] = p0;
-                        vb[S.at[j][1]] = p1;
+                        __builtin_nontemporal_store(p0, vb + S.at[j][0]);
+                        __builtin_nontemporal_store(p1, vb + S.at[j][1]);
                     }
                 }
             }
@@ -291,7 +295,10 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
     if (a.S == 1) {
         // non-temporal: y kept in the L2s would take room from the next
         // SpMV's x (the 1024^2 Laplacian: 14.6 -> 13.9 us)
-        for (uint32_t k = tid; k < rows; k += nt) __builtin_nontemporal_store(yl[k], out + k);
+        for (uint32_t k = tid; k < rows; k += nt) {
+            __builtin_nontemporal_store(yl[k], out + k);
+            if (res) res[row0 + k] = yl[k] - bvec[row0 + k];  // forward + adjoint: r = y - b, read next
+        }
     } else if (a.part && !VB) {
         // the slice's partial vector, read once by the merge: non-temporal
         double *ps = a.part + size_t(s) * a.nrows + row0;
@@ -302,7 +309,8 @@ __global__ void __launch_bounds__(1024) rct_kernel(RctArgs a, const double *__re
 }
 
 // Value gradient in entry order: vbar[e] = ybar[row(e)] * x[col(e)].  The
-// tile's ybar rows sit in LDS; x is gathered in the tile's column order.
+// tile's ybar rows sit in LDS; x is gathered in the tile's column order;
+// vbar goes out with non-temporal stores.
 template <int KEY>
 __global__ void __launch_bounds__(1024) rct_vbar_kernel(RctArgs a, const double *__restrict__ ybar,
                                                         const double *__restrict__ x, double *__restrict__ vbar) {
@@ -326,12 +334,12 @@ __global__ void __launch_bounds__(1024) rct_vbar_kernel(RctArgs a, const double 
 #pragma unroll
         for (int j = 0; j < U; ++j) xv[j] = x[c[j]];
 #pragma unroll
-        for (int j = 0; j < U; ++j) vbar[e + j * nt] = yb[r[j]] * xv[j];
+        for (int j = 0; j < U; ++j) __builtin_nontemporal_store(yb[r[j]] * xv[j], vbar + e + j * nt);
     }
     for (; e < end; e += nt) {
         uint32_t c, r;
         ent.at(e, c, r);
-        vbar[e] = yb[r] * x[c];
+        __builtin_nontemporal_store(yb[r] * x[c], vbar + e);
     }
 }
 
@@ -441,8 +449,9 @@ hipError_t rct_vbar(const RctArgs &a, const double *ybar, const double *x, doubl
 }
 
 hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid, const double *xr,
-               double *vbar) {
+               double *vbar, const double *rb, double *res) {
     if (!a.nrows) return hipSuccess;
+    if (vbar) rb = res = nullptr;  // the residual belongs to a forward pass
     const uint32_t per = 8 / a.S;
     const uint32_t groups = (a.nrb + per - 1) / per;
     const unsigned grid = groups * 8;
@@ -458,14 +467,14 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
         }
     }
     const int km = key_mode(a);
-#define AX_RCT_LAUNCH(VT, P)                                                             \
-    do {                                                                                 \
-        if (vb)                                                                          \
-            rct_kernel<VT, P, true, false><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar);  \
-        else if (a.nt_stream)                                                            \
-            rct_kernel<VT, P, false, true><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar);  \
-        else                                                                             \
-            rct_kernel<VT, P, false, false><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar); \
+#define AX_RCT_LAUNCH(VT, P)                                                                         \
+    do {                                                                                             \
+        if (vb)                                                                                      \
+            rct_kernel<VT, P, true, false><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar, rb, res);  \
+        else if (a.nt_stream)                                                                        \
+            rct_kernel<VT, P, false, true><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar, rb, res);  \
+        else                                                                                         \
+            rct_kernel<VT, P, false, false><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar, rb, res); \
     } while (0)
     if (a.vtype == 1) {
         if (km == KEY_DIAG)
@@ -486,7 +495,9 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
     hipError_t e = hipGetLastError();
     if (partials) {
         if (e == hipSuccess && mid) e = hipEventRecord(mid, s);
-        if (e == hipSuccess) e = part_merge(a.part, a.S, a.nrows, y, s);
+        if (e == hipSuccess) e = part_merge(a.part, a.S, a.nrows, y, s, rb, res);
+    } else if (res && a.S > 1 && e == hipSuccess) {
+        e = residual(a.nrows, y, rb, res, s);  // after the slices' adds
     }
     return e;
 }

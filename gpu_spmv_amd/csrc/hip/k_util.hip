@@ -47,9 +47,12 @@ __global__ void __launch_bounds__(256) clear_kernel(double *__restrict__ y, size
 // the room the next SpMV's x slices need (RCT on the 2^20-row power-law
 // matrix, y stored normally: 61.7 or 78.5 us per SpMV depending on where y
 // lies, as with a y cleared by ordinary stores).
+// With res (a forward + adjoint pass), res = y - rb as well, stored
+// normally: the adjoint reads it next.
 template <bool VEC>
 __global__ void __launch_bounds__(256) merge_kernel(const double *__restrict__ part, uint32_t S, uint32_t nrows,
-                                                    double *__restrict__ y) {
+                                                    double *__restrict__ y, const double *__restrict__ rb,
+                                                    double *__restrict__ res) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if constexpr (VEC) {
         if (2 * i >= nrows) return;
@@ -57,11 +60,13 @@ __global__ void __launch_bounds__(256) merge_kernel(const double *__restrict__ p
         f64x2 v = __builtin_nontemporal_load(p2);
         for (uint32_t k = 1; k < S; ++k) v += __builtin_nontemporal_load(p2 + size_t(k) * (nrows / 2));
         __builtin_nontemporal_store(v, reinterpret_cast<f64x2 *>(y) + i);
+        if (res) reinterpret_cast<f64x2 *>(res)[i] = v - reinterpret_cast<const f64x2 *>(rb)[i];
     } else {
         if (i >= nrows) return;
         double v = __builtin_nontemporal_load(part + i);
         for (uint32_t k = 1; k < S; ++k) v += __builtin_nontemporal_load(part + size_t(k) * nrows + i);
         __builtin_nontemporal_store(v, y + i);
+        if (res) res[i] = v - rb[i];
     }
 }
 
@@ -113,6 +118,73 @@ __global__ void __launch_bounds__(1024) permute_chunk_kernel(const uint32_t *__r
         for (int u = 0; u < U; ++u) {
             const uint32_t i = i0 + u * nt;
             if (i < len) dst[W ? to[u] : base + i] = lds[j[u]];
+        }
+    }
+}
+
+// The second pass of one permutation (y[c L + i] = t[c L + b2[c L + i]]),
+// the residual r = y - bv, and the first pass of another that takes r from
+// the same chunks (t2[w1[c L + i]] = r[c L + b1[c L + i]]), one chunk per
+// workgroup: y's chunk, then r's, stay in LDS (2 L doubles), so r is not
+// read back.
+__global__ void __launch_bounds__(1024) back_residual_in_kernel(const uint16_t *__restrict__ b2,
+                                                                const uint32_t *__restrict__ w1,
+                                                                const uint16_t *__restrict__ b1, uint32_t n,
+                                                                uint32_t L, const double *__restrict__ t,
+                                                                const double *__restrict__ bv, double *__restrict__ y,
+                                                                double *__restrict__ r, double *__restrict__ t2) {
+    constexpr int U = 8;
+    extern __shared__ double lds[];
+    double *lr = lds + L;
+    const uint64_t base = uint64_t(blockIdx.x) * L;
+    const uint32_t len = n - base < L ? uint32_t(n - base) : L, nt = blockDim.x;
+    for (uint32_t i0 = threadIdx.x; i0 < len; i0 += U * nt) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * nt;
+            if (i < len) v[u] = t[base + i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * nt;
+            if (i < len) lds[i] = v[u];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i0 = threadIdx.x; i0 < len; i0 += U * nt) {
+        uint32_t j[U];
+        double bb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * nt;
+            j[u] = i < len ? b2[base + i] : 0;
+            bb[u] = i < len ? bv[base + i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * nt;
+            if (i < len) {
+                const double v = lds[j[u]], rv = v - bb[u];
+                y[base + i] = v;
+                r[base + i] = rv;
+                lr[i] = rv;
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i0 = threadIdx.x; i0 < len; i0 += U * nt) {
+        uint32_t j[U], to[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * nt;
+            j[u] = i < len ? b1[base + i] : 0;
+            to[u] = i < len ? w1[base + i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * nt;
+            if (i < len) t2[to[u]] = lr[j[u]];
         }
     }
 }
@@ -245,17 +317,26 @@ hipError_t clear_f64(double *y, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t part_merge(const double *part, uint32_t S, uint32_t nrows, double *y, hipStream_t s) {
+hipError_t part_merge(const double *part, uint32_t S, uint32_t nrows, double *y, hipStream_t s, const double *rb,
+                      double *res) {
     if (!nrows) return hipSuccess;
-    if (nrows % 2 == 0 && reinterpret_cast<uintptr_t>(part) % 16 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0)
-        merge_kernel<true><<<(nrows / 2 + 255) / 256, 256, 0, s>>>(part, S, nrows, y);
+    auto a16 = [](const void *q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
+    if (nrows % 2 == 0 && a16(part) && a16(y) && (!res || (a16(res) && a16(rb))))
+        merge_kernel<true><<<(nrows / 2 + 255) / 256, 256, 0, s>>>(part, S, nrows, y, rb, res);
     else
-        merge_kernel<false><<<(nrows + 255) / 256, 256, 0, s>>>(part, S, nrows, y);
+        merge_kernel<false><<<(nrows + 255) / 256, 256, 0, s>>>(part, S, nrows, y, rb, res);
     return hipGetLastError();
 }
 
 hipError_t fill_u32(uint32_t *dst, size_t n, uint32_t v, hipStream_t s) {
     if (n) fill_u32_kernel<<<grid_for(n, 256), 256, 0, s>>>(dst, n, v);
+    return hipGetLastError();
+}
+
+hipError_t back_residual_in(const uint16_t *b2, const uint32_t *w1, const uint16_t *b1, uint32_t n, uint32_t L,
+                            const double *t, const double *bv, double *y, double *r, double *t2, hipStream_t s) {
+    const unsigned grid = unsigned((uint64_t(n) + L - 1) / L);
+    if (n) back_residual_in_kernel<<<grid, 1024, size_t(L) * 16, s>>>(b2, w1, b1, n, L, t, bv, y, r, t2);
     return hipGetLastError();
 }
 

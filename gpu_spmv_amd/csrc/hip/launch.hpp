@@ -109,9 +109,12 @@ struct RctArgs {
 // kernel clears y, then the tile kernel adds into it; mid is recorded
 // between those.  vbar != nullptr (a plan of A^T, 16 B of LDS per tile row,
 // no partials): also vbar[e] = xr[row(e)] * x[col(e)] for every entry, in
-// entry order.
+// entry order.  res != nullptr (vbar == nullptr): also res = y - rb, written
+// where y's rows are (the merge, or one slice's tile kernel; after the adds,
+// a pass of its own).
 hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipEvent_t mid = nullptr,
-               const double *xr = nullptr, double *vbar = nullptr);
+               const double *xr = nullptr, double *vbar = nullptr, const double *rb = nullptr,
+               double *res = nullptr);
 // Y = A X (k interleaved right-hand sides) on a one-slice plan, kt (8, 16,
 // 32 or 64) right-hand sides per launch; needs R * kt * 8 bytes of LDS
 hipError_t rct_spmm(const RctArgs &a, uint32_t k, uint32_t kt, const double *X, double *Y, hipStream_t s);
@@ -230,8 +233,10 @@ hipError_t fill_f64(double *dst, size_t n, double v, hipStream_t s);
 hipError_t fill_u32(uint32_t *dst, size_t n, uint32_t v, hipStream_t s);
 // y[0 .. n) = 0 in one pass (16-byte stores where y is 16-byte aligned)
 hipError_t clear_f64(double *y, size_t n, hipStream_t s);
-// y[r] = part[r] + part[nrows + r] + ... + part[(S - 1) nrows + r], in that order
-hipError_t part_merge(const double *part, uint32_t S, uint32_t nrows, double *y, hipStream_t s);
+// y[r] = part[r] + part[nrows + r] + ... + part[(S - 1) nrows + r], in that
+// order; with res, also res[r] = y[r] - rb[r]
+hipError_t part_merge(const double *part, uint32_t S, uint32_t nrows, double *y, hipStream_t s,
+                      const double *rb = nullptr, double *res = nullptr);
 // reads p[0 .. n) (n even, p 16-byte aligned): evicts caches without dirtying them
 hipError_t touch(const double *p, size_t n, double *sink, hipStream_t s);
 // the wave64 primitive op (enum axspmv_wave_op) lane by lane; n a multiple of 64
@@ -243,6 +248,13 @@ hipError_t wave_selftest(uint32_t op, const double *v, const uint32_t *h, double
 // of dst, every b value < the chunk's length)
 hipError_t permute_chunks(const uint32_t *w, const uint16_t *b, uint32_t n, uint32_t L, const double *src,
                           double *dst, hipStream_t s);
+// For every chunk c of L positions (n in all; L <= 10240: 2 L * 8 bytes of
+// LDS): y[c L + i] = t[c L + b2[c L + i]], r[c L + i] = y[c L + i] -
+// bv[c L + i], then t2[w1[c L + i]] = r[c L + b1[c L + i]] -- the second pass
+// of one permutation and the first of another through the same chunks,
+// joined by a residual (t2 must not overlap t)
+hipError_t back_residual_in(const uint16_t *b2, const uint32_t *w1, const uint16_t *b1, uint32_t n, uint32_t L,
+                            const double *t, const double *bv, double *y, double *r, double *t2, hipStream_t s);
 // dst[i k + j] = src[idx[i] k + j] for i < n, j < k (every idx[i] k + k <= the
 // length of src)
 hipError_t gather_rows(const uint32_t *idx, uint32_t n, uint32_t k, const double *src, double *dst, hipStream_t s);

@@ -23,6 +23,7 @@ ADJ_KERNELS = [
     ("csr_naive", ax.FMT_CSR, dict(kernel=ax.KER_NAIVE)),
     ("rct_auto", ax.FMT_RCT, dict()),
     ("rct_r512_s4", ax.FMT_RCT, dict(tile_rows=512, slices=4)),
+    ("rct_r512_s4_atomic", ax.FMT_RCT, dict(tile_rows=512, slices=4, flags=ax.PLAN_RCT_ATOMIC)),
     ("rct_r1024_s2_split", ax.FMT_RCT, dict(tile_rows=1024, slices=2, flags=ax.PLAN_RCT_SPLIT)),
     # taller than the transposed plan's tiles may be (x_bar sums and x values
     # of the fused gradient: 16 B of LDS per row): that plan is cut to fit
@@ -37,6 +38,8 @@ def mats():
     yield "random4096", ax.random_sparse(4096, 5, 15, seed=8)
     yield "banded_empty", ax.banded(3000, 3, 0.3, seed=6)
     yield "stencil16p", ax.stencil27(16, permute=True, seed=4)
+    # an odd row count: the slice merge (and its residual) in 8-byte steps
+    yield "random_odd", ax.random_sparse(3001, 2, 30, seed=9)
     # long rows (more than 64 units: the fused kernel's second sweep)
     n = 2000
     rl = rng.integers(0, 8, n)
