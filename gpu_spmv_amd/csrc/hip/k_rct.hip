@@ -467,9 +467,15 @@ hipError_t rct(const RctArgs &a, const double *x, double *y, hipStream_t s, hipE
         }
     }
     const int km = key_mode(a);
+    // the value-gradient sweep too streams with non-temporal loads where the
+    // plan asks for them (its v_bar stores are non-temporal anyway): the
+    // permuted 128^3 stencil's forward + adjoint, renumbered, 472.8 -> 467.7
+    // us, one box
 #define AX_RCT_LAUNCH(VT, P)                                                                         \
     do {                                                                                             \
-        if (vb)                                                                                      \
+        if (vb && a.nt_stream)                                                                       \
+            rct_kernel<VT, P, true, true><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar, rb, res);   \
+        else if (vb)                                                                                 \
             rct_kernel<VT, P, true, false><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar, rb, res);  \
         else if (a.nt_stream)                                                                        \
             rct_kernel<VT, P, false, true><<<grid, a.block, lds, s>>>(a, x, y, xr, vbar, rb, res);  \

@@ -95,7 +95,7 @@ struct RctArgs {
     const uint32_t *rb_start;  // nrb + 1 row-block starts
     uint32_t nrows, R, S, nrb, vtype, block, rbits, ncols;
     double *part;  // S > 1: S x nrows slice partials (nullptr: atomic adds into y)
-    uint32_t nt_stream = 0;  // 1: the SpMV tile kernel streams the matrix with non-temporal loads
+    uint32_t nt_stream = 0;  // 1: the tile kernel (SpMV, value-gradient sweep) streams with non-temporal loads
     // narrow bands: a 16-bit key per entry (key16 != nullptr; key, col and
     // lrow unused) with the column as an offset of cbits16 bits from sbase
     // (runs of 128 entries) and row - column - dmin above it (dmin modulo
