@@ -25,7 +25,7 @@ its structure, and the 256-matrix banded batch in one launch.
   python bench.py [--steps K] [--warmup W] [--formats flagship|all]
                   [--suite all|headline]
   python bench.py --calibrate      copy bandwidth and the x-footprint sweep
-  python bench.py --pmc-probe [stencil|spmm|stencil_rcm|batch256]
+  python bench.py --pmc-probe [stencil|spmm|stencil_rcm|batch256|fwd_adj_rcm]
                                    a 256 MB copy then 5 SpMVs of each flagship
                                    kernel on the power-law matrix (or on the
                                    permuted 27-point stencil), or 3 SpMMs at
@@ -467,12 +467,25 @@ def pmc_probe(matrix: str = "powerlaw"):
     the reordered RCT (r8192, one slice) and AXT UNC th = 1 plans on the
     permuted stencil; with "batch256", 5 SpMVs of the 256 banded matrices'
     block-diagonal RCT plan with its default 16-bit keys, then 5 with 32-bit
-    keys."""
+    keys; with "fwd_adj_rcm", 5 fused forward + adjoint passes of the
+    reordered RCT plan on the stencil structure with values N(1, 0.1)."""
     n = 32 << 20
     src, dst = ax.DeviceBuffer(8 * n), ax.DeviceBuffer(8 * n)
     L.call("axspmv_copy_f64", src.ptr, dst.ptr, n, None)
     L.call("axspmv_device_sync")
     del src, dst
+    if matrix == "fwd_adj_rcm":
+        m3, _ = W.config3()
+        m, x, b = W.config5(m3)
+        del m3
+        p = ax.Plan(m, ax.FMT_RCT, flags=ax.PLAN_TRANSPOSE | ax.PLAN_REORDER)
+        dx, db = ax.DeviceBuffer.from_array(x), ax.DeviceBuffer.from_array(b)
+        outs = [ax.DeviceBuffer(8 * m.nrows) for _ in range(3)] + [ax.DeviceBuffer(8 * m.nnz)]
+        for _ in range(5):
+            p.fwd_adj(dx, db, *outs, fused=True)
+        L.call("axspmv_device_sync")
+        log(f"pmc-probe: fwd_adj {p.info()}")
+        return
     if matrix == "batch256":
         mats = W.config4b()
         for kw in ({}, dict(flags=ax.PLAN_RCT_WIDE_KEY)):
@@ -702,7 +715,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true")
     ap.add_argument("--pmc-probe", nargs="?", const="powerlaw", choices=["powerlaw", "stencil", "spmm", "stencil_rcm",
-                                                                           "batch256"])
+                                                                           "batch256", "fwd_adj_rcm"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: SpMV here runs on a single MI355X; "
