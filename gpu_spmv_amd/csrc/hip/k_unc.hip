@@ -207,13 +207,13 @@ hipError_t run(const Src &src, const axspmv::launch::SegArgs &a, const double *x
     return hipGetLastError();
 }
 
-template <class VT>
+template <class VT, bool NT>
 hipError_t unc_h1_vt(const axspmv::launch::SegArgs &a, const double *x, double *y, hipStream_t s) {
     const uint32_t thw_log = __builtin_ctz(a.thw);
     const VT *vp = static_cast<const VT *>(a.vplane);
-    if (a.thw >= 4) return run(UncH1Src<VT, 4>{vp, a.cplane, a.rowsrc, thw_log - 2}, a, x, y, s);
-    if (a.thw == 2) return run(UncH1Src<VT, 2>{vp, a.cplane, a.rowsrc, 0}, a, x, y, s);
-    return run(UncH1Src<VT, 1>{vp, a.cplane, a.rowsrc, 0}, a, x, y, s);
+    if (a.thw >= 4) return run(UncH1Src<VT, 4, NT>{vp, a.cplane, a.rowsrc, thw_log - 2}, a, x, y, s);
+    if (a.thw == 2) return run(UncH1Src<VT, 2, NT>{vp, a.cplane, a.rowsrc, 0}, a, x, y, s);
+    return run(UncH1Src<VT, 1, NT>{vp, a.cplane, a.rowsrc, 0}, a, x, y, s);
 }
 
 template <class VT>
@@ -241,7 +241,8 @@ namespace axspmv {
 namespace launch {
 
 hipError_t axt_unc_h1(const SegArgs &a, const double *x, double *y, hipStream_t s) {
-    return a.vtype == 1 ? unc_h1_vt<float>(a, x, y, s) : unc_h1_vt<double>(a, x, y, s);
+    if (a.nt) return a.vtype == 1 ? unc_h1_vt<float, true>(a, x, y, s) : unc_h1_vt<double, true>(a, x, y, s);
+    return a.vtype == 1 ? unc_h1_vt<float, false>(a, x, y, s) : unc_h1_vt<double, false>(a, x, y, s);
 }
 
 hipError_t axt_unc_h(const SegArgs &a, const double *x, double *y, hipStream_t s) {

@@ -43,6 +43,8 @@ struct SegArgs {
                               // rows without entries in slice s), summed in
                               // slice order by a merge kernel
     uint32_t S;               // column slices (with part)
+    uint32_t nt = 0;          // AXT UNC th == 1: 1 streams the planes with
+                              // non-temporal loads
 };
 hipError_t axt_unc_h1(const SegArgs &a, const double *x, double *y, hipStream_t s);
 // As axt_unc_h1 on the interleaved layout: a.vplane is the FP64 ax array

@@ -49,59 +49,76 @@ __device__ __forceinline__ XWindow stage_window(double *xw, const Window w, cons
     return XWindow{xw, w.cmin, w.width};
 }
 
-template <class VT, int K>
+// One load of the matrix stream; NT: non-temporal (the plan asks for it
+// where the stream passes through the Infinity Cache without staying, so its
+// lines need not evict the x lines the gathers reuse).
+template <bool NT, class T>
+__device__ __forceinline__ T stream_load(const T *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+typedef double sf64x2 __attribute__((ext_vector_type(2)));
+typedef float sf32x2 __attribute__((ext_vector_type(2)));
+typedef float sf32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t su32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t su32x4 __attribute__((ext_vector_type(4)));
+
+template <class VT, int K, bool NT = false>
 struct VecLoad;
-template <>
-struct VecLoad<double, 4> {
+template <bool NT>
+struct VecLoad<double, 4, NT> {
     __device__ static void get(const double *p, double *v) {
-        const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+        const sf64x2 a = stream_load<NT>(reinterpret_cast<const sf64x2 *>(p)),
+                     b = stream_load<NT>(reinterpret_cast<const sf64x2 *>(p) + 1);
         v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
     }
 };
-template <>
-struct VecLoad<double, 2> {
+template <bool NT>
+struct VecLoad<double, 2, NT> {
     __device__ static void get(const double *p, double *v) {
-        const double2 a = *reinterpret_cast<const double2 *>(p);
+        const sf64x2 a = stream_load<NT>(reinterpret_cast<const sf64x2 *>(p));
         v[0] = a.x; v[1] = a.y;
     }
 };
-template <>
-struct VecLoad<double, 1> {
-    __device__ static void get(const double *p, double *v) { v[0] = *p; }
+template <bool NT>
+struct VecLoad<double, 1, NT> {
+    __device__ static void get(const double *p, double *v) { v[0] = stream_load<NT>(p); }
 };
-template <>
-struct VecLoad<float, 4> {
+template <bool NT>
+struct VecLoad<float, 4, NT> {
     __device__ static void get(const float *p, float *v) {
-        const float4 a = *reinterpret_cast<const float4 *>(p);
+        const sf32x4 a = stream_load<NT>(reinterpret_cast<const sf32x4 *>(p));
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
     }
 };
-template <>
-struct VecLoad<float, 2> {
+template <bool NT>
+struct VecLoad<float, 2, NT> {
     __device__ static void get(const float *p, float *v) {
-        const float2 a = *reinterpret_cast<const float2 *>(p);
+        const sf32x2 a = stream_load<NT>(reinterpret_cast<const sf32x2 *>(p));
         v[0] = a.x; v[1] = a.y;
     }
 };
-template <>
-struct VecLoad<float, 1> {
-    __device__ static void get(const float *p, float *v) { v[0] = *p; }
+template <bool NT>
+struct VecLoad<float, 1, NT> {
+    __device__ static void get(const float *p, float *v) { v[0] = stream_load<NT>(p); }
 };
-template <int K>
+template <int K, bool NT = false>
 __device__ __forceinline__ void load_cols(const uint32_t *p, uint32_t *c) {
     if constexpr (K == 4) {
-        const uint4 a = *reinterpret_cast<const uint4 *>(p);
+        const su32x4 a = stream_load<NT>(reinterpret_cast<const su32x4 *>(p));
         c[0] = a.x; c[1] = a.y; c[2] = a.z; c[3] = a.w;
     } else if constexpr (K == 2) {
-        const uint2 a = *reinterpret_cast<const uint2 *>(p);
+        const su32x2 a = stream_load<NT>(reinterpret_cast<const su32x2 *>(p));
         c[0] = a.x; c[1] = a.y;
     } else {
-        c[0] = *p;
+        c[0] = stream_load<NT>(p);
     }
 }
 
-// th == 1: unit u covers plane slots [u*K, u*K + K), tile (u*K)/thw.
-template <class VT, int K>
+// th == 1: unit u covers plane slots [u*K, u*K + K), tile (u*K)/thw.  NT:
+// the planes are streamed with non-temporal loads.
+template <class VT, int K, bool NT = false>
 struct UncH1Src {
     const VT *__restrict__ vp;
     const uint32_t *__restrict__ cp;
@@ -112,8 +129,8 @@ struct UncH1Src {
         const size_t q = size_t(u) * K;
         VT v[K];
         uint32_t c[K];
-        VecLoad<VT, K>::get(vp + q, v);
-        load_cols<K>(cp + q, c);
+        VecLoad<VT, K, NT>::get(vp + q, v);
+        load_cols<K, NT>(cp + q, c);
         double xv[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) xv[j] = x(c[j]);
@@ -128,8 +145,8 @@ struct UncH1Src {
         const size_t q = size_t(u) * K;
         VT v[K];
         uint32_t c[K];
-        VecLoad<VT, K>::get(vp + q, v);
-        load_cols<K>(cp + q, c);
+        VecLoad<VT, K, NT>::get(vp + q, v);
+        load_cols<K, NT>(cp + q, c);
 #pragma unroll
         for (int j = 0; j < K; ++j) xv[j] = xpad(x, c[j]);
         double acc = 0.0;
@@ -139,7 +156,7 @@ struct UncH1Src {
     }
     __device__ __forceinline__ void gather_x(uint32_t u, const double *__restrict__ x, double *xv) const {
         uint32_t c[K];
-        load_cols<K>(cp + size_t(u) * K, c);
+        load_cols<K, NT>(cp + size_t(u) * K, c);
 #pragma unroll
         for (int j = 0; j < K; ++j) xv[j] = xpad(x, c[j]);
     }

@@ -344,6 +344,22 @@ int upload_ax_layout(Plan &p, const axspmv_csr *a, const axspmv_axt *m, hipStrea
 // only gathers from its slice of x -- 2 MB at S = 4 on the 2^20-row matrix,
 // resident in its 4 MB L2.  Rows accumulate across slices with FP64 atomic
 // adds into a y cleared first.
+bool narrow_bands(const axspmv_csr *a, uint32_t rows, uint64_t cols);
+
+// AXT UNC th = 1 streams its planes with non-temporal loads where the stream
+// is larger than the 256 MB Infinity Cache (it comes from HBM on every SpMV
+// anyway) and the rows gather x from wide column ranges (so the x lines the
+// gathers share are what the cache should keep).  One box, per SpMV: the
+// 2^20-row power-law matrix 218.7 -> 209.2 us (4 column slices 168.5 ->
+// 157.8), the permuted 128^3 stencil 806.7 -> 784.5 us (slices 452.5 ->
+// 412.0); the stencil renumbered by reverse Cuthill-McKee, whose rows read
+// narrow bands, lost with them (197.3 -> 204.2 us).
+uint32_t axt_stream_nt(const axspmv_csr *a, uint64_t stream_bytes) {
+    if (stream_bytes <= (256ull << 20)) return 0;
+    const uint32_t rows = uint32_t(std::min<uint64_t>(16384, ceil_div(ceil_div(uint64_t(a->nrows), 256), 64) * 64));
+    return narrow_bands(a, std::max<uint32_t>(64, rows), (2ull << 20) / 8) ? 0 : 1;
+}
+
 int plan_axt_sliced(Plan &p, const axspmv_csr *a, hipStream_t s) {
     const uint32_t thw = p.o.thw, S = p.o.slices, n = a->nrows;
     if (S != 2 && S != 4 && S != 8) return fail(AXSPMV_E_UNSUPPORTED, "AXT column slices must be 2, 4 or 8, got %u", S);
@@ -434,6 +450,7 @@ int plan_axt_sliced(Plan &p, const axspmv_csr *a, hipStream_t s) {
     p.slices = S;
     p.slots = vcat.size();
     p.stream_bytes = (vs + 4) * p.slots + 4ull * scat.size() + 8ull * list.size();
+    p.seg_nt = axt_stream_nt(a, p.stream_bytes);
     p.launches = 2;
     p.kind = Plan::AXT_UNC_H1_SLICED;
     std::snprintf(p.name, sizeof p.name, "axt_unc_h1_thw%u_s%u%s", thw, S, det ? "_det" : "");
@@ -510,6 +527,7 @@ int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
         return AXSPMV_OK;
     }
     p.kind = Plan::AXT_UNC_H1;
+    p.seg_nt = axt_stream_nt(a, p.stream_bytes);
     std::snprintf(p.name, sizeof p.name, "axt_unc_h1_thw%u", thw);
     // Rows are contiguous in the planes: first slot of row r is con[r]/2.
     std::vector<uint32_t> first(a->nrows ? a->nrows : 1);
@@ -998,6 +1016,7 @@ launch::SegArgs seg_args(const Plan &p, const uint32_t *rowsrc) {
     g.vtype = p.o.vtype;
     g.block = p.o.block;
     g.win = p.d_win;
+    g.nt = p.seg_nt;
     return g;
 }
 
