@@ -216,13 +216,13 @@ hipError_t unc_h1_vt(const axspmv::launch::SegArgs &a, const double *x, double *
     return run(UncH1Src<VT, 1, NT>{vp, a.cplane, a.rowsrc, 0}, a, x, y, s);
 }
 
-template <class VT>
+template <class VT, bool NT>
 hipError_t unc_h_vt(const axspmv::launch::SegArgs &a, const double *x, double *y, hipStream_t s) {
     const uint32_t thw_log = __builtin_ctz(a.thw);
     const VT *vp = static_cast<const VT *>(a.vplane);
     switch (a.th) {
 #define AX_TH_CASE(T) \
-    case T: return run(UncHSrc<VT, T>{vp, a.cplane, a.rowsrc, thw_log, a.th}, a, x, y, s);
+    case T: return run(UncHSrc<VT, T, NT>{vp, a.cplane, a.rowsrc, thw_log, a.th}, a, x, y, s);
         AX_TH_CASE(2)
         AX_TH_CASE(4)
         AX_TH_CASE(6)
@@ -231,7 +231,7 @@ hipError_t unc_h_vt(const axspmv::launch::SegArgs &a, const double *x, double *y
         AX_TH_CASE(12)
         AX_TH_CASE(16)
 #undef AX_TH_CASE
-        default: return run(UncHSrc<VT, 0>{vp, a.cplane, a.rowsrc, thw_log, a.th}, a, x, y, s);
+        default: return run(UncHSrc<VT, 0, NT>{vp, a.cplane, a.rowsrc, thw_log, a.th}, a, x, y, s);
     }
 }
 
@@ -246,7 +246,8 @@ hipError_t axt_unc_h1(const SegArgs &a, const double *x, double *y, hipStream_t 
 }
 
 hipError_t axt_unc_h(const SegArgs &a, const double *x, double *y, hipStream_t s) {
-    return a.vtype == 1 ? unc_h_vt<float>(a, x, y, s) : unc_h_vt<double>(a, x, y, s);
+    if (a.nt) return a.vtype == 1 ? unc_h_vt<float, true>(a, x, y, s) : unc_h_vt<double, true>(a, x, y, s);
+    return a.vtype == 1 ? unc_h_vt<float, false>(a, x, y, s) : unc_h_vt<double, false>(a, x, y, s);
 }
 
 hipError_t axt_unc_h1_ax(const SegArgs &a, double *y, hipStream_t s) {

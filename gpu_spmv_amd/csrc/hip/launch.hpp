@@ -43,7 +43,7 @@ struct SegArgs {
                               // rows without entries in slice s), summed in
                               // slice order by a merge kernel
     uint32_t S;               // column slices (with part)
-    uint32_t nt = 0;          // AXT UNC th == 1: 1 streams the planes with
+    uint32_t nt = 0;          // AXT UNC: 1 streams the planes with
                               // non-temporal loads
 };
 hipError_t axt_unc_h1(const SegArgs &a, const double *x, double *y, hipStream_t s);

@@ -187,8 +187,9 @@ struct AxH1Src {
 };
 
 // th > 1: unit g is global tile column g; level f sits at
-// (g/thw)*thw*th + f*thw + g%thw.  TH > 0 fixes th at compile time.
-template <class VT, int TH>
+// (g/thw)*thw*th + f*thw + g%thw.  TH > 0 fixes th at compile time.  NT: the
+// planes are streamed with non-temporal loads.
+template <class VT, int TH, bool NT = false>
 struct UncHSrc {
     const VT *__restrict__ vp;
     const uint32_t *__restrict__ cp;
@@ -205,14 +206,16 @@ struct UncHSrc {
             VT v[TH];
 #pragma unroll
             for (int f = 0; f < TH; ++f) {
-                v[f] = vp[base + size_t(f) * thw];
-                xv[f] = x(cp[base + size_t(f) * thw]);
+                v[f] = stream_load<NT>(vp + base + size_t(f) * thw);
+                xv[f] = x(stream_load<NT>(cp + base + size_t(f) * thw));
             }
 #pragma unroll
             for (int f = 0; f < TH; ++f) acc = fma(double(v[f]), xv[f], acc);
         } else {
 #pragma unroll 4
-            for (uint32_t f = 0; f < h; ++f) acc = fma(double(vp[base + size_t(f) * thw]), x(cp[base + size_t(f) * thw]), acc);
+            for (uint32_t f = 0; f < h; ++f)
+                acc = fma(double(stream_load<NT>(vp + base + size_t(f) * thw)),
+                          x(stream_load<NT>(cp + base + size_t(f) * thw)), acc);
         }
         return acc;
     }

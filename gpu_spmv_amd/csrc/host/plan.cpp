@@ -346,14 +346,16 @@ int upload_ax_layout(Plan &p, const axspmv_csr *a, const axspmv_axt *m, hipStrea
 // adds into a y cleared first.
 bool narrow_bands(const axspmv_csr *a, uint32_t rows, uint64_t cols);
 
-// AXT UNC th = 1 streams its planes with non-temporal loads where the stream
-// is larger than the 256 MB Infinity Cache (it comes from HBM on every SpMV
+// AXT UNC streams its planes with non-temporal loads where the stream is
+// larger than the 256 MB Infinity Cache (it comes from HBM on every SpMV
 // anyway) and the rows gather x from wide column ranges (so the x lines the
 // gathers share are what the cache should keep).  One box, per SpMV: the
-// 2^20-row power-law matrix 218.7 -> 209.2 us (4 column slices 168.5 ->
-// 157.8), the permuted 128^3 stencil 806.7 -> 784.5 us (slices 452.5 ->
-// 412.0); the stencil renumbered by reverse Cuthill-McKee, whose rows read
-// narrow bands, lost with them (197.3 -> 204.2 us).
+// 2^20-row power-law matrix th = 1 218.7 -> 209.2 us (4 column slices 168.5
+// -> 157.8), th = 4 / 8 226.0 -> 218.3 / 238.0 -> 227.8 us; the permuted
+// 128^3 stencil th = 1 806.7 -> 784.5 us (slices 452.5 -> 412.0), th = 4 /
+// 8 805.4 -> 798.4 / 822.8 -> 813.0 us; the stencil renumbered by reverse
+// Cuthill-McKee, whose rows read narrow bands, lost with them (th = 1:
+// 197.3 -> 204.2 us).
 uint32_t axt_stream_nt(const axspmv_csr *a, uint64_t stream_bytes) {
     if (stream_bytes <= (256ull << 20)) return 0;
     const uint32_t rows = uint32_t(std::min<uint64_t>(16384, ceil_div(ceil_div(uint64_t(a->nrows), 256), 64) * 64));
@@ -522,6 +524,7 @@ int plan_axt(Plan &p, const axspmv_csr *a, hipStream_t s) {
     }
     p.stream_bytes = (vs + 4) * p.slots + 4ull * m->lenSEC + 8ull * p.nblocks;
     if (th != 1) {
+        p.seg_nt = axt_stream_nt(a, p.stream_bytes);
         p.kind = Plan::AXT_UNC_H;
         std::snprintf(p.name, sizeof p.name, "axt_unc_h%u_thw%u", th, thw);
         return AXSPMV_OK;
