@@ -101,8 +101,8 @@ __device__ __forceinline__ void put_vbar(uint32_t u, uint32_t r, double s, const
 }
 
 // v_bar[k] = y_bar[row(k)] * x[col(k)], one lane per unit of K slots.
-template <class VT, int K>
-__global__ void __launch_bounds__(256) vbar_kernel(UncH1Src<VT, K> src, uint32_t nunits, const double *__restrict__ x,
+template <class VT, int K, bool NT>
+__global__ void __launch_bounds__(256) vbar_kernel(UncH1Src<VT, K, NT> src, uint32_t nunits, const double *__restrict__ x,
                                                    const double *__restrict__ ybar, double *__restrict__ vbar,
                                                    VbarArgs va) {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -119,8 +119,8 @@ __global__ void __launch_bounds__(256) vbar_kernel(UncH1Src<VT, K> src, uint32_t
 // gathered x values are still in registers; the row's residual goes back to
 // its lanes with ballot + bpermute.  A block of more than 64 units is a
 // single long row and takes a second sweep for v_bar.
-template <class VT, int K>
-__global__ void __launch_bounds__(256) fused_kernel(UncH1Src<VT, K> src, const Block *__restrict__ blocks,
+template <class VT, int K, bool NT>
+__global__ void __launch_bounds__(256) fused_kernel(UncH1Src<VT, K, NT> src, const Block *__restrict__ blocks,
                                                     uint32_t nblocks, const uint32_t *__restrict__ empty,
                                                     uint32_t nempty, const double *__restrict__ x,
                                                     const double *__restrict__ b, double *__restrict__ y,
@@ -259,37 +259,50 @@ hipError_t axt_unc_h1_ax(const SegArgs &a, double *y, hipStream_t s) {
 }
 
 namespace {
-template <class VT, int K>
+template <class VT, int K, bool NT>
 hipError_t vbar_vk(const SegArgs &a, uint32_t ushift, uint32_t nunits, const double *x, const double *ybar,
                    double *vbar, const VbarArgs &va, hipStream_t s) {
-    const UncH1Src<VT, K> src{static_cast<const VT *>(a.vplane), a.cplane, a.rowsrc, ushift};
-    if (nunits) vbar_kernel<VT, K><<<(nunits + 255) / 256, 256, 0, s>>>(src, nunits, x, ybar, vbar, va);
+    const UncH1Src<VT, K, NT> src{static_cast<const VT *>(a.vplane), a.cplane, a.rowsrc, ushift};
+    if (nunits) vbar_kernel<VT, K, NT><<<(nunits + 255) / 256, 256, 0, s>>>(src, nunits, x, ybar, vbar, va);
     return hipGetLastError();
 }
-template <class VT, int K>
+template <class VT, int K, bool NT>
 hipError_t fused_vk(const SegArgs &a, uint32_t ushift, const double *x, const double *b, double *y, double *res,
                     double *vbar, const VbarArgs &va, hipStream_t s) {
-    const UncH1Src<VT, K> src{static_cast<const VT *>(a.vplane), a.cplane, a.rowsrc, ushift};
+    const UncH1Src<VT, K, NT> src{static_cast<const VT *>(a.vplane), a.cplane, a.rowsrc, ushift};
     const uint32_t waves = a.nblocks ? a.nblocks : 1;
-    fused_kernel<VT, K><<<(waves + 3) / 4, 256, 0, s>>>(src, a.blocks, a.nblocks, a.empty, a.nempty, x, b, y, res,
-                                                       vbar, va);
+    fused_kernel<VT, K, NT><<<(waves + 3) / 4, 256, 0, s>>>(src, a.blocks, a.nblocks, a.empty, a.nempty, x, b, y,
+                                                           res, vbar, va);
     return hipGetLastError();
 }
 template <class VT>
 hipError_t vbar_v(const SegArgs &a, uint32_t nunits, const double *x, const double *ybar, double *vbar,
                   const VbarArgs &va, hipStream_t s) {
     const uint32_t tl = __builtin_ctz(a.thw);
-    if (a.thw >= 4) return vbar_vk<VT, 4>(a, tl - 2, nunits, x, ybar, vbar, va, s);
-    if (a.thw == 2) return vbar_vk<VT, 2>(a, 0, nunits, x, ybar, vbar, va, s);
-    return vbar_vk<VT, 1>(a, 0, nunits, x, ybar, vbar, va, s);
+    if (a.nt) {
+        if (a.thw >= 4) return vbar_vk<VT, 4, true>(a, tl - 2, nunits, x, ybar, vbar, va, s);
+        if (a.thw == 2) return vbar_vk<VT, 2, true>(a, 0, nunits, x, ybar, vbar, va, s);
+        return vbar_vk<VT, 1, true>(a, 0, nunits, x, ybar, vbar, va, s);
+    }
+    if (a.thw >= 4) return vbar_vk<VT, 4, false>(a, tl - 2, nunits, x, ybar, vbar, va, s);
+    if (a.thw == 2) return vbar_vk<VT, 2, false>(a, 0, nunits, x, ybar, vbar, va, s);
+    return vbar_vk<VT, 1, false>(a, 0, nunits, x, ybar, vbar, va, s);
 }
+// The plan's non-temporal plane loads (a.nt) apply here too: forward +
+// adjoint on the 2^20-row power-law matrix 548.8 -> 531.0 us fused, 738.9
+// -> 733.0 us split (one box).
 template <class VT>
 hipError_t fused_v(const SegArgs &a, const double *x, const double *b, double *y, double *res, double *vbar,
                    const VbarArgs &va, hipStream_t s) {
     const uint32_t tl = __builtin_ctz(a.thw);
-    if (a.thw >= 4) return fused_vk<VT, 4>(a, tl - 2, x, b, y, res, vbar, va, s);
-    if (a.thw == 2) return fused_vk<VT, 2>(a, 0, x, b, y, res, vbar, va, s);
-    return fused_vk<VT, 1>(a, 0, x, b, y, res, vbar, va, s);
+    if (a.nt) {
+        if (a.thw >= 4) return fused_vk<VT, 4, true>(a, tl - 2, x, b, y, res, vbar, va, s);
+        if (a.thw == 2) return fused_vk<VT, 2, true>(a, 0, x, b, y, res, vbar, va, s);
+        return fused_vk<VT, 1, true>(a, 0, x, b, y, res, vbar, va, s);
+    }
+    if (a.thw >= 4) return fused_vk<VT, 4, false>(a, tl - 2, x, b, y, res, vbar, va, s);
+    if (a.thw == 2) return fused_vk<VT, 2, false>(a, 0, x, b, y, res, vbar, va, s);
+    return fused_vk<VT, 1, false>(a, 0, x, b, y, res, vbar, va, s);
 }
 }  // namespace
 
