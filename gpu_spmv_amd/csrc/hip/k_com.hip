@@ -94,8 +94,9 @@ __device__ __forceinline__ void combine(const LaneSum &L, bool last_chunk, uint3
     }
 }
 
-// th == 1: lane holds slots [p0, p0 + 4) of the CSR-ordered planes.
-template <class VT>
+// th == 1: lane holds slots [p0, p0 + 4) of the CSR-ordered planes.  NT:
+// the planes are streamed with non-temporal loads.
+template <class VT, bool NT>
 __device__ __forceinline__ LaneSum lane_com_h1(const ComArgs &a, uint32_t p0, uint32_t pend, uint32_t in_row,
                                                const double *__restrict__ x, double *__restrict__ y,
                                                uint32_t first_row, bool cont_in, double *head) {
@@ -107,19 +108,20 @@ __device__ __forceinline__ LaneSum lane_com_h1(const ComArgs &a, uint32_t p0, ui
     uint32_t hdr[4], col[4];
     VT v[4];
     if (p0 + 4 <= pend) {
-        const uint4 h4 = *reinterpret_cast<const uint4 *>(a.hdr + p0);
-        const uint4 c4 = *reinterpret_cast<const uint4 *>(a.cplane + p0);
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 h4 = stream_load<NT>(reinterpret_cast<const u32x4 *>(a.hdr + p0));
+        const u32x4 c4 = stream_load<NT>(reinterpret_cast<const u32x4 *>(a.cplane + p0));
         hdr[0] = h4.x; hdr[1] = h4.y; hdr[2] = h4.z; hdr[3] = h4.w;
         col[0] = c4.x; col[1] = c4.y; col[2] = c4.z; col[3] = c4.w;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) { v[j] = vp[p0 + j]; act[j] = true; }
+        for (int j = 0; j < 4; ++j) { v[j] = stream_load<NT>(vp + p0 + j); act[j] = true; }
     } else {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             act[j] = p0 + j < pend;
-            hdr[j] = act[j] ? a.hdr[p0 + j] : 0u;
-            col[j] = act[j] ? a.cplane[p0 + j] : 0u;
-            v[j] = act[j] ? vp[p0 + j] : VT(0);
+            hdr[j] = act[j] ? stream_load<NT>(a.hdr + p0 + j) : 0u;
+            col[j] = act[j] ? stream_load<NT>(a.cplane + p0 + j) : 0u;
+            v[j] = act[j] ? stream_load<NT>(vp + p0 + j) : VT(0);
         }
     }
 #pragma unroll
@@ -152,7 +154,7 @@ __device__ __forceinline__ LaneSum lane_com_h1(const ComArgs &a, uint32_t p0, ui
     return L;
 }
 
-template <class VT>
+template <class VT, bool NT>
 __global__ void __launch_bounds__(256) com_h1_kernel(ComArgs a, const double *__restrict__ x, double *__restrict__ y) {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (wave >= a.nwaves) return;
@@ -182,7 +184,7 @@ __global__ void __launch_bounds__(256) com_h1_kernel(ComArgs a, const double *__
         excl = max(excl, crow_hdr);
         const uint32_t in_row = excl ? excl - 1 : 0;
         LaneSum L{NOROW, NOROW, 0.0, 0.0, false, false};
-        if (p0 < w.end) L = lane_com_h1<VT>(a, p0, w.end, in_row, x, y, first_row, cont_in, head);
+        if (p0 < w.end) L = lane_com_h1<VT, NT>(a, p0, w.end, in_row, x, y, first_row, cont_in, head);
         const bool last = c0 + 256 >= w.end;
         combine(L, last, crow, cval, first_row, cont_in, cont_out, y, head, tail);
         crow_hdr = max(crow_hdr, readlane_u32(incl, 63));
@@ -190,7 +192,7 @@ __global__ void __launch_bounds__(256) com_h1_kernel(ComArgs a, const double *__
 }
 
 // th > 1: lane holds global tile column g (slots g*th .. g*th + th - 1).
-template <class VT, int TH>
+template <class VT, int TH, bool NT>
 __global__ void __launch_bounds__(256) com_h_kernel(ComArgs a, const double *__restrict__ x, double *__restrict__ y) {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (wave >= a.nwaves) return;
@@ -222,9 +224,9 @@ __global__ void __launch_bounds__(256) com_h_kernel(ComArgs a, const double *__r
                 for (uint32_t f = 0; f < NB; ++f) {
                     const size_t q = base + size_t(f) * thw;
                     const bool ok = f < nf;
-                    hd[f] = ok ? a.hdr[q] : 0u;
-                    cc[f] = ok ? a.cplane[q] : 0u;
-                    vv[f] = ok ? vp[q] : VT(0);
+                    hd[f] = ok ? stream_load<NT>(a.hdr + q) : 0u;
+                    cc[f] = ok ? stream_load<NT>(a.cplane + q) : 0u;
+                    vv[f] = ok ? stream_load<NT>(vp + q) : VT(0);
                 }
 #pragma unroll
                 for (uint32_t f = 0; f < NB; ++f) pr[f] = f < nf ? double(vv[f]) * x[cc[f]] : 0.0;
@@ -238,8 +240,8 @@ __global__ void __launch_bounds__(256) com_h_kernel(ComArgs a, const double *__r
                     prod = pr[f];
                 } else {
                     const size_t q = base + size_t(f) * thw;
-                    h = a.hdr[q];
-                    prod = double(vp[q]) * x[a.cplane[q]];
+                    h = stream_load<NT>(a.hdr + q);
+                    prod = double(stream_load<NT>(vp + q)) * x[stream_load<NT>(a.cplane + q)];
                 }
                 if (f == 0 || h != 0) cur = h >> a.log;
                 if (!L.any) {
@@ -274,24 +276,32 @@ __global__ void __launch_bounds__(256) com_fixup(const uint32_t *__restrict__ sp
     y[row] = acc;
 }
 
+template <class VT, bool NT>
+void com_launch(const ComArgs &a, const double *x, double *y, hipStream_t s) {
+    const unsigned grid = (a.nwaves + 3) / 4;
+    if (a.th == 1) {
+        com_h1_kernel<VT, NT><<<grid, 256, 0, s>>>(a, x, y);
+        return;
+    }
+    switch (a.th) {
+#define AX_TH(T) \
+    case T: com_h_kernel<VT, T, NT><<<grid, 256, 0, s>>>(a, x, y); break;
+        AX_TH(2)
+        AX_TH(4)
+        AX_TH(8)
+        AX_TH(16)
+#undef AX_TH
+        default: com_h_kernel<VT, 0, NT><<<grid, 256, 0, s>>>(a, x, y);
+    }
+}
+
 template <class VT>
 hipError_t com_vt(const ComArgs &a, const double *x, double *y, hipStream_t s) {
-    const unsigned grid = (a.nwaves + 3) / 4;
     if (a.nwaves) {
-        if (a.th == 1) {
-            com_h1_kernel<VT><<<grid, 256, 0, s>>>(a, x, y);
-        } else {
-            switch (a.th) {
-#define AX_TH(T) \
-    case T: com_h_kernel<VT, T><<<grid, 256, 0, s>>>(a, x, y); break;
-                AX_TH(2)
-                AX_TH(4)
-                AX_TH(8)
-                AX_TH(16)
-#undef AX_TH
-                default: com_h_kernel<VT, 0><<<grid, 256, 0, s>>>(a, x, y);
-            }
-        }
+        if (a.nt)
+            com_launch<VT, true>(a, x, y, s);
+        else
+            com_launch<VT, false>(a, x, y, s);
     }
     if (a.nempty) {
         // rows without entries never appear in a header

@@ -22,7 +22,8 @@ namespace {
 using namespace axspmv::dev;
 using axspmv::launch::K1Item;
 
-template <class VT>
+// NT: val and col are streamed with non-temporal loads.
+template <class VT, bool NT>
 __global__ void __launch_bounds__(256) k1_kernel(const K1Item *__restrict__ items, uint32_t nitems, uint32_t chunk,
                                                  const VT *__restrict__ val, const uint32_t *__restrict__ col,
                                                  const uint32_t *__restrict__ permi, uint32_t nrows,
@@ -41,9 +42,11 @@ __global__ void __launch_bounds__(256) k1_kernel(const K1Item *__restrict__ item
     uint32_t j = w.j0;
     for (; j + 4 <= w.j1; j += 4) {
         const uint64_t q = base + uint64_t(j) * chunk;
-        const VT v0 = val[q], v1 = val[q + chunk], v2 = val[q + 2 * chunk], v3 = val[q + 3 * chunk];
-        const double x0 = xpad(x, col[q]), x1 = xpad(x, col[q + chunk]), x2 = xpad(x, col[q + 2 * chunk]),
-                     x3 = xpad(x, col[q + 3 * chunk]);
+        const VT v0 = stream_load<NT>(val + q), v1 = stream_load<NT>(val + q + chunk),
+                 v2 = stream_load<NT>(val + q + 2 * chunk), v3 = stream_load<NT>(val + q + 3 * chunk);
+        const double x0 = xpad(x, stream_load<NT>(col + q)), x1 = xpad(x, stream_load<NT>(col + q + chunk)),
+                     x2 = xpad(x, stream_load<NT>(col + q + 2 * chunk)),
+                     x3 = xpad(x, stream_load<NT>(col + q + 3 * chunk));
         acc = fma(double(v0), x0, acc);
         acc = fma(double(v1), x1, acc);
         acc = fma(double(v2), x2, acc);
@@ -51,7 +54,7 @@ __global__ void __launch_bounds__(256) k1_kernel(const K1Item *__restrict__ item
     }
     for (; j < w.j1; ++j) {
         const uint64_t q = base + uint64_t(j) * chunk;
-        acc = fma(double(val[q]), xpad(x, col[q]), acc);
+        acc = fma(double(stream_load<NT>(val + q)), xpad(x, stream_load<NT>(col + q)), acc);
     }
     if (w.slot == NOROW) {
         if (gid < nrows) y[permi[gid]] = acc;
@@ -111,7 +114,11 @@ template <class VT>
 hipError_t k1_vt(const axspmv::launch::K1Args &a, const double *x, double *y, hipStream_t s) {
     const uint32_t per_wave = 64 / a.chunk;
     const uint32_t waves = (a.nitems + per_wave - 1) / per_wave;
-    if (waves) k1_kernel<VT><<<(waves + 3) / 4, 256, 0, s>>>(a.items, a.nitems, a.chunk, static_cast<const VT *>(a.val),
+    if (waves && a.nt)
+        k1_kernel<VT, true><<<(waves + 3) / 4, 256, 0, s>>>(a.items, a.nitems, a.chunk, static_cast<const VT *>(a.val),
+                                                            a.col, a.permi, a.nrows, x, y, a.part);
+    else if (waves)
+        k1_kernel<VT, false><<<(waves + 3) / 4, 256, 0, s>>>(a.items, a.nitems, a.chunk, static_cast<const VT *>(a.val),
                                                              a.col, a.permi, a.nrows, x, y, a.part);
     if (a.nsplit) {
         const uint32_t th = a.nsplit * a.chunk;

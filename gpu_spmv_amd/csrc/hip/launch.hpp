@@ -170,6 +170,7 @@ struct K1Args {
     const void *val;
     const uint32_t *col, *permi;
     double *part;
+    uint32_t nt = 0;  // 1: val and col streamed with non-temporal loads
 };
 hipError_t k1(const K1Args &a, const double *x, double *y, hipStream_t s);
 
@@ -200,6 +201,7 @@ struct ComArgs {
     uint32_t nspan;
     const uint32_t *empty;      // rows without entries
     uint32_t nempty;
+    uint32_t nt = 0;            // 1: planes streamed with non-temporal loads
 };
 hipError_t axt_com(const ComArgs &a, const double *x, double *y, hipStream_t s);
 

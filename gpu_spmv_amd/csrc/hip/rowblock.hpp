@@ -49,15 +49,6 @@ __device__ __forceinline__ XWindow stage_window(double *xw, const Window w, cons
     return XWindow{xw, w.cmin, w.width};
 }
 
-// One load of the matrix stream; NT: non-temporal (the plan asks for it
-// where the stream passes through the Infinity Cache without staying, so its
-// lines need not evict the x lines the gathers reuse).
-template <bool NT, class T>
-__device__ __forceinline__ T stream_load(const T *p) {
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-
 typedef double sf64x2 __attribute__((ext_vector_type(2)));
 typedef float sf32x2 __attribute__((ext_vector_type(2)));
 typedef float sf32x4 __attribute__((ext_vector_type(4)));

@@ -21,6 +21,15 @@ __device__ __forceinline__ double xpad(const double *__restrict__ x, uint32_t c)
     return c == 0xffffffffu ? 0.0 : x[c];
 }
 
+// One load of a matrix stream; NT: non-temporal (the plans ask for it where
+// the stream passes through the Infinity Cache without staying, so its
+// lines need not evict the x lines the gathers reuse).
+template <bool NT, class T>
+__device__ __forceinline__ T stream_load(const T *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 // DPP control words (GFX9 encoding).
 enum : int {
     DPP_ROW_SHR1 = 0x111,

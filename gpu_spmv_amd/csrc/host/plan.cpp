@@ -224,6 +224,8 @@ uint32_t row_of(const axspmv_csr *a, uint32_t q) {
     return uint32_t(std::upper_bound(a->row, a->row + a->nrows + 1, q) - a->row) - 1;
 }
 
+bool wide_gathers(const axspmv_csr *a);
+
 int plan_axt_com(Plan &p, const axspmv_csr *a, hipStream_t s) {
     const uint32_t thw = p.o.thw, th = p.o.th;
     axspmv_axt *m = nullptr;
@@ -299,9 +301,17 @@ int plan_axt_com(Plan &p, const axspmv_csr *a, hipStream_t s) {
     c.nspan = uint32_t(span.size() / 3);
     c.empty = dempty;
     c.nempty = uint32_t(empty.size());
-    p.com = c;
     const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
     p.stream_bytes = (vs + 8) * p.slots + sizeof(launch::ComWave) * waves.size();
+    // Non-temporal plane loads where the stream (more than 128 MB) would
+    // share the Infinity Cache with the x lines the gathers reuse; th = 1 on
+    // narrow bands keeps ordinary loads, as AXT UNC th = 1 does.  One box, per
+    // SpMV, th = 1 / th = 4: the 2^20-row power-law matrix 271.2 -> 255.4 /
+    // 251.5 -> 235.1 us, the permuted 128^3 stencil 867 -> 839.5 / 826 ->
+    // 803 us, the same renumbered by reverse Cuthill-McKee 306.5 -> 311.5
+    // (hence the exception) / 281.5 -> 266.3 us.
+    c.nt = p.stream_bytes > (128ull << 20) && (th > 1 || wide_gathers(a)) ? 1 : 0;
+    p.com = c;
     p.nblocks = c.nwaves;
     p.launches = 1 + (c.nspan > 0) + (c.nempty > 0);
     p.kind = Plan::AXT_COM;
@@ -356,10 +366,12 @@ bool narrow_bands(const axspmv_csr *a, uint32_t rows, uint64_t cols);
 // 8 805.4 -> 798.4 / 822.8 -> 813.0 us; the stencil renumbered by reverse
 // Cuthill-McKee, whose rows read narrow bands, lost with them (th = 1:
 // 197.3 -> 204.2 us).
-uint32_t axt_stream_nt(const axspmv_csr *a, uint64_t stream_bytes) {
-    if (stream_bytes <= (256ull << 20)) return 0;
+bool wide_gathers(const axspmv_csr *a) {
     const uint32_t rows = uint32_t(std::min<uint64_t>(16384, ceil_div(ceil_div(uint64_t(a->nrows), 256), 64) * 64));
-    return narrow_bands(a, std::max<uint32_t>(64, rows), (2ull << 20) / 8) ? 0 : 1;
+    return !narrow_bands(a, std::max<uint32_t>(64, rows), (2ull << 20) / 8);
+}
+uint32_t axt_stream_nt(const axspmv_csr *a, uint64_t stream_bytes) {
+    return stream_bytes > (256ull << 20) && wide_gathers(a) ? 1 : 0;
 }
 
 int plan_axt_sliced(Plan &p, const axspmv_csr *a, hipStream_t s) {
@@ -603,6 +615,13 @@ int plan_k1(Plan &p, const axspmv_csr *a, hipStream_t s) {
                           val, col, permi, part};
     p.slots = m->lenVC;
     p.stream_bytes = (vs + 4) * uint64_t(m->lenVC) + 4ull * a->nrows + sizeof(launch::K1Item) * items.size();
+    // A stream of more than 128 MB shares the Infinity Cache with the x
+    // lines the gathers reuse (a chunk's lanes are rows of one length, not
+    // neighbours): non-temporal loads keep the cache for x.  One box, per
+    // SpMV: the 2^20-row power-law matrix 226.5 -> 214.4 us, the permuted
+    // 128^3 stencil 740 -> 719.6 us, the same renumbered by reverse
+    // Cuthill-McKee 209.2 -> 193.2 us.
+    p.k1.nt = p.stream_bytes > (128ull << 20) ? 1 : 0;
     p.launches = split.empty() ? 1 : 2;
     p.nblocks = uint32_t(items.size());
     p.kind = Plan::K1;
