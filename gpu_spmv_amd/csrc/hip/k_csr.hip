@@ -21,7 +21,8 @@ __global__ void __launch_bounds__(256) csr_naive_kernel(uint32_t nrows, const VT
     y[r] = acc;
 }
 
-template <class VT, int G>
+// NT: val and col are streamed with non-temporal loads.
+template <class VT, int G, bool NT>
 __global__ void __launch_bounds__(256) csr_vector_kernel(uint32_t nrows, const VT *__restrict__ val,
                                                          const uint32_t *__restrict__ col,
                                                          const uint32_t *__restrict__ row, uint32_t long_len,
@@ -32,12 +33,13 @@ __global__ void __launch_bounds__(256) csr_vector_kernel(uint32_t nrows, const V
     const uint32_t b = row[r], e = row[r + 1];
     if (e - b > long_len) return;  // csr_long's row
     double acc = 0.0;
-    for (uint32_t j = b + g; j < e; j += G) acc = fma(double(val[j]), x[col[j]], acc);
+    for (uint32_t j = b + g; j < e; j += G)
+        acc = fma(double(axspmv::dev::stream_load<NT>(val + j)), x[axspmv::dev::stream_load<NT>(col + j)], acc);
     acc = axspmv::dev::group_sum_f64<G>(acc);
     if (g == 0) y[r] = acc;
 }
 
-template <class VT>
+template <class VT, bool NT>
 __global__ void __launch_bounds__(256) csr_long_kernel(const uint32_t *__restrict__ rows, uint32_t nlong,
                                                        const VT *__restrict__ val, const uint32_t *__restrict__ col,
                                                        const uint32_t *__restrict__ row, const double *__restrict__ x,
@@ -46,19 +48,20 @@ __global__ void __launch_bounds__(256) csr_long_kernel(const uint32_t *__restric
     if (w >= nlong) return;
     const uint32_t r = rows[w];
     double acc = 0.0;
-    for (uint32_t j = row[r] + lane; j < row[r + 1]; j += 64) acc = fma(double(val[j]), x[col[j]], acc);
+    for (uint32_t j = row[r] + lane; j < row[r + 1]; j += 64)
+        acc = fma(double(axspmv::dev::stream_load<NT>(val + j)), x[axspmv::dev::stream_load<NT>(col + j)], acc);
     acc = axspmv::dev::group_sum_f64<64>(acc);
     if (lane == 0) y[r] = acc;
 }
 
-template <class VT>
+template <class VT, bool NT>
 hipError_t csr_vector_vt(uint32_t nrows, const VT *val, const uint32_t *col, const uint32_t *row, uint32_t lanes,
                          uint32_t long_len, const uint32_t *longrows, uint32_t nlong, const double *x, double *y,
                          hipStream_t s) {
     const unsigned grid = unsigned((uint64_t(nrows) * lanes + 255) / 256);
     switch (lanes) {
 #define AX_G_CASE(G) \
-    case G: csr_vector_kernel<VT, G><<<grid, 256, 0, s>>>(nrows, val, col, row, long_len, x, y); break;
+    case G: csr_vector_kernel<VT, G, NT><<<grid, 256, 0, s>>>(nrows, val, col, row, long_len, x, y); break;
         AX_G_CASE(1)
         AX_G_CASE(2)
         AX_G_CASE(4)
@@ -69,7 +72,7 @@ hipError_t csr_vector_vt(uint32_t nrows, const VT *val, const uint32_t *col, con
 #undef AX_G_CASE
         default: return hipErrorInvalidValue;
     }
-    if (nlong) csr_long_kernel<VT><<<(nlong + 3) / 4, 256, 0, s>>>(longrows, nlong, val, col, row, x, y);
+    if (nlong) csr_long_kernel<VT, NT><<<(nlong + 3) / 4, 256, 0, s>>>(longrows, nlong, val, col, row, x, y);
     return hipGetLastError();
 }
 
@@ -136,12 +139,15 @@ hipError_t residual(uint32_t n, const double *y, const double *b, double *r, hip
 
 hipError_t csr_vector(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,
                       uint32_t lanes, uint32_t long_len, const uint32_t *longrows, uint32_t nlong, const double *x,
-                      double *y, hipStream_t s) {
+                      double *y, hipStream_t s, uint32_t nt) {
     if (!nrows) return hipSuccess;
-    if (vtype == 1)
-        return csr_vector_vt(nrows, static_cast<const float *>(val), col, row, lanes, long_len, longrows, nlong, x, y,
-                             s);
-    return csr_vector_vt(nrows, static_cast<const double *>(val), col, row, lanes, long_len, longrows, nlong, x, y, s);
+    const float *vf = static_cast<const float *>(val);
+    const double *vd = static_cast<const double *>(val);
+    if (nt)
+        return vtype == 1 ? csr_vector_vt<float, true>(nrows, vf, col, row, lanes, long_len, longrows, nlong, x, y, s)
+                          : csr_vector_vt<double, true>(nrows, vd, col, row, lanes, long_len, longrows, nlong, x, y, s);
+    return vtype == 1 ? csr_vector_vt<float, false>(nrows, vf, col, row, lanes, long_len, longrows, nlong, x, y, s)
+                      : csr_vector_vt<double, false>(nrows, vd, col, row, lanes, long_len, longrows, nlong, x, y, s);
 }
 
 hipError_t csr_naive(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,

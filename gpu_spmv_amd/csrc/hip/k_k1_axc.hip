@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256) k1_fixup(const uint32_t *__restrict__ spl
 
 // AXC: G lanes per row; rows listed in `longrows` are skipped here and done
 // by axc_long (one wave per row).
-template <class VT, int G>
+template <class VT, int G, bool NT>
 __global__ void __launch_bounds__(256) axc_kernel(uint32_t nrows, const uint32_t *__restrict__ brp,
                                                   const VT *__restrict__ vp, const uint32_t *__restrict__ mapx,
                                                   uint32_t long_slots, const double *__restrict__ x,
@@ -90,12 +90,12 @@ __global__ void __launch_bounds__(256) axc_kernel(uint32_t nrows, const uint32_t
     const uint32_t b = brp[r] >> 1, e = brp[r + 1] >> 1;
     if (e - b > long_slots) return;
     double acc = 0.0;
-    for (uint32_t q = b + g; q < e; q += G) acc = fma(double(vp[q]), xpad(x, mapx[q]), acc);
+    for (uint32_t q = b + g; q < e; q += G) acc = fma(double(stream_load<NT>(vp + q)), xpad(x, stream_load<NT>(mapx + q)), acc);
     acc = group_sum_f64<G>(acc);
     if (g == 0) y[r] = acc;
 }
 
-template <class VT>
+template <class VT, bool NT>
 __global__ void __launch_bounds__(256) axc_long(const uint32_t *__restrict__ rows, uint32_t nlong,
                                                 const uint32_t *__restrict__ brp, const VT *__restrict__ vp,
                                                 const uint32_t *__restrict__ mapx, const double *__restrict__ x,
@@ -105,7 +105,8 @@ __global__ void __launch_bounds__(256) axc_long(const uint32_t *__restrict__ row
     const uint32_t r = rows[wave];
     const uint32_t b = brp[r] >> 1, e = brp[r + 1] >> 1;
     double acc = 0.0;
-    for (uint32_t q = b + lane; q < e; q += 64) acc = fma(double(vp[q]), xpad(x, mapx[q]), acc);
+    for (uint32_t q = b + lane; q < e; q += 64)
+        acc = fma(double(stream_load<NT>(vp + q)), xpad(x, stream_load<NT>(mapx + q)), acc);
     acc = group_sum_f64<64>(acc);
     if (lane == 0) y[r] = acc;
 }
@@ -127,14 +128,14 @@ hipError_t k1_vt(const axspmv::launch::K1Args &a, const double *x, double *y, hi
     return hipGetLastError();
 }
 
-template <class VT>
+template <class VT, bool NT>
 hipError_t axc_vt(const axspmv::launch::AxcArgs &a, const double *x, double *y, hipStream_t s) {
     const VT *vp = static_cast<const VT *>(a.vplane);
     const uint64_t th = uint64_t(a.nrows) * a.hbs;
     const unsigned grid = unsigned((th + 255) / 256);
     if (a.nrows) switch (a.hbs) {
 #define AX_G(G) \
-    case G: axc_kernel<VT, G><<<grid, 256, 0, s>>>(a.nrows, a.brp, vp, a.mapx, a.long_slots, x, y); break;
+    case G: axc_kernel<VT, G, NT><<<grid, 256, 0, s>>>(a.nrows, a.brp, vp, a.mapx, a.long_slots, x, y); break;
         AX_G(1)
         AX_G(2)
         AX_G(4)
@@ -145,7 +146,7 @@ hipError_t axc_vt(const axspmv::launch::AxcArgs &a, const double *x, double *y, 
 #undef AX_G
         default: return hipErrorInvalidValue;
     }
-    if (a.nlong) axc_long<VT><<<(a.nlong + 3) / 4, 256, 0, s>>>(a.longrows, a.nlong, a.brp, vp, a.mapx, x, y);
+    if (a.nlong) axc_long<VT, NT><<<(a.nlong + 3) / 4, 256, 0, s>>>(a.longrows, a.nlong, a.brp, vp, a.mapx, x, y);
     return hipGetLastError();
 }
 
@@ -159,7 +160,8 @@ hipError_t k1(const K1Args &a, const double *x, double *y, hipStream_t s) {
 }
 
 hipError_t axc(const AxcArgs &a, const double *x, double *y, hipStream_t s) {
-    return a.vtype == 1 ? axc_vt<float>(a, x, y, s) : axc_vt<double>(a, x, y, s);
+    if (a.nt) return a.vtype == 1 ? axc_vt<float, true>(a, x, y, s) : axc_vt<double, true>(a, x, y, s);
+    return a.vtype == 1 ? axc_vt<float, false>(a, x, y, s) : axc_vt<double, false>(a, x, y, s);
 }
 
 }  // namespace launch

@@ -153,7 +153,7 @@ hipError_t csr_naive(uint32_t nrows, const void *val, uint32_t vtype, const uint
 // are listed in longrows and done one wave per row
 hipError_t csr_vector(uint32_t nrows, const void *val, uint32_t vtype, const uint32_t *col, const uint32_t *row,
                       uint32_t lanes, uint32_t long_len, const uint32_t *longrows, uint32_t nlong, const double *x,
-                      double *y, hipStream_t s);
+                      double *y, hipStream_t s, uint32_t nt = 0);
 
 // K1: a work item is steps [j0, j1) of one chunk; slot == 0xffffffff writes
 // y directly, otherwise partial sums go to part[slot*chunk + r] and the
@@ -182,6 +182,7 @@ struct AxcArgs {
     const void *vplane;
     const uint32_t *longrows;
     uint32_t nlong;
+    uint32_t nt = 0;  // 1: vplane and mapx streamed with non-temporal loads
 };
 hipError_t axc(const AxcArgs &a, const double *x, double *y, hipStream_t s);
 

@@ -161,6 +161,9 @@ uint32_t lanes_for_mean(const axspmv_csr *a) {
     return g;
 }
 
+// True when the rows gather x from wide column ranges (not narrow bands).
+bool wide_gathers(const axspmv_csr *a);
+
 int plan_csr(Plan &p, const axspmv_csr *a, hipStream_t s) {
     const size_t vs = p.o.vtype == AXSPMV_F32 ? 4 : 8;
     p.d_val = p.dalloc(vs * a->nnz);
@@ -182,6 +185,12 @@ int plan_csr(Plan &p, const axspmv_csr *a, hipStream_t s) {
         const uint32_t g = p.o.hbs ? p.o.hbs : lanes_for_mean(a);
         if (!is_pow2(g) || g > 64) return fail(AXSPMV_E_UNSUPPORTED, "CSR vector: lanes per row %u not a power of two <= 64", g);
         p.lanes = g;
+        // Non-temporal val / col loads for streams of more than twice the
+        // Infinity Cache gathering wide: one box, the permuted 128^3 stencil
+        // (669 MB) 797 -> 780 us; the 2^20-row power-law matrix (256 MB)
+        // lost with them (259 -> 266 us), the stencil renumbered by reverse
+        // Cuthill-McKee (narrow bands) stays level.
+        p.seg_nt = p.stream_bytes > (512ull << 20) && wide_gathers(a) ? 1 : 0;
         p.kind = Plan::CSR_VECTOR;
         std::snprintf(p.name, sizeof p.name, "csr_vector_g%u", g);
         p.launches = 2;
@@ -223,8 +232,6 @@ int upload_blocks(Plan &p, const std::vector<launch::Block> &blocks, const std::
 uint32_t row_of(const axspmv_csr *a, uint32_t q) {
     return uint32_t(std::upper_bound(a->row, a->row + a->nrows + 1, q) - a->row) - 1;
 }
-
-bool wide_gathers(const axspmv_csr *a);
 
 int plan_axt_com(Plan &p, const axspmv_csr *a, hipStream_t s) {
     const uint32_t thw = p.o.thw, th = p.o.th;
@@ -670,6 +677,12 @@ int plan_axc(Plan &p, const axspmv_csr *a, hipStream_t s) {
     p.axc = launch::AxcArgs{a->nrows, G, p.o.vtype, long_slots, brp, mapx, vp, lr, uint32_t(longrows.size())};
     p.slots = nslots;
     p.stream_bytes = (vs + 4) * nslots + 4ull * m->lenBRP;
+    // Non-temporal plane loads for streams of more than twice the Infinity
+    // Cache gathering wide: one box, the permuted 128^3 stencil hbs 4 / 64
+    // 1214.7 -> 1163.0 / 945.7 -> 914.3 us; on the 2^20-row power-law matrix
+    // (streams of 300-400 MB) hbs 4 lost with them (301 -> 347 us) where hbs
+    // 64 gained (400 -> 384 us); renumbered (narrow bands) level.
+    p.axc.nt = p.stream_bytes > (512ull << 20) && wide_gathers(a) ? 1 : 0;
     p.launches = longrows.empty() ? 1 : 2;
     p.kind = Plan::AXC;
     std::snprintf(p.name, sizeof p.name, "axc_hbs%u", G);
@@ -1301,7 +1314,7 @@ hipError_t plan_launch(axspmv_plan *p, const double *x, double *y, hipStream_t s
             return launch::csr_naive(p->nrows, p->d_val, p->o.vtype, p->d_col, p->d_row, x, y, p->o.block, s);
         case Plan::CSR_VECTOR:
             return launch::csr_vector(p->nrows, p->d_val, p->o.vtype, p->d_col, p->d_row, p->lanes, p->spmm.long_slots,
-                                      p->spmm.longrows, p->spmm.nlong, x, y, s);
+                                      p->spmm.longrows, p->spmm.nlong, x, y, s, p->seg_nt);
         case Plan::AXT_UNC_H1: return launch::axt_unc_h1(seg_args(*p, p->d_sec), x, y, s);
         case Plan::AXT_UNC_H1_SLICED: {
             launch::SegArgs g = seg_args(*p, p->d_sec);

@@ -54,7 +54,7 @@ struct axspmv_plan {
     // AXT UNC th == 1: blocks of <= 64 units for the fused adjoint pass
     axspmv::launch::Block *d_fblocks = nullptr;
     uint32_t nfblocks = 0, nunits = 0;
-    uint32_t seg_nt = 0;  // AXT UNC: planes streamed with non-temporal loads
+    uint32_t seg_nt = 0;  // AXT UNC, CSR vector: stream read with non-temporal loads
     // AX layout (AXSPMV_PLAN_AX_LAYOUT): interleaved ax array and the column
     // of every value slot (0xffffffff for padding)
     double *d_ax = nullptr;
