@@ -71,6 +71,27 @@ def test_stencil_config3_full_size(stencil, name, fmt, kw):
 
 
 @pytest.mark.timeout(300)
+def test_long_rows_full_size(stencil):
+    """The permuted stencil (a stream of more than 512 MB gathering wide: the
+    CSR vector and AXC plans read it with non-temporal loads) with 8 rows of
+    5000 entries appended: their one-wave-per-row kernels take the same
+    loads."""
+    s, _, _ = stencil
+    rng = np.random.default_rng(31)
+    extra = [np.sort(rng.choice(s.ncols, 5000, replace=False)).astype(np.uint32) for _ in range(8)]
+    col = np.concatenate([s.col] + extra)
+    val = np.concatenate([s.val, rng.standard_normal(8 * 5000)])
+    row = np.concatenate([s.row, s.row[-1] + 5000 * np.arange(1, 9, dtype=np.uint64)]).astype(np.uint32)
+    m = ax.CsrMatrix(row, col, val, ncols=s.ncols)
+    x = ax.uniform(1, m.ncols)
+    ref = orc.ncsr(m, x, nthreads=NT)
+    for fmt, kw in ((ax.FMT_CSR, dict(kernel=ax.KER_STREAM)), (ax.FMT_AXC, dict(hbs=4))):
+        y = ax.spmv_host(ax.Plan(m, fmt, **kw), x)
+        assert rel_err(ref, y) <= TOL, kw
+        assert rel_err(ref[-8:], y[-8:]) <= TOL, kw
+
+
+@pytest.mark.timeout(300)
 @pytest.mark.parametrize("fmt,kw", [(ax.FMT_AXT, dict(thw=4)), (ax.FMT_RCT, dict()),
                                     (ax.FMT_AXT, dict(thw=4, flags=ax.PLAN_REORDER)),
                                     (ax.FMT_RCT, dict(flags=ax.PLAN_REORDER))],
