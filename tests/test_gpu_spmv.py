@@ -153,9 +153,16 @@ def test_powerlaw_config2_full_size():
                     (ax.FMT_AXT, dict(thw=2, slices=4)), (ax.FMT_AXT, dict(thw=4, slices=4)),
                     (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=256)),
                     (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4)), (ax.FMT_K1, dict(chunk=64)),
-                    (ax.FMT_AXC, dict(hbs=4)), (ax.FMT_RCT, dict())):
+                    (ax.FMT_AXC, dict(hbs=4)), (ax.FMT_RCT, dict()),
+                    (ax.FMT_AXT, dict(thw=2, slices=4, flags=ax.PLAN_DETERMINISTIC))):
         y = ax.spmv_host(ax.Plan(m, fmt, **kw), x)
         assert rel_err(ref, y) <= TOL, kw
+    # FP32 values: K1's and COM's streams still pass 128 MB (non-temporal loads)
+    ref32 = orc.ncsr_f32val(m, x, nthreads=int(os.environ.get("OMP_NUM_THREADS", "8")))
+    for fmt, kw in ((ax.FMT_K1, dict(chunk=64)), (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=1, bs=256)),
+                    (ax.FMT_AXT, dict(mode=ax.AXT_COM, thw=64, th=4))):
+        y = ax.spmv_host(ax.Plan(m, fmt, vtype=ax.F32, **kw), x)
+        assert rel_err(ref32, y) <= TOL, kw
 
 
 @pytest.mark.parametrize("gen,slices", [
