@@ -372,7 +372,9 @@ bool narrow_bands(const axspmv_csr *a, uint32_t rows, uint64_t cols);
 // 128^3 stencil th = 1 806.7 -> 784.5 us (slices 452.5 -> 412.0), th = 4 /
 // 8 805.4 -> 798.4 / 822.8 -> 813.0 us; the stencil renumbered by reverse
 // Cuthill-McKee, whose rows read narrow bands, lost with them (th = 1:
-// 197.3 -> 204.2 us).
+// 197.3 -> 204.2 us).  A 128 MB bound (K1's) would take in the power-law
+// matrix with FP32 values (188 MB): th = 1 196.7 -> 192.5 us, but its
+// 4-slice plan 123.0 -> 146.0 us.
 bool wide_gathers(const axspmv_csr *a) {
     const uint32_t rows = uint32_t(std::min<uint64_t>(16384, ceil_div(ceil_div(uint64_t(a->nrows), 256), 64) * 64));
     return !narrow_bands(a, std::max<uint32_t>(64, rows), (2ull << 20) / 8);
